@@ -1,0 +1,114 @@
+"""ctypes binding of libgx_fluid.so (the C-ABI declared in include/gx_fluid.h).
+
+The library is loaded from genasis_amd/lib; a missing library raises -- there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgx_fluid.so")
+
+GX_OK = 0
+GX_BC_PERIODIC, GX_BC_OUTFLOW, GX_BC_REFLECTING = 0, 1, 2
+GX_HLL, GX_HLLC = 0, 1
+GX_FP64, GX_FP32 = 0, 1
+GX_N_FIELDS = 23
+
+FIELD_NAMES = (
+    "FastEigenspeedPlus_1", "FastEigenspeedPlus_2", "FastEigenspeedPlus_3",
+    "FastEigenspeedMinus_1", "FastEigenspeedMinus_2", "FastEigenspeedMinus_3",
+    "ComovingBaryonDensity", "ConservedBaryonDensity", "BaryonMass",
+    "Velocity_1", "Velocity_2", "Velocity_3",
+    "MomentumDensity_1", "MomentumDensity_2", "MomentumDensity_3",
+    "InternalEnergy", "ConservedEnergy", "Pressure", "Temperature",
+    "EntropyPerBaryon", "ConservedEntropy", "SoundSpeed", "MachNumber",
+)
+
+# Every symbol include/gx_fluid.h declares.
+EXPORTS = (
+    "gx_default_params", "gx_create", "gx_destroy", "gx_upload", "gx_download",
+    "gx_upload_features", "gx_download_features", "gx_compute_time_step",
+    "gx_step_rk2", "gx_sync", "gx_totals", "gx_set_state", "gx_set_primitive",
+    "gx_get_conserved", "gx_get_primitive", "gx_get_flags", "gx_set_flags",
+    "gx_set_tuning", "gx_stream", "gx_last_error",
+)
+
+
+class GxGrid(ctypes.Structure):
+    _fields_ = [
+        ("nCells", ctypes.c_int * 3),
+        ("MinCoordinate", ctypes.c_double * 3),
+        ("MaxCoordinate", ctypes.c_double * 3),
+        ("BoundaryCondition", ctypes.c_int * 6),
+    ]
+
+
+class GxParams(ctypes.Structure):
+    _fields_ = [
+        ("AdiabaticIndex", ctypes.c_double),
+        ("LimiterParameter", ctypes.c_double),
+        ("ShockThreshold", ctypes.c_double),
+        ("CourantFactor", ctypes.c_double),
+        ("BaryonDensityMin", ctypes.c_double),
+        ("BaryonMassReference", ctypes.c_double),
+        ("UseLimiter", ctypes.c_int),
+        ("RiemannSolverType", ctypes.c_int),
+        ("Precision", ctypes.c_int),
+    ]
+
+
+class GxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (once) and declare the signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GxError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    vp, d, i, ll = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_longlong
+    pd = P(ctypes.c_double)
+    sig = {
+        "gx_default_params": (None, [P(GxParams)]),
+        "gx_create": (i, [P(GxGrid), P(GxParams), P(vp)]),
+        "gx_destroy": (i, [vp]),
+        "gx_upload": (i, [vp, pd, ll, P(ctypes.c_int)]),
+        "gx_download": (i, [vp, pd, ll, P(ctypes.c_int)]),
+        "gx_upload_features": (i, [vp, pd, ll]),
+        "gx_download_features": (i, [vp, pd, ll]),
+        "gx_compute_time_step": (i, [vp, pd]),
+        "gx_step_rk2": (i, [vp, d, d]),
+        "gx_sync": (i, [vp]),
+        "gx_totals": (i, [vp, pd]),
+        "gx_set_state": (i, [vp, pd, pd]),
+        "gx_set_primitive": (i, [vp, pd]),
+        "gx_get_conserved": (i, [vp, pd]),
+        "gx_get_primitive": (i, [vp, pd]),
+        "gx_get_flags": (i, [vp, P(ctypes.c_ubyte)]),
+        "gx_set_flags": (i, [vp, P(ctypes.c_ubyte)]),
+        "gx_set_tuning": (i, [vp, i]),
+        "gx_stream": (vp, [vp]),
+        "gx_last_error": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != GX_OK:
+        msg = _lib.gx_last_error().decode() if _lib is not None else ""
+        raise GxError(f"{what} failed ({rc}): {msg}")

@@ -1,0 +1,1069 @@
+// gx_fluid.hip -- MI355X (gfx950) finite-volume step for the GenASiS
+// polytropic ideal-gas fluid, behind the C-ABI of include/gx_fluid.h.
+//
+// Device layout: the proper cells only (no ghost storage), five conserved
+// fields D, S_1, S_2, S_3, G as separate FP64 slabs of n0*n1*n2 values, x
+// fastest.  Ghost values are produced on load by index mapping: periodic wrap,
+// or the mirror copy of OUTFLOW / REFLECTING (CH/SingleLevelCharts/
+// Chart_SL__Template.f90:486-538, Chart_SL__Kernel.f90:12-113).  This equals
+// the reference's ghost handling for every value its sweeps read: the swept
+// dimension's ghosts over the transverse interior.
+//
+// One RK2 step = two launches of the fused stage kernel (reconstruction,
+// fluxes and divergence of all three dimensions, plus the RK combination)
+// and two small launches that build the shock flags of the next step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/gx_fluid.h"
+#include "gx_kernels.hpp"
+
+namespace gx {
+
+constexpr int TX = 32;           // tile width in x (cells)
+constexpr int TY = 8;            // tile height in y
+constexpr int NT = TX * TY;      // threads per block
+constexpr int HX = TX + 4;       // plane tile with two ghost layers
+constexpr int HY = TY + 4;
+constexpr int SXW = TX + 2;      // x-slope row length
+constexpr int FXW = TX + 1;      // x-flux row length
+constexpr int WAVE = 64;
+
+__host__ __device__ inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+struct Field5 {
+  const double *f[5];
+};
+struct Field5W {
+  double *f[5];
+};
+
+struct StageArgs {
+  int n[3];
+  int bc[6];
+  int tz;               // planes per block (z-chunk)
+  int src_is_prim;      // stage input holds primitives (first stage after upload)
+  double dt;
+  Physics ph;
+  Field5 src;           // stage input: conserved (or primitive) state
+  Field5 aux;           // stage 1: U (Solution); stage 2: W = U + K1/2
+  Field5W out1;         // stage 1: Y = U + K1; stage 2: U_new
+  Field5W out2;         // stage 1: W
+  const uint8_t *flags; // DiffusiveFlux bits of the step
+  unsigned long long *invdt_bits;  // stage 2: max CFL term (as bits)
+  Geo1D geo[3];
+};
+
+__device__ __forceinline__ long long lin(const int n[3], int x, int y, int z) {
+  return (long long)x + (long long)n[0] * ((long long)y + (long long)n[1] * z);
+}
+
+// Ghost index rule of one dimension: periodic wrap, or the mirror copy of
+// OUTFLOW / REFLECTING; *flip is set for a reflecting face.
+__device__ __forceinline__ int map_coord(int c, int n, int bc_in, int bc_out,
+                                         bool *flip) {
+  if (c < 0) {
+    if (bc_in == BC_PERIODIC) return c + n;
+    *flip = bc_in == BC_REFLECTING;
+    return -1 - c;
+  }
+  if (c >= n) {
+    if (bc_out == BC_PERIODIC) return c - n;
+    *flip = bc_out == BC_REFLECTING;
+    return 2 * n - 1 - c;
+  }
+  return c;
+}
+
+// Primitive state of cell (x, y, z), at most one coordinate outside the chart
+// by at most two cells; ghost values by the boundary rule of that face.
+__device__ __forceinline__ void load_prim(const StageArgs &a, int x, int y,
+                                          int z, double V[5]) {
+  bool fx = false, fy = false, fz = false;
+  const int cx = map_coord(x, a.n[0], a.bc[0], a.bc[1], &fx);
+  const int cy = map_coord(y, a.n[1], a.bc[2], a.bc[3], &fy);
+  const int cz = map_coord(z, a.n[2], a.bc[4], a.bc[5], &fz);
+  const long long i =
+      (long long)cx + (long long)a.n[0] * ((long long)cy + (long long)a.n[1] * cz);
+  if (a.src_is_prim) {
+#pragma unroll
+    for (int f = 0; f < 5; f++) V[f] = a.src.f[f][i];
+  } else {
+    double U[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) U[f] = a.src.f[f][i];
+    prim_from_cons(U, a.ph.mass, a.ph.nmin, V);
+  }
+  if (fx) V[1] = -V[1];
+  if (fy) V[2] = -V[2];
+  if (fz) V[3] = -V[3];
+}
+
+__device__ __forceinline__ double slope_at(const Geo1D &g, int c, double vm,
+                                           double v0, double vp,
+                                           const Physics &ph) {
+  const int p = c + 2;
+  return slope(vm, v0, vp, g.dxl[p], g.dxr[p], g.dxl2[p], g.dxr2[p], g.den[p],
+               ph.theta, ph.use_limiter);
+}
+
+__device__ __forceinline__ bool flag_bit(const StageArgs &a, int x, int y,
+                                         int z, int d) {
+  const int c[3] = {x, y, z};
+  if (c[d] >= a.n[d]) return false;  // ghost face: cleared in the reference
+  return (a.flags[lin(a.n, x, y, z)] >> d) & 1;
+}
+
+// Fused stage: K = dt * (-div F) over the three dimensions, then the RK
+// combination.  A block owns a TX x TY column tile and marches along z over
+// a.tz planes; x/y stencils come from the plane tile in LDS, the z stencil
+// from registers.
+template <int STAGE>
+__global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
+  __shared__ double sP[5][HY * HX];
+  __shared__ double sSx[5][TY * SXW];       // x-slopes
+  __shared__ double sSy[5][(TY + 2) * TX];  // y-slopes
+  __shared__ double sFx[5][TY * FXW];       // x-face fluxes
+  __shared__ double sFy[5][(TY + 1) * TX];  // y-face fluxes
+  __shared__ double sRed[NT / WAVE];
+
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * a.tz;
+  const int tw = min(TX, nx - x0), th = min(TY, ny - y0);
+  const int zend = min(z0 + a.tz, nz);
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const bool active = tx < tw && ty < th;
+  const int x = x0 + tx, y = y0 + ty;
+  const Physics ph = a.ph;
+  const Geo1D &gx = a.geo[0], &gy = a.geo[1], &gz = a.geo[2];
+
+  double Vk[5], Vk1[5], sk[5], Flo[5];
+  double tmax = 0.0;
+
+  if (active) {
+    double Vm2[5], Vm1[5], sm1[5], WL[5], WR[5];
+    load_prim(a, x, y, z0 - 2, Vm2);
+    load_prim(a, x, y, z0 - 1, Vm1);
+    load_prim(a, x, y, z0, Vk);
+    load_prim(a, x, y, z0 + 1, Vk1);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      sm1[f] = slope_at(gz, z0 - 1, Vm2[f], Vm1[f], Vk[f], ph);
+      sk[f] = slope_at(gz, z0, Vm1[f], Vk[f], Vk1[f], ph);
+      WL[f] = Vm1[f] + gz.wr[z0 - 1 + 2] * sm1[f];
+      WR[f] = Vk[f] - gz.wl[z0 + 2] * sk[f];
+    }
+    face_flux(WL, WR, 2, flag_bit(a, x, y, z0, 2), ph, Flo);
+  }
+
+  const int nsx = round_up((tw + 2) * th, WAVE);  // x-slope tasks, padded
+  const int nsy = tw * (th + 2);
+  const int nfx = round_up((tw + 1) * th, WAVE);  // x-face tasks, padded
+  const int nfy = tw * (th + 1);
+  const int nhalo = 4 * th + 4 * tw;
+
+  for (int k = z0; k < zend; ++k) {
+    double Fhi[5], sk1[5], Vk2[5];
+    if (active) {
+      double WL[5], WR[5];
+      load_prim(a, x, y, k + 2, Vk2);
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        sk1[f] = slope_at(gz, k + 1, Vk[f], Vk1[f], Vk2[f], ph);
+        WL[f] = Vk[f] + gz.wr[k + 2] * sk[f];
+        WR[f] = Vk1[f] - gz.wl[k + 1 + 2] * sk1[f];
+      }
+      face_flux(WL, WR, 2, flag_bit(a, x, y, k + 1, 2), ph, Fhi);
+#pragma unroll
+      for (int f = 0; f < 5; f++) sP[f][(ty + 2) * HX + tx + 2] = Vk[f];
+    }
+    // Ghost rows and columns of the plane tile.
+    for (int t = threadIdx.x; t < nhalo; t += NT) {
+      int hx, hy;
+      if (t < 4 * th) {
+        const int j = t >> 2, s = t & 3;
+        hy = y0 + j;
+        hx = s < 2 ? x0 - 2 + s : x0 + tw + (s - 2);
+      } else {
+        const int u = t - 4 * th;
+        const int i = u % tw, s = u / tw;
+        hx = x0 + i;
+        hy = s < 2 ? y0 - 2 + s : y0 + th + (s - 2);
+      }
+      double V[5];
+      load_prim(a, hx, hy, k, V);
+#pragma unroll
+      for (int f = 0; f < 5; f++) sP[f][(hy - y0 + 2) * HX + (hx - x0 + 2)] = V[f];
+    }
+    __syncthreads();
+
+    // Limited slopes: x for cells x0-1 .. x0+tw, y for cells y0-1 .. y0+th.
+    for (int t = threadIdx.x; t < nsx + nsy; t += NT) {
+      int base, stride, c, out;
+      const Geo1D *g;
+      const bool isx = t < nsx;
+      if (isx) {
+        if (t >= (tw + 2) * th) continue;
+        const int i = t % (tw + 2), j = t / (tw + 2);
+        base = (j + 2) * HX + (i + 1);
+        stride = 1;
+        c = x0 - 1 + i;
+        g = &gx;
+        out = j * SXW + i;
+      } else {
+        const int u = t - nsx;
+        const int i = u % tw, j = u / tw;
+        base = (j + 1) * HX + (i + 2);
+        stride = HX;
+        c = y0 - 1 + j;
+        g = &gy;
+        out = j * TX + i;
+      }
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        const double s = slope_at(*g, c, sP[f][base - stride], sP[f][base],
+                                  sP[f][base + stride], ph);
+        if (isx)
+          sSx[f][out] = s;
+        else
+          sSy[f][out] = s;
+      }
+    }
+    __syncthreads();
+
+    // Face fluxes: x faces x0 .. x0+tw, y faces y0 .. y0+th.  The x-task
+    // count is padded to whole waves so that every wave runs one direction.
+#pragma unroll 1
+    for (int t = threadIdx.x; t < nfx + nfy; t += NT) {
+      double WL[5], WR[5], F[5];
+      if (t < nfx) {
+        if (t >= (tw + 1) * th) continue;
+        const int i = t % (tw + 1), j = t / (tw + 1);
+        const int bl = (j + 2) * HX + (i + 1);
+        const double wr = gx.wr[x0 + i - 1 + 2], wl = gx.wl[x0 + i + 2];
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          WL[f] = sP[f][bl] + wr * sSx[f][j * SXW + i];
+          WR[f] = sP[f][bl + 1] - wl * sSx[f][j * SXW + i + 1];
+        }
+        face_flux(WL, WR, 0, flag_bit(a, x0 + i, y0 + j, k, 0), ph, F);
+#pragma unroll
+        for (int f = 0; f < 5; f++) sFx[f][j * FXW + i] = F[f];
+      } else {
+        const int u = t - nfx;
+        const int i = u % tw, j = u / tw;
+        const int bl = (j + 1) * HX + (i + 2);
+        const double wr = gy.wr[y0 + j - 1 + 2], wl = gy.wl[y0 + j + 2];
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          WL[f] = sP[f][bl] + wr * sSy[f][j * TX + i];
+          WR[f] = sP[f][bl + HX] - wl * sSy[f][(j + 1) * TX + i];
+        }
+        face_flux(WL, WR, 1, flag_bit(a, x0 + i, y0 + j, k, 1), ph, F);
+#pragma unroll
+        for (int f = 0; f < 5; f++) sFy[f][j * TX + i] = F[f];
+      }
+    }
+    __syncthreads();
+
+    if (active) {
+      // ComputeIncrement_CSL_Kernel (EV/Increments/IncrementDivergence_FV__
+      // Kernel.f90:672-752), dimensions in order 1, 2, 3, onto K = 0.
+      const double dX = gx.width[x + 2], dY = gy.width[y + 2], dZ = gz.width[k + 2];
+      const double Vol = dX * dY * dZ;
+      const double A1 = dY * dZ, A2 = dZ * dX, A3 = dX * dY;
+      const double dt = a.dt;
+      const long long ic = lin(a.n, x, y, k);
+      double Kc[5];
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        const double fxl = sFx[f][ty * FXW + tx], fxh = sFx[f][ty * FXW + tx + 1];
+        const double fyl = sFy[f][ty * TX + tx], fyh = sFy[f][(ty + 1) * TX + tx];
+        double K = 0.0;
+        K = K + dt * (A1 * fxl - A1 * fxh) / Vol;
+        K = K + dt * (A2 * fyl - A2 * fyh) / Vol;
+        K = K + dt * (A3 * Flo[f] - A3 * Fhi[f]) / Vol;
+        Kc[f] = K;
+      }
+      if (STAGE == 1) {
+        // Y = Solution + 1*K1; Solution + 1/2 K1 kept for the final combination
+        // (Step_RK__Template.f90:240-278, MultiplyAdd_Command.f90:56-94).
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          const double u = a.aux.f[f][ic];
+          a.out1.f[f][ic] = u + 1.0 * Kc[f];
+          a.out2.f[f][ic] = u + 0.5 * Kc[f];
+        }
+      } else {
+        // Solution = (Solution + 1/2 K1) + 1/2 K2, StoreSolution's conserved
+        // fix, and the CFL term of the next step.
+        double U[5], V[5];
+#pragma unroll
+        for (int f = 0; f < 5; f++) U[f] = a.aux.f[f][ic] + 0.5 * Kc[f];
+        fix_conserved(U, ph.mass, ph.nmin, V);
+#pragma unroll
+        for (int f = 0; f < 5; f++) a.out1.f[f][ic] = U[f];
+        const double tc = cfl_term(V, ph.gamma, ph.mass, dX, dY, dZ);
+        tmax = tc > tmax ? tc : tmax;
+      }
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        Flo[f] = Fhi[f];
+        sk[f] = sk1[f];
+        Vk[f] = Vk1[f];
+        Vk1[f] = Vk2[f];
+      }
+    }
+  }
+
+  if (STAGE == 2) {
+    // Block max of the non-negative CFL terms; IEEE order of non-negative
+    // doubles is the order of their bit patterns.
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+      const double v = __shfl_xor(tmax, o, WAVE);
+      tmax = v > tmax ? v : tmax;
+    }
+    if ((threadIdx.x % WAVE) == 0) sRed[threadIdx.x / WAVE] = tmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = 0.0;
+      for (int w = 0; w < NT / WAVE; w++) m = sRed[w] > m ? sRed[w] : m;
+      atomicMax(a.invdt_bits, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+// Max CFL term over the current state (the first step after an upload, or
+// whenever the stage-2 value is not available).
+__global__ __launch_bounds__(256) void cfl_kernel(StageArgs a) {
+  __shared__ double sRed[256 / WAVE];
+  const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
+  double tmax = 0.0;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    const int x = (int)(i % a.n[0]);
+    const int y = (int)((i / a.n[0]) % a.n[1]);
+    const int z = (int)(i / ((long long)a.n[0] * a.n[1]));
+    double V[5];
+    load_prim(a, x, y, z, V);
+    const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, a.geo[0].width[x + 2],
+                               a.geo[1].width[y + 2], a.geo[2].width[z + 2]);
+    tmax = tc > tmax ? tc : tmax;
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    const double v = __shfl_xor(tmax, o, WAVE);
+    tmax = v > tmax ? v : tmax;
+  }
+  if ((threadIdx.x % WAVE) == 0) sRed[threadIdx.x / WAVE] = tmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < 256 / WAVE; w++) m = sRed[w] > m ? sRed[w] : m;
+    atomicMax(a.invdt_bits, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// Shock test of DetectShocks_CSL_Kernel (F/FluidFeatures_P__Kernel.f90:
+// 12-186) between each cell and its lower neighbour in every dimension, on
+// the stored state: bit d set when |dP|/max(min(P), sqrt(tiny)) > threshold
+// and dV_d <= 0.  Cells on the lower face of a dimension get no bit there
+// (no flag that survives the boundary clearing reads it).
+__global__ __launch_bounds__(256) void shock_bits_kernel(StageArgs a,
+                                                         uint8_t *bits) {
+  const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
+  const double sqrt_tiny = 1.4916681462400413e-154;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    const int c[3] = {(int)(i % a.n[0]), (int)((i / a.n[0]) % a.n[1]),
+                      (int)(i / ((long long)a.n[0] * a.n[1]))};
+    double V[5];
+    load_prim(a, c[0], c[1], c[2], V);
+    const double P = (a.ph.gamma - 1.0) * V[4];
+    uint8_t b = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (c[d] == 0) continue;
+      double W[5];
+      load_prim(a, c[0] - (d == 0), c[1] - (d == 1), c[2] - (d == 2), W);
+      const double Pm = (a.ph.gamma - 1.0) * W[4];
+      const double dP = fabs(P - Pm);
+      const double P_Min = fmax(fmin(P, Pm), sqrt_tiny);
+      const double dLnP = dP / P_Min;
+      const double dV = V[1 + d] - W[1 + d];
+      if (dLnP > a.ph.st && dV <= 0.0) b |= (uint8_t)(1u << d);
+    }
+    bits[i] = b;
+  }
+}
+
+// DiffusiveFlux_I flags from the shock bits: a shock in dimension i marks the
+// transverse faces on both sides of the two cells it separates
+// (F/FluidFeatures_P__Kernel.f90:137-147); ClearBoundary_CSL_Kernel
+// (:189-313) then clears the first and last two layers of every dimension, so
+// flags live on cells 1 .. n-2 only.
+__global__ __launch_bounds__(256) void flags_kernel(const int3 n,
+                                                    const uint8_t *bits,
+                                                    uint8_t *flags) {
+  const long long ntot = (long long)n.x * n.y * n.z;
+  const long long s[3] = {1, n.x, (long long)n.x * n.y};
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    const int c[3] = {(int)(i % n.x), (int)((i / n.x) % n.y),
+                      (int)(i / ((long long)n.x * n.y))};
+    const int nn[3] = {n.x, n.y, n.z};
+    uint8_t out = 0;
+    bool inside = true;
+#pragma unroll
+    for (int e = 0; e < 3; e++) inside = inside && c[e] >= 1 && c[e] <= nn[e] - 2;
+    if (inside) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        bool f = false;
+#pragma unroll
+        for (int id = 0; id < 3; id++) {
+          if (id == d) continue;
+          const uint8_t m = (uint8_t)(1u << id);
+          f = f || (bits[i] & m) || (bits[i + s[id]] & m) ||
+              (bits[i - s[d]] & m) || (bits[i + s[id] - s[d]] & m);
+        }
+        if (f) out |= (uint8_t)(1u << d);
+      }
+    }
+    flags[i] = out;
+  }
+}
+
+// Current fields of ComputeFromConservedCommon for download
+// (F/Fluid_P_I__Form.f90:604-722), proper cells, on the stored state.
+__global__ __launch_bounds__(256) void current_kernel(StageArgs a, double c_v,
+                                                      double *out) {
+  const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
+  const double sqrt_huge = 1.3407807929942596e+154;  // sqrt(DBL_MAX)
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    double U[5], V[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) U[f] = a.aux.f[f][i];
+    fix_conserved(U, a.ph.mass, a.ph.nmin, V);
+    double *o = out;
+    const double g = a.ph.gamma, M = a.ph.mass;
+    const double N = V[0], E = V[4];
+    const double P = (g - 1.0) * E;
+    double T, SB, CS;
+    if (N > 0.0 && P > 0.0) {
+      T = E / (c_v * N);
+      SB = c_v * log(P / 1.0 * pow(1.0 / N, g));
+      CS = sqrt(g * P / (M * N));
+    } else {
+      T = 0.0;
+      SB = -sqrt_huge;
+      CS = 0.0;
+    }
+    double MN = 0.0;
+    if (CS > 0.0) MN = sqrt((V[1] * V[1] + V[2] * V[2]) + V[3] * V[3]) / CS;
+    const double vals[GX_N_FIELDS] = {
+        V[1] + CS, V[2] + CS, V[3] + CS, V[1] - CS, V[2] - CS, V[3] - CS,
+        N,         U[0],      M,         V[1],      V[2],      V[3],
+        U[1],      U[2],      U[3],      E,         U[4],      P,
+        T,         SB,        SB * N,    CS,        MN};
+#pragma unroll
+    for (int f = 0; f < GX_N_FIELDS; f++) o[(long long)f * ntot + i] = vals[f];
+  }
+}
+
+// ComputeFromInitial = ComputeFromPrimitiveCommon on the Current
+// (F/Fluid_P_I__Form.f90:509-601): the density floor N<0 -> 0 is written back
+// to the primitive field, then D, S, G.
+__global__ __launch_bounds__(256) void from_primitive_kernel(long long ntot,
+                                                             double mass,
+                                                             Field5W P,
+                                                             Field5W U) {
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    double N = P.f[0][i];
+    if (N < 0.0) {
+      N = 0.0;
+      P.f[0][i] = N;
+    }
+    const double V1 = P.f[1][i], V2 = P.f[2][i], V3 = P.f[3][i];
+    const double S1 = mass * N * V1, S2 = mass * N * V2, S3 = mass * N * V3;
+    U.f[0][i] = N;
+    U.f[1][i] = S1;
+    U.f[2][i] = S2;
+    U.f[3][i] = S3;
+    U.f[4][i] = P.f[4][i] + 0.5 * ((S1 * V1 + S2 * V2) + S3 * V3);
+  }
+}
+
+// Per-block partial sums of U * volume over proper cells.
+__global__ __launch_bounds__(256) void totals_kernel(StageArgs a,
+                                                     double *partial) {
+  __shared__ double s[5][256];
+  const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    const int x = (int)(i % a.n[0]);
+    const int y = (int)((i / a.n[0]) % a.n[1]);
+    const int z = (int)(i / ((long long)a.n[0] * a.n[1]));
+    const double vol = a.geo[0].width[x + 2] * a.geo[1].width[y + 2] *
+                       a.geo[2].width[z + 2];
+#pragma unroll
+    for (int f = 0; f < 5; f++) acc[f] += a.aux.f[f][i] * vol;
+  }
+#pragma unroll
+  for (int f = 0; f < 5; f++) s[f][threadIdx.x] = acc[f];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int f = 0; f < 5; f++) s[f][threadIdx.x] += s[f][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) partial[blockIdx.x * 5 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+}  // namespace gx
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+using namespace gx;
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                       \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(GX_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct gx_ctx {
+  gx_grid grid;
+  gx_params params;
+  int n[3];
+  long long ncell;
+  hipStream_t stream = nullptr;
+  int tz = 16;
+  // device buffers
+  double *geo_buf = nullptr;  // 8 arrays x 3 dimensions x (nmax+4)
+  Geo1D geo[3];
+  double *U[5] = {}, *Y[5] = {}, *W[5] = {}, *P0[5] = {};
+  uint8_t *flags = nullptr, *bits = nullptr;
+  unsigned long long *invdt = nullptr;
+  double *partial = nullptr;
+  int n_partial = 0;
+  double *scratch = nullptr;  // download staging: GX_N_FIELDS x ncell
+  bool uploaded = false;
+  bool prims_valid = false;   // next stage 1 reads P0
+  bool invdt_valid = false;   // invdt holds the max for the current state
+  std::vector<double> geo_host[3][8];
+};
+
+static void geometry_1d(int n, double lo, double hi, std::vector<double> out[8]) {
+  // Same construction as the reference chart (Chart_Template.f90:463-593),
+  // including single-precision ghost widths (real(KDL) at :477-478).
+  const int g = 2;
+  std::vector<double> e(n + 2 * g + 1);
+  auto E = [&](int i) -> double & { return e[i + g - 1]; };
+  E(1) = lo;
+  const double width = (hi - lo) / n;
+  for (int ic = 2; ic <= n + 1; ic++) E(ic) = E(ic - 1) + width;
+  for (int ic = 1; ic <= g; ic++) {
+    const float w_ig = (float)(E(ic + 1) - E(ic));
+    const float w_og = (float)(E(n - ic + 2) - E(n - ic + 1));
+    E(1 - ic) = E(2 - ic) - (double)w_ig;
+    E(n + 1 + ic) = E(n + ic) + (double)w_og;
+  }
+  const int m = n + 2 * g;
+  std::vector<double> c(m);
+  for (int a = 0; a < 8; a++) out[a].assign(m, 0.0);
+  for (int p = 0; p < m; p++) {
+    const int ic = p - g + 1;
+    c[p] = 0.5 * (E(ic) + E(ic + 1));
+    out[0][p] = c[p] - E(ic);
+    out[1][p] = E(ic + 1) - c[p];
+  }
+  for (int p = 0; p < m; p++) {
+    if (p >= 1 && p + 1 < m) {
+      const double dxl = c[p] - c[p - 1];
+      const double dxr = c[p + 1] - c[p];
+      out[2][p] = dxl;
+      out[3][p] = dxr;
+      out[4][p] = dxl * dxl;
+      out[5][p] = dxr * dxr;
+      out[6][p] = dxl * dxr * (dxl + dxr);
+    } else {
+      out[2][p] = out[3][p] = out[4][p] = out[5][p] = 1.0;
+      out[6][p] = 1.0;
+    }
+    out[7][p] = out[0][p] + out[1][p];
+  }
+}
+
+static StageArgs base_args(const gx_ctx *c) {
+  StageArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int d = 0; d < 3; d++) a.n[d] = c->n[d];
+  for (int i = 0; i < 6; i++) a.bc[i] = c->grid.BoundaryCondition[i];
+  a.tz = c->tz;
+  a.ph.gamma = c->params.AdiabaticIndex;
+  a.ph.theta = c->params.LimiterParameter;
+  a.ph.mass = c->params.BaryonMassReference;
+  a.ph.nmin = c->params.BaryonDensityMin;
+  a.ph.st = c->params.ShockThreshold;
+  a.ph.use_limiter = c->params.UseLimiter;
+  a.ph.hllc = c->params.RiemannSolverType == GX_HLLC;
+  for (int d = 0; d < 3; d++) a.geo[d] = c->geo[d];
+  a.flags = c->flags;
+  a.invdt_bits = c->invdt;
+  return a;
+}
+
+static int grid_blocks(long long ncell) {
+  long long b = (ncell + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+extern "C" {
+
+void gx_default_params(gx_params *p) {
+  if (!p) return;
+  p->AdiabaticIndex = 1.4;
+  p->LimiterParameter = 1.4;
+  p->ShockThreshold = 0.1;
+  p->CourantFactor = 0.7;
+  p->BaryonDensityMin = 0.0;
+  p->BaryonMassReference = 1.0;
+  p->UseLimiter = 1;
+  p->RiemannSolverType = GX_HLLC;
+  p->Precision = GX_FP64;
+}
+
+const char *gx_last_error(void) { return g_last_error.c_str(); }
+
+int gx_destroy(gx_ctx *c) {
+  if (!c) return GX_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->geo_buf);
+  for (int f = 0; f < 5; f++) {
+    (void)hipFree(c->U[f]);
+    (void)hipFree(c->Y[f]);
+    (void)hipFree(c->W[f]);
+    (void)hipFree(c->P0[f]);
+  }
+  (void)hipFree(c->flags);
+  (void)hipFree(c->bits);
+  (void)hipFree(c->invdt);
+  (void)hipFree(c->partial);
+  (void)hipFree(c->scratch);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GX_OK;
+}
+
+int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
+  if (!grid || !out) return fail(GX_E_ARG, "gx_create: null argument");
+  *out = nullptr;
+  gx_params p;
+  if (params)
+    p = *params;
+  else
+    gx_default_params(&p);
+  if (p.Precision != GX_FP64)
+    return fail(GX_E_ARG, "gx_create: only Precision = GX_FP64 is built");
+  if (p.RiemannSolverType != GX_HLL && p.RiemannSolverType != GX_HLLC)
+    return fail(GX_E_ARG, "gx_create: RiemannSolverType must be GX_HLL or GX_HLLC");
+  for (int d = 0; d < 3; d++) {
+    if (grid->nCells[d] < 2)
+      return fail(GX_E_ARG, "gx_create: nCells must be >= 2 in every dimension");
+    const int bi = grid->BoundaryCondition[2 * d], bo = grid->BoundaryCondition[2 * d + 1];
+    if (bi < 0 || bi > 2 || bo < 0 || bo > 2)
+      return fail(GX_E_ARG, "gx_create: unknown boundary condition");
+    if ((bi == GX_BC_PERIODIC) != (bo == GX_BC_PERIODIC))
+      return fail(GX_E_ARG, "gx_create: PERIODIC must hold on both faces of a dimension");
+    if (!(grid->MaxCoordinate[d] > grid->MinCoordinate[d]))
+      return fail(GX_E_ARG, "gx_create: MaxCoordinate must exceed MinCoordinate");
+  }
+  gx_ctx *c = new (std::nothrow) gx_ctx();
+  if (!c) return fail(GX_E_NOMEM, "gx_create: host allocation failed");
+  c->grid = *grid;
+  c->params = p;
+  c->ncell = 1;
+  for (int d = 0; d < 3; d++) {
+    c->n[d] = grid->nCells[d];
+    c->ncell *= c->n[d];
+  }
+#define TRY_CLEAN(expr)                                                   \
+  do {                                                                    \
+    hipError_t e_ = (expr);                                               \
+    if (e_ != hipSuccess) {                                               \
+      std::string m_ = std::string(#expr ": ") + hipGetErrorString(e_);   \
+      gx_destroy(c);                                                      \
+      return fail(GX_E_HIP, m_);                                          \
+    }                                                                     \
+  } while (0)
+  TRY_CLEAN(hipStreamCreate(&c->stream));
+  size_t geo_len = 0;
+  for (int d = 0; d < 3; d++) {
+    geometry_1d(c->n[d], grid->MinCoordinate[d], grid->MaxCoordinate[d], c->geo_host[d]);
+    geo_len += 8 * (size_t)(c->n[d] + 4);
+  }
+  TRY_CLEAN(hipMalloc(&c->geo_buf, geo_len * sizeof(double)));
+  {
+    std::vector<double> host(geo_len);
+    size_t off = 0;
+    for (int d = 0; d < 3; d++) {
+      const double *ptrs[8];
+      for (int a = 0; a < 8; a++) {
+        memcpy(&host[off], c->geo_host[d][a].data(), sizeof(double) * (c->n[d] + 4));
+        ptrs[a] = c->geo_buf + off;
+        off += c->n[d] + 4;
+      }
+      c->geo[d] = Geo1D{ptrs[0], ptrs[1], ptrs[2], ptrs[3],
+                        ptrs[4], ptrs[5], ptrs[6], ptrs[7]};
+    }
+    TRY_CLEAN(hipMemcpy(c->geo_buf, host.data(), geo_len * sizeof(double),
+                        hipMemcpyHostToDevice));
+  }
+  const size_t bytes = sizeof(double) * (size_t)c->ncell;
+  for (int f = 0; f < 5; f++) {
+    TRY_CLEAN(hipMalloc(&c->U[f], bytes));
+    TRY_CLEAN(hipMalloc(&c->Y[f], bytes));
+    TRY_CLEAN(hipMalloc(&c->W[f], bytes));
+    TRY_CLEAN(hipMalloc(&c->P0[f], bytes));
+  }
+  TRY_CLEAN(hipMalloc(&c->flags, (size_t)c->ncell));
+  TRY_CLEAN(hipMalloc(&c->bits, (size_t)c->ncell));
+  TRY_CLEAN(hipMemset(c->flags, 0, (size_t)c->ncell));
+  TRY_CLEAN(hipMalloc(&c->invdt, sizeof(unsigned long long)));
+  c->n_partial = grid_blocks(c->ncell);
+  if (c->n_partial > 1024) c->n_partial = 1024;
+  TRY_CLEAN(hipMalloc(&c->partial, sizeof(double) * 5 * c->n_partial));
+#undef TRY_CLEAN
+  *out = c;
+  return GX_OK;
+}
+
+int gx_set_tuning(gx_ctx *c, int zChunk) {
+  if (!c || zChunk < 1) return fail(GX_E_ARG, "gx_set_tuning: bad argument");
+  c->tz = zChunk;
+  return GX_OK;
+}
+
+void *gx_stream(gx_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
+  if (!c || !primitive || !conserved) return fail(GX_E_ARG, "gx_set_state: null argument");
+  const size_t bytes = sizeof(double) * (size_t)c->ncell;
+  for (int f = 0; f < 5; f++) {
+    HIP_TRY(hipMemcpyAsync(c->P0[f], primitive + (size_t)f * c->ncell, bytes,
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->U[f], conserved + (size_t)f * c->ncell, bytes,
+                           hipMemcpyHostToDevice, c->stream));
+  }
+  HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->uploaded = true;
+  c->prims_valid = true;
+  c->invdt_valid = false;
+  return GX_OK;
+}
+
+int gx_set_primitive(gx_ctx *c, const double *primitive) {
+  if (!c || !primitive) return fail(GX_E_ARG, "gx_set_primitive: null argument");
+  const size_t bytes = sizeof(double) * (size_t)c->ncell;
+  Field5W P, U;
+  for (int f = 0; f < 5; f++) {
+    HIP_TRY(hipMemcpyAsync(c->P0[f], primitive + (size_t)f * c->ncell, bytes,
+                           hipMemcpyHostToDevice, c->stream));
+    P.f[f] = c->P0[f];
+    U.f[f] = c->U[f];
+  }
+  from_primitive_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
+      c->ncell, c->params.BaryonMassReference, P, U);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->uploaded = true;
+  c->prims_valid = true;
+  c->invdt_valid = false;
+  return GX_OK;
+}
+
+static const int PRIM_FIELDS[5] = {GX_COMOVING_BARYON_DENSITY, GX_VELOCITY_U_1,
+                                   GX_VELOCITY_U_2, GX_VELOCITY_U_3,
+                                   GX_INTERNAL_ENERGY};
+static const int CONS_FIELDS[5] = {GX_CONSERVED_BARYON_DENSITY,
+                                   GX_MOMENTUM_DENSITY_D_1,
+                                   GX_MOMENTUM_DENSITY_D_2,
+                                   GX_MOMENTUM_DENSITY_D_3,
+                                   GX_CONSERVED_ENERGY};
+
+int gx_upload(gx_ctx *c, const double *value, long long ld, const int *map) {
+  if (!c || !value || !map) return fail(GX_E_ARG, "gx_upload: null argument");
+  const int m0 = c->n[0] + 4, m1 = c->n[1] + 4, m2 = c->n[2] + 4;
+  if (ld < (long long)m0 * m1 * m2) return fail(GX_E_ARG, "gx_upload: ldValue < nValues");
+  for (int f = 0; f < 5; f++)
+    if (map[PRIM_FIELDS[f]] <= 0 || map[CONS_FIELDS[f]] <= 0)
+      return fail(GX_E_ARG, "gx_upload: primitive and conserved fields are required");
+  std::vector<double> prim(5 * (size_t)c->ncell), cons(5 * (size_t)c->ncell);
+  for (int f = 0; f < 5; f++) {
+    const double *vp = value + (size_t)(map[PRIM_FIELDS[f]] - 1) * ld;
+    const double *vc = value + (size_t)(map[CONS_FIELDS[f]] - 1) * ld;
+    for (int z = 0; z < c->n[2]; z++)
+      for (int y = 0; y < c->n[1]; y++)
+        for (int x = 0; x < c->n[0]; x++) {
+          const size_t src = (size_t)(x + 2) + (size_t)m0 * ((y + 2) + (size_t)m1 * (z + 2));
+          const size_t dst = (size_t)f * c->ncell + x + (size_t)c->n[0] * (y + (size_t)c->n[1] * z);
+          prim[dst] = vp[src];
+          cons[dst] = vc[src];
+        }
+  }
+  return gx_set_state(c, prim.data(), cons.data());
+}
+
+int gx_get_conserved(gx_ctx *c, double *conserved) {
+  if (!c || !conserved) return fail(GX_E_ARG, "gx_get_conserved: null argument");
+  if (!c->uploaded) return fail(GX_E_STATE, "gx_get_conserved: no state uploaded");
+  const size_t bytes = sizeof(double) * (size_t)c->ncell;
+  for (int f = 0; f < 5; f++)
+    HIP_TRY(hipMemcpyAsync(conserved + (size_t)f * c->ncell, c->U[f], bytes,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+static int run_current(gx_ctx *c) {
+  if (!c->scratch)
+    HIP_TRY(hipMalloc(&c->scratch, sizeof(double) * GX_N_FIELDS * (size_t)c->ncell));
+  StageArgs a = base_args(c);
+  for (int f = 0; f < 5; f++) a.aux.f[f] = c->U[f];
+  // c_v = k/(mu(gamma-1)) with the default gamma (Fluid_P_I__Form.f90:206-223)
+  const double c_v = 1.0 / (1.0 * (1.4 - 1.0));
+  current_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a, c_v, c->scratch);
+  HIP_TRY(hipGetLastError());
+  return GX_OK;
+}
+
+int gx_get_primitive(gx_ctx *c, double *primitive) {
+  if (!c || !primitive) return fail(GX_E_ARG, "gx_get_primitive: null argument");
+  if (!c->uploaded) return fail(GX_E_STATE, "gx_get_primitive: no state uploaded");
+  const size_t bytes = sizeof(double) * (size_t)c->ncell;
+  if (c->prims_valid) {
+    for (int f = 0; f < 5; f++)
+      HIP_TRY(hipMemcpyAsync(primitive + (size_t)f * c->ncell, c->P0[f], bytes,
+                             hipMemcpyDeviceToHost, c->stream));
+  } else {
+    int rc = run_current(c);
+    if (rc) return rc;
+    for (int f = 0; f < 5; f++)
+      HIP_TRY(hipMemcpyAsync(primitive + (size_t)f * c->ncell,
+                             c->scratch + (size_t)PRIM_FIELDS[f] * c->ncell, bytes,
+                             hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_download(gx_ctx *c, double *value, long long ld, const int *map) {
+  if (!c || !value || !map) return fail(GX_E_ARG, "gx_download: null argument");
+  if (!c->uploaded) return fail(GX_E_STATE, "gx_download: no state uploaded");
+  const int m0 = c->n[0] + 4, m1 = c->n[1] + 4, m2 = c->n[2] + 4;
+  if (ld < (long long)m0 * m1 * m2) return fail(GX_E_ARG, "gx_download: ldValue < nValues");
+  int rc = run_current(c);
+  if (rc) return rc;
+  std::vector<double> host(GX_N_FIELDS * (size_t)c->ncell);
+  HIP_TRY(hipMemcpyAsync(host.data(), c->scratch, sizeof(double) * host.size(),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->prims_valid) {
+    // The Current still holds the uploaded primitives.
+    std::vector<double> p(5 * (size_t)c->ncell);
+    for (int f = 0; f < 5; f++)
+      HIP_TRY(hipMemcpy(p.data() + (size_t)f * c->ncell, c->P0[f],
+                        sizeof(double) * c->ncell, hipMemcpyDeviceToHost));
+    for (int f = 0; f < 5; f++)
+      memcpy(host.data() + (size_t)PRIM_FIELDS[f] * c->ncell,
+             p.data() + (size_t)f * c->ncell, sizeof(double) * c->ncell);
+  }
+  const int g = 2;
+  const int n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
+  for (int f = 0; f < GX_N_FIELDS; f++) {
+    if (map[f] <= 0) continue;
+    double *col = value + (size_t)(map[f] - 1) * ld;
+    const double *src = host.data() + (size_t)f * c->ncell;
+    for (int pz = 0; pz < m2; pz++)
+      for (int py = 0; py < m1; py++)
+        for (int px = 0; px < m0; px++) {
+          int q[3] = {px - g, py - g, pz - g};
+          bool ok = true;
+          for (int d = 0; d < 3; d++) {
+            if (q[d] < 0 || q[d] >= c->n[d]) {
+              if (c->grid.BoundaryCondition[2 * d] != GX_BC_PERIODIC) ok = false;
+              q[d] = (q[d] + c->n[d]) % c->n[d];
+            }
+          }
+          if (!ok) continue;
+          col[(size_t)px + (size_t)m0 * (py + (size_t)m1 * pz)] =
+              src[(size_t)q[0] + (size_t)n0 * (q[1] + (size_t)n1 * q[2])];
+        }
+    (void)n2;
+  }
+  return GX_OK;
+}
+
+int gx_get_flags(gx_ctx *c, unsigned char *flags) {
+  if (!c || !flags) return fail(GX_E_ARG, "gx_get_flags: null argument");
+  HIP_TRY(hipMemcpyAsync(flags, c->flags, (size_t)c->ncell, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_set_flags(gx_ctx *c, const unsigned char *flags) {
+  if (!c || !flags) return fail(GX_E_ARG, "gx_set_flags: null argument");
+  HIP_TRY(hipMemcpyAsync(c->flags, flags, (size_t)c->ncell, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_upload_features(gx_ctx *c, const double *df, long long ld) {
+  if (!c || !df) return fail(GX_E_ARG, "gx_upload_features: null argument");
+  const int m0 = c->n[0] + 4, m1 = c->n[1] + 4;
+  std::vector<unsigned char> b((size_t)c->ncell, 0);
+  for (int d = 0; d < 3; d++)
+    for (int z = 0; z < c->n[2]; z++)
+      for (int y = 0; y < c->n[1]; y++)
+        for (int x = 0; x < c->n[0]; x++) {
+          const size_t src = (size_t)d * ld + (size_t)(x + 2) + (size_t)m0 * ((y + 2) + (size_t)m1 * (z + 2));
+          if (df[src] > 0.0)
+            b[x + (size_t)c->n[0] * (y + (size_t)c->n[1] * z)] |= (unsigned char)(1u << d);
+        }
+  return gx_set_flags(c, b.data());
+}
+
+int gx_download_features(gx_ctx *c, double *df, long long ld) {
+  if (!c || !df) return fail(GX_E_ARG, "gx_download_features: null argument");
+  const int m0 = c->n[0] + 4, m1 = c->n[1] + 4;
+  std::vector<unsigned char> b((size_t)c->ncell);
+  int rc = gx_get_flags(c, b.data());
+  if (rc) return rc;
+  for (int d = 0; d < 3; d++)
+    for (int z = 0; z < c->n[2]; z++)
+      for (int y = 0; y < c->n[1]; y++)
+        for (int x = 0; x < c->n[0]; x++) {
+          const size_t dst = (size_t)d * ld + (size_t)(x + 2) + (size_t)m0 * ((y + 2) + (size_t)m1 * (z + 2));
+          df[dst] = ((b[x + (size_t)c->n[0] * (y + (size_t)c->n[1] * z)] >> d) & 1) ? 1.0 : 0.0;
+        }
+  return GX_OK;
+}
+
+int gx_compute_time_step(gx_ctx *c, double *candidate) {
+  if (!c || !candidate) return fail(GX_E_ARG, "gx_compute_time_step: null argument");
+  if (!c->uploaded) return fail(GX_E_STATE, "gx_compute_time_step: no state uploaded");
+  if (!c->invdt_valid) {
+    HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
+    StageArgs a = base_args(c);
+    a.src_is_prim = c->prims_valid ? 1 : 0;
+    for (int f = 0; f < 5; f++) a.src.f[f] = c->prims_valid ? c->P0[f] : c->U[f];
+    cfl_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a);
+    HIP_TRY(hipGetLastError());
+  }
+  unsigned long long bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, c->invdt, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double invdt;
+  memcpy(&invdt, &bits, sizeof(invdt));
+  c->invdt_valid = true;
+  // ComputeTimeStepKernel_CSL tail and ComputeTimeStep_C_ASC
+  const double tsi = fmax(DBL_MIN, invdt);
+  const double ts = fmin(DBL_MAX, 1.0 / tsi);
+  *candidate = c->params.CourantFactor * ts;
+  return GX_OK;
+}
+
+int gx_step_rk2(gx_ctx *c, double time, double dt) {
+  (void)time;
+  if (!c) return fail(GX_E_ARG, "gx_step_rk2: null context");
+  if (!c->uploaded) return fail(GX_E_STATE, "gx_step_rk2: no state uploaded");
+  const dim3 block(NT);
+  const dim3 grid((c->n[0] + TX - 1) / TX, (c->n[1] + TY - 1) / TY,
+                  (c->n[2] + c->tz - 1) / c->tz);
+  // Stage 1: Current = uploaded primitives or the Solution.
+  StageArgs a1 = base_args(c);
+  a1.dt = dt;
+  a1.src_is_prim = c->prims_valid ? 1 : 0;
+  for (int f = 0; f < 5; f++) {
+    a1.src.f[f] = c->prims_valid ? c->P0[f] : c->U[f];
+    a1.aux.f[f] = c->U[f];
+    a1.out1.f[f] = c->Y[f];
+    a1.out2.f[f] = c->W[f];
+  }
+  stage_kernel<1><<<grid, block, 0, c->stream>>>(a1);
+  HIP_TRY(hipGetLastError());
+  // Stage 2: Current = Y.
+  HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
+  StageArgs a2 = base_args(c);
+  a2.dt = dt;
+  for (int f = 0; f < 5; f++) {
+    a2.src.f[f] = c->Y[f];
+    a2.aux.f[f] = c->W[f];
+    a2.out1.f[f] = c->U[f];
+  }
+  stage_kernel<2><<<grid, block, 0, c->stream>>>(a2);
+  HIP_TRY(hipGetLastError());
+  // Shock flags of the next step from the new state.
+  StageArgs a3 = base_args(c);
+  for (int f = 0; f < 5; f++) a3.src.f[f] = c->U[f];
+  shock_bits_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a3, c->bits);
+  HIP_TRY(hipGetLastError());
+  flags_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
+      make_int3(c->n[0], c->n[1], c->n[2]), c->bits, c->flags);
+  HIP_TRY(hipGetLastError());
+  c->prims_valid = false;
+  c->invdt_valid = true;
+  return GX_OK;
+}
+
+int gx_sync(gx_ctx *c) {
+  if (!c) return fail(GX_E_ARG, "gx_sync: null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_totals(gx_ctx *c, double out[5]) {
+  if (!c || !out) return fail(GX_E_ARG, "gx_totals: null argument");
+  if (!c->uploaded) return fail(GX_E_STATE, "gx_totals: no state uploaded");
+  StageArgs a = base_args(c);
+  for (int f = 0; f < 5; f++) a.aux.f[f] = c->U[f];
+  totals_kernel<<<c->n_partial, 256, 0, c->stream>>>(a, c->partial);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h(5 * (size_t)c->n_partial);
+  HIP_TRY(hipMemcpyAsync(h.data(), c->partial, sizeof(double) * h.size(),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int f = 0; f < 5; f++) {
+    double s = 0.0;
+    for (int b = 0; b < c->n_partial; b++) s += h[(size_t)b * 5 + f];
+    out[f] = s;
+  }
+  return GX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+"""Host mirror of the reference's fluid-step interface on the HIP path.
+
+``FluidParameters`` carries the reference's parameter names and defaults
+(Modules/Physics/StressEnergies/Fluids/Fluid_ASC__Form.f90:128-177,
+Fluid_P_I__Form.f90:206-223, FluidFeatures_ASC__Form.f90:61-65,
+Evolutions/Integrators/Integrator_C_PS__Form.f90:136-140).
+
+``Step_RK2_C_ASC`` owns one device context (gx_create) and exposes the two
+calls the integrator makes across the boundary -- ``time_step_candidate``
+(Integrator_C_PS % ComputeTimeStep_C_ASC) and ``step``
+(Step_RK_C_ASC_Template % Compute) -- plus state upload and download.
+All arithmetic runs in the HIP library; this module only moves buffers.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+from .chart import Chart
+
+
+@dataclass
+class FluidParameters:
+    AdiabaticIndex: float = 1.4
+    LimiterParameter: float = 1.4
+    ShockThreshold: float = 0.1
+    CourantFactor: float = 0.7
+    BaryonDensityMin: float = 0.0
+    BaryonMassReference: float = 1.0
+    UseLimiter: bool = True
+    RiemannSolverType: str = "HLLC"
+    Precision: str = "FP64"
+
+    def to_c(self) -> _capi.GxParams:
+        if self.RiemannSolverType not in ("HLL", "HLLC"):
+            raise ValueError(f"RiemannSolverType not recognized: {self.RiemannSolverType}")
+        if self.Precision not in ("FP64", "FP32"):
+            raise ValueError(f"Precision not recognized: {self.Precision}")
+        p = _capi.GxParams()
+        p.AdiabaticIndex = self.AdiabaticIndex
+        p.LimiterParameter = self.LimiterParameter
+        p.ShockThreshold = self.ShockThreshold
+        p.CourantFactor = self.CourantFactor
+        p.BaryonDensityMin = self.BaryonDensityMin
+        p.BaryonMassReference = self.BaryonMassReference
+        p.UseLimiter = int(bool(self.UseLimiter))
+        p.RiemannSolverType = _capi.GX_HLLC if self.RiemannSolverType == "HLLC" else _capi.GX_HLL
+        p.Precision = _capi.GX_FP64 if self.Precision == "FP64" else _capi.GX_FP32
+        return p
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Step_RK2_C_ASC:
+    """Device-resident RK2 step of one Fluid_P_I chart."""
+
+    def __init__(self, chart: Chart, params: FluidParameters | None = None):
+        self.lib = _capi.load()
+        self.chart = chart
+        self.params = params or FluidParameters()
+        g = _capi.GxGrid()
+        for d in range(3):
+            g.nCells[d] = chart.n[d]
+            g.MinCoordinate[d] = chart.lo[d]
+            g.MaxCoordinate[d] = chart.hi[d]
+        for i, code in enumerate(chart.bc_codes):
+            g.BoundaryCondition[i] = code
+        h = ctypes.c_void_p()
+        cp = self.params.to_c()
+        _capi.check(self.lib.gx_create(ctypes.byref(g), ctypes.byref(cp), ctypes.byref(h)),
+                    "gx_create")
+        self._h = h
+        self.shape = chart.n[::-1]  # (n2, n1, n0), x fastest
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.gx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- state -------------------------------------------------------------
+    def _proper(self, a: np.ndarray) -> np.ndarray:
+        a = np.asarray(a, dtype=np.float64)
+        if a.shape == self.chart.m[::-1]:
+            a = self.chart.proper(a)
+        if a.shape != self.shape:
+            raise ValueError(f"field shape {a.shape} does not match the chart {self.shape}")
+        return a
+
+    def set_primitive(self, N, V1, V2, V3, E):
+        """Primitives on proper or padded arrays; conserved fields follow on
+        the device (ComputeFromInitial)."""
+        buf = np.ascontiguousarray(np.stack([self._proper(a) for a in (N, V1, V2, V3, E)]))
+        _capi.check(self.lib.gx_set_primitive(self._h, _pd(buf)), "gx_set_primitive")
+
+    def set_state(self, primitive, conserved):
+        p = np.ascontiguousarray(np.stack([self._proper(a) for a in primitive]))
+        c = np.ascontiguousarray(np.stack([self._proper(a) for a in conserved]))
+        _capi.check(self.lib.gx_set_state(self._h, _pd(p), _pd(c)), "gx_set_state")
+
+    def conserved(self) -> np.ndarray:
+        out = np.empty((5,) + self.shape)
+        _capi.check(self.lib.gx_get_conserved(self._h, _pd(out)), "gx_get_conserved")
+        return out
+
+    def primitive(self) -> np.ndarray:
+        out = np.empty((5,) + self.shape)
+        _capi.check(self.lib.gx_get_primitive(self._h, _pd(out)), "gx_get_primitive")
+        return out
+
+    def flags(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.uint8)
+        _capi.check(self.lib.gx_get_flags(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))), "gx_get_flags")
+        return out
+
+    def set_flags(self, flags: np.ndarray):
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        _capi.check(self.lib.gx_set_flags(
+            self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))), "gx_set_flags")
+
+    def totals(self) -> np.ndarray:
+        out = np.empty(5)
+        _capi.check(self.lib.gx_totals(self._h, _pd(out)), "gx_totals")
+        return out
+
+    def upload(self, value: np.ndarray, field_map=None):
+        """Reference layout Value(nValues, nFields) as a (nFields, nValues)
+        C-contiguous array (column f is value[f])."""
+        v = np.ascontiguousarray(value, dtype=np.float64)
+        fm = np.arange(1, _capi.GX_N_FIELDS + 1, dtype=np.int32) if field_map is None \
+            else np.ascontiguousarray(field_map, dtype=np.int32)
+        _capi.check(self.lib.gx_upload(self._h, _pd(v), v.shape[1],
+                                       fm.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+                    "gx_upload")
+
+    def download(self, value: np.ndarray, field_map=None):
+        fm = np.arange(1, _capi.GX_N_FIELDS + 1, dtype=np.int32) if field_map is None \
+            else np.ascontiguousarray(field_map, dtype=np.int32)
+        assert value.flags.c_contiguous and value.dtype == np.float64
+        _capi.check(self.lib.gx_download(self._h, _pd(value), value.shape[1],
+                                         fm.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+                    "gx_download")
+
+    # --- the two boundary calls of the integrator --------------------------
+    def time_step_candidate(self) -> float:
+        c = ctypes.c_double()
+        _capi.check(self.lib.gx_compute_time_step(self._h, ctypes.byref(c)),
+                    "gx_compute_time_step")
+        return c.value
+
+    def step(self, time: float, dt: float):
+        _capi.check(self.lib.gx_step_rk2(self._h, float(time), float(dt)), "gx_step_rk2")
+
+    def sync(self):
+        _capi.check(self.lib.gx_sync(self._h), "gx_sync")
+
+    def set_tuning(self, z_chunk: int):
+        _capi.check(self.lib.gx_set_tuning(self._h, int(z_chunk)), "gx_set_tuning")
+
+    @property
+    def stream(self) -> int:
+        return self.lib.gx_stream(self._h) or 0

@@ -1,0 +1,182 @@
+/*
+ * gx_fluid.h -- C-ABI of the MI355X finite-volume step for the GenASiS
+ * polytropic ideal-gas fluid (Fluid_P_I): PLM/HLLC RK2 with CFL time step,
+ * periodic / outflow / reflecting ghost fill and shock flags.
+ *
+ * Plain C types only.  Every call returns 0 on success and a negative GX_E_*
+ * code on failure; gx_last_error() then holds the message (per thread).  One
+ * context owns all device memory of one problem and one HIP stream; host
+ * buffers passed in are borrowed for the duration of the call.  A context is
+ * not re-entrant: call it from one host thread at a time.
+ *
+ * Reference interfaces each entry point replaces (GenASiS tree, one rank):
+ *   gx_create          FluidBox_Form % Initialize with Fluid_P_I, Chart_SLD and
+ *                      Step_RK2_C_ASC (Modules/Physics/Universes/
+ *                      FluidBox_Form.f90:37-92, 205-241)
+ *   gx_upload          Integrator_C_PS % PrepareEvolution: ExchangeGhostData +
+ *                      ComputeFromInitial + UpdateDevice (Modules/Mathematics/
+ *                      Solvers/DifferentialEquations/Evolutions/Integrators/
+ *                      Integrator_C_PS__Form.f90:175-224)
+ *   gx_compute_time_step  Integrator_C_PS % ComputeTimeStep_C_ASC
+ *                      (Integrator_C_PS__Form.f90:374-423,
+ *                      Integrator_C_PS__Kernel.f90:12-144)
+ *   gx_step_rk2        Step_RK_C_ASC_Template % Compute (…/Evolutions/Steps/
+ *                      Step_RK_C_ASC__Template.f90:504-542) with the Heun
+ *                      tableau of Step_RK2_C_ASC__Form.f90:51-57
+ *   gx_download        Integrator_C_PS % UpdateHost (Integrator_C_PS__Form.
+ *                      f90:280-299): the Current in the reference layout
+ *   gx_totals          VolumeIntegral of the conserved fields (Modules/
+ *                      Mathematics/Operations/Calculus/Integrals/
+ *                      VolumeIntegral_Form.f90:63), one rank
+ */
+#ifndef GX_FLUID_H
+#define GX_FLUID_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GX_OK 0
+#define GX_E_ARG -1     /* invalid argument */
+#define GX_E_HIP -2     /* HIP runtime error */
+#define GX_E_NOMEM -3   /* device or host allocation failed */
+#define GX_E_STATE -4   /* call out of order (e.g. step before upload) */
+
+/* boundary conditions (Atlas_SC__Template.f90:244-324) */
+#define GX_BC_PERIODIC 0
+#define GX_BC_OUTFLOW 1
+#define GX_BC_REFLECTING 2
+
+/* RiemannSolverType (Fluid_ASC__Form.f90:128-136) */
+#define GX_HLL 0
+#define GX_HLLC 1
+
+/* Precision of the state and sweep arithmetic */
+#define GX_FP64 0
+#define GX_FP32 1
+
+/* Fluid_P_I fields, in the order of the reference's field map
+ * (FLD/Current_Template.f90:752-753, F/Fluid_D__Form.f90:650-654,
+ * F/Fluid_P__Template.f90:718-725). */
+enum {
+  GX_FAST_EIGENSPEED_PLUS_1 = 0,
+  GX_FAST_EIGENSPEED_PLUS_2,
+  GX_FAST_EIGENSPEED_PLUS_3,
+  GX_FAST_EIGENSPEED_MINUS_1,
+  GX_FAST_EIGENSPEED_MINUS_2,
+  GX_FAST_EIGENSPEED_MINUS_3,
+  GX_COMOVING_BARYON_DENSITY,
+  GX_CONSERVED_BARYON_DENSITY,
+  GX_BARYON_MASS,
+  GX_VELOCITY_U_1,
+  GX_VELOCITY_U_2,
+  GX_VELOCITY_U_3,
+  GX_MOMENTUM_DENSITY_D_1,
+  GX_MOMENTUM_DENSITY_D_2,
+  GX_MOMENTUM_DENSITY_D_3,
+  GX_INTERNAL_ENERGY,
+  GX_CONSERVED_ENERGY,
+  GX_PRESSURE,
+  GX_TEMPERATURE,
+  GX_ENTROPY_PER_BARYON,
+  GX_CONSERVED_ENTROPY,
+  GX_SOUND_SPEED,
+  GX_MACH_NUMBER,
+  GX_N_FIELDS
+};
+
+/* Chart: proper cells, coordinate box and one boundary condition per face
+ * (BoundaryCondition[2*d] inner, [2*d+1] outer).  nGhostLayers is 2. */
+typedef struct {
+  int nCells[3];
+  double MinCoordinate[3];
+  double MaxCoordinate[3];
+  int BoundaryCondition[6];
+} gx_grid;
+
+/* Parameters, named as the reference's GetParameter keys. */
+typedef struct {
+  double AdiabaticIndex;      /* 1.4  (Fluid_P_I__Form.f90:210) */
+  double LimiterParameter;    /* 1.4  (Fluid_ASC__Form.f90:173-177) */
+  double ShockThreshold;      /* 0.1  (FluidFeatures_ASC__Form.f90:61) */
+  double CourantFactor;       /* 0.7  (Integrator_C_PS__Form.f90:136) */
+  double BaryonDensityMin;    /* 0.0  (Fluid_D__Form.f90:35) */
+  double BaryonMassReference; /* 1.0 */
+  int UseLimiter;             /* 1 */
+  int RiemannSolverType;      /* GX_HLLC */
+  int Precision;              /* GX_FP64 */
+} gx_params;
+
+typedef struct gx_ctx gx_ctx;
+
+/* Fills p with the reference defaults above. */
+void gx_default_params(gx_params *p);
+
+int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **ctx);
+int gx_destroy(gx_ctx *ctx);
+
+/* Upload the Current from the reference layout Value(nValues, nVariables),
+ * column-major: field f of value iv at value[(fieldMap[f]-1)*ldValue + iv],
+ * nValues = prod(nCells+4) with x fastest.  fieldMap has GX_N_FIELDS entries,
+ * 1-based column numbers; an entry <= 0 marks an absent field.  The five
+ * primitive fields (N, V_1..3, E) and the five conserved fields (D, S_1..3,
+ * G) must be present.  The first step after an upload reconstructs from the
+ * uploaded primitives, as the reference's first stage does.  Shock flags are
+ * cleared. */
+int gx_upload(gx_ctx *ctx, const double *value, long long ldValue,
+              const int *fieldMap);
+
+/* Write the Current (all present fields of fieldMap) for proper cells, and
+ * for ghost cells of periodic dimensions (their periodic image).  Ghost cells
+ * across non-periodic faces are not written. */
+int gx_download(gx_ctx *ctx, double *value, long long ldValue,
+                const int *fieldMap);
+
+/* DiffusiveFlux_I_1..3 of FluidFeatures_P (F/FluidFeatures_Template.f90:156)
+ * in the same padded layout, three consecutive columns of ldValue values. */
+int gx_upload_features(gx_ctx *ctx, const double *df, long long ldValue);
+int gx_download_features(gx_ctx *ctx, double *df, long long ldValue);
+
+/* ComputeTimeStep_C_ASC: *candidate = CourantFactor * min(huge,
+ * 1/max(tiny, invdt)), invdt = max over proper cells of
+ * sum_d max(FEP_d, -FEM_d)/(WidthLeft_d + WidthRight_d).  Synchronises. */
+int gx_compute_time_step(gx_ctx *ctx, double *candidate);
+
+/* One Heun RK2 step of size dt (stream-ordered, asynchronous).  time is the
+ * reference's Time argument and does not enter the arithmetic. */
+int gx_step_rk2(gx_ctx *ctx, double time, double dt);
+
+/* Block until all work of the context is done. */
+int gx_sync(gx_ctx *ctx);
+
+/* Sum over proper cells of D, S_1, S_2, S_3, G times the cell volume, in
+ * FP64.  Synchronises. */
+int gx_totals(gx_ctx *ctx, double out[5]);
+
+/* Proper-cell views without the padded layout: five contiguous fields of
+ * nCells[0]*nCells[1]*nCells[2] doubles each, x fastest. */
+int gx_set_state(gx_ctx *ctx, const double *primitive, const double *conserved);
+/* Primitives only; the conserved fields follow on the device as
+ * ComputeFromInitial does (F/Fluid_P_I__Form.f90:509-601). */
+int gx_set_primitive(gx_ctx *ctx, const double *primitive);
+int gx_get_conserved(gx_ctx *ctx, double *conserved);
+int gx_get_primitive(gx_ctx *ctx, double *primitive);
+/* Packed shock flags: one byte per proper cell, bit d = DiffusiveFlux_I_(d+1). */
+int gx_get_flags(gx_ctx *ctx, unsigned char *flags);
+int gx_set_flags(gx_ctx *ctx, const unsigned char *flags);
+
+/* Launch-shape control for measurement: z-chunk length of the fused stage
+ * kernel (planes per thread block, default 16). */
+int gx_set_tuning(gx_ctx *ctx, int zChunk);
+
+/* The HIP stream of the context (a hipStream_t), for event timing. */
+void *gx_stream(gx_ctx *ctx);
+
+const char *gx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

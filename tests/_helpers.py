@@ -1,0 +1,49 @@
+"""Shared drivers for the parity tests: the same seeded problem on the oracle
+(CPU restatement) and on the HIP library, stepped with the same time steps."""
+from __future__ import annotations
+
+import numpy as np
+
+from genasis_amd.chart import Chart
+from genasis_amd.integrator import Integrator_C_PS, IntegratorParameters
+
+import oracle
+
+
+class OracleStepper:
+    def __init__(self, o):
+        self.o = o
+
+    def time_step_candidate(self):
+        return self.o.time_step_candidate()[0]
+
+    def step(self, time, dt):
+        self.o.step(dt)
+
+
+def oracle_conserved(o, chart: Chart) -> np.ndarray:
+    return np.stack([chart.proper(a) for a in o.conserved()])
+
+
+def oracle_primitive(o, chart: Chart) -> np.ndarray:
+    return np.stack([chart.proper(a) for a in o.primitive()])
+
+
+def oracle_flag_bits(o, chart: Chart) -> np.ndarray:
+    bits = np.zeros(chart.n[::-1], dtype=np.uint8)
+    for d in range(3):
+        bits |= (chart.proper(o.flags(d)) > 0).astype(np.uint8) << d
+    return bits
+
+
+def make_oracle(chart, prim, **kw):
+    o = oracle.OracleFluid(chart, **kw)
+    o.set_primitive(*prim)
+    return o
+
+
+def evolve(stepper, n_cycles, finish_time=1.0e30, n_write=1):
+    it = Integrator_C_PS(stepper, IntegratorParameters(
+        FinishTime=finish_time, FinishCycle=n_cycles, nWrite=n_write))
+    it.evolve()
+    return it
