@@ -1,0 +1,188 @@
+"""Benchmark: PLM + HLLC RK2 FP64 step of the Fluid_P_I fluid on one MI355X.
+
+Workload: 3-D Kelvin-Helmholtz on a 256^3 periodic unit cube, gamma 1.4,
+CourantFactor 0.8, LimiterParameter 1.4, nRampCycles 100, seeded vy
+perturbation (genasis_amd.problems.kelvin_helmholtz).  One step = one RK2
+cycle of the integrator: CFL time step (device max-reduction, read back),
+two fused stage kernels, shock-flag kernels.  The state is resident in HBM
+before the timed region; ``warmup`` cycles run first, then ``steps`` cycles
+are timed between device synchronisations.
+
+Prints one JSON line.  ``roofline`` prices the dominant kernel (the stage-2
+kernel) against HBM: algorithmic bytes per launch (120 B per cell: read Y and
+U + K1/2, write U_new, FP64, five fields) over its average duration from HIP
+events in the timed region.  ``cpu_baseline`` times the oracle (the CPU
+restatement of the reference step) on the host cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("cell-updates/s, PLM+HLLC RK2 FP64 256^3; %HBM peak; FP32 err; "
+          "adjoint/fwd")
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per cell and launch (FP64, five conserved fields):
+# stage 1 reads U and writes U + K1 (80 B; U + K1/2 is an extra 40 B written
+# because the reference adds K1/2 and K2/2 to U one after the other), stage 2
+# reads U + K1 and U + K1/2 and writes U_new (120 B).
+ALG_BYTES_STAGE2 = 120.0
+ALG_BYTES_STEP = 200.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_kh256_fp64_traffic.txt")
+
+
+def _read_traffic():
+    """HBM bytes per stage-2 launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("stage2_hbm_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def cpu_baseline(n: int, steps: int, threads: int):
+    """Oracle (CPU restatement of the reference step, OpenMP) on the host."""
+    import oracle
+    from genasis_amd import problems
+    from genasis_amd.chart import Chart
+
+    chart = Chart((n, n, n))
+    o = oracle.OracleFluid(chart, courant=0.8, n_threads=threads)
+    o.set_primitive(*problems.kelvin_helmholtz(chart, seed=1234))
+    c = o.time_step_candidate()[0]
+    o.step(0.01 * c)  # untimed: first touch of the oracle's arrays
+    t0 = time.perf_counter()
+    for s in range(steps):
+        c = o.time_step_candidate()[0]
+        o.step((s + 2) / 100.0 * c)
+    t1 = time.perf_counter()
+    o.close()
+    return chart.n_proper * steps / (t1 - t0)
+
+
+def copy_calibration():
+    """A plain 1 GiB device copy (torch), for scaling the PMC byte counters."""
+    import torch
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": a.numel() * 8,
+                      "copy_calibration_bytes_written": b.numel() * 8}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=98)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--z-chunk", type=int, default=16)
+    ap.add_argument("--cpu-baseline", choices=("auto", "off"), default="auto")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
+              "supported (multi-GPU sharding is a design note only).", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+
+    from genasis_amd import problems
+    from genasis_amd.chart import Chart
+    from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
+    from genasis_amd.integrator import Integrator_C_PS, IntegratorParameters
+
+    n = args.n
+    chart = Chart((n, n, n))
+    params = FluidParameters(CourantFactor=0.8)
+    stepper = Step_RK2_C_ASC(chart, params)
+    stepper.set_tuning(args.z_chunk)
+    stepper.set_primitive(*problems.kelvin_helmholtz(chart, seed=1234))
+    total0 = stepper.totals()
+    integ = Integrator_C_PS(stepper, IntegratorParameters(
+        FinishTime=1.0e30, FinishCycle=args.warmup + args.steps, nWrite=1))
+    integ._administer_checkpoint()
+
+    for _ in range(args.warmup):
+        integ.compute_cycle()
+    stepper.sync()
+    stepper.set_profiling(True)
+    stepper.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        integ.compute_cycle()
+    stepper.sync()
+    t1 = time.perf_counter()
+    kt = stepper.kernel_times()
+    stepper.set_profiling(False)
+    total1 = stepper.totals()
+
+    elapsed = t1 - t0
+    value = chart.n_proper * args.steps / elapsed
+    s2_ms, s2_n = kt["stage2"]
+    s2_avg = s2_ms / max(s2_n, 1) * 1e-3
+    achieved = ALG_BYTES_STAGE2 * chart.n_proper / s2_avg / 1e9
+    step_kernel_s = sum(v[0] for v in kt.values()) / max(s2_n, 1) * 1e-3
+    traffic = _read_traffic()
+
+    cpu = None
+    if args.cpu_baseline == "auto":
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) \
+            or os.cpu_count() or 1
+        cpu_value = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
+        cpu = {"value": cpu_value, "unit": "cell-updates/s", "cores": threads,
+               "kind": "port",
+               "sample": (f"oracle (C/OpenMP restatement of the reference step) on "
+                          f"the same KH problem at {args.cpu_n}^3, {args.cpu_steps} "
+                          f"timed RK2 steps after 1 untimed step, {threads} threads")}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Kelvin-Helmholtz initial data)",
+        "config": {"workload": f"Kelvin-Helmholtz {n}^3 periodic, PLM(MC theta=1.4)"
+                               f"+HLLC, RK2, CFL 0.8, FP64",
+                   "cells": chart.n_proper, "z_chunk": args.z_chunk},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic},
+        "step_roofline": {"alg_bytes_per_cell_update": ALG_BYTES_STEP,
+                          "kernel_s_per_step": step_kernel_s,
+                          "frac": ALG_BYTES_STEP * chart.n_proper / step_kernel_s
+                          / 1e9 / HBM_PEAK_GBS},
+        "kernel_ms_avg": {k: v[0] / max(v[1], 1) for k, v in kt.items()},
+        "conserved_totals_change": [float(a - b) for a, b in zip(total1, total0)],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -571,6 +571,11 @@ struct gx_ctx {
   bool uploaded = false;
   bool prims_valid = false;   // next stage 1 reads P0
   bool invdt_valid = false;   // invdt holds the max for the current state
+  // per-kernel timing (gx_set_profiling): stage 1, stage 2, shock bits, flags
+  bool profiling = false;
+  hipEvent_t ev[5] = {};
+  double kernel_ms[4] = {0, 0, 0, 0};
+  long long kernel_n[4] = {0, 0, 0, 0};
   std::vector<double> geo_host[3][8];
 };
 
@@ -661,6 +666,8 @@ const char *gx_last_error(void) { return g_last_error.c_str(); }
 int gx_destroy(gx_ctx *c) {
   if (!c) return GX_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int e = 0; e < 5; e++)
+    if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
   (void)hipFree(c->geo_buf);
   for (int f = 0; f < 5; f++) {
     (void)hipFree(c->U[f]);
@@ -768,6 +775,27 @@ int gx_set_tuning(gx_ctx *c, int zChunk) {
 }
 
 void *gx_stream(gx_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int gx_set_profiling(gx_ctx *c, int on) {
+  if (!c) return fail(GX_E_ARG, "gx_set_profiling: null context");
+  if (on && !c->ev[0])
+    for (int e = 0; e < 5; e++) HIP_TRY(hipEventCreate(&c->ev[e]));
+  c->profiling = on != 0;
+  for (int k = 0; k < 4; k++) {
+    c->kernel_ms[k] = 0.0;
+    c->kernel_n[k] = 0;
+  }
+  return GX_OK;
+}
+
+int gx_kernel_times(gx_ctx *c, double ms[4], long long count[4]) {
+  if (!c || !ms || !count) return fail(GX_E_ARG, "gx_kernel_times: null argument");
+  for (int k = 0; k < 4; k++) {
+    ms[k] = c->kernel_ms[k];
+    count[k] = c->kernel_n[k];
+  }
+  return GX_OK;
+}
 
 int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
   if (!c || !primitive || !conserved) return fail(GX_E_ARG, "gx_set_state: null argument");
@@ -1005,6 +1033,7 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   const dim3 block(NT);
   const dim3 grid((c->n[0] + TX - 1) / TX, (c->n[1] + TY - 1) / TY,
                   (c->n[2] + c->tz - 1) / c->tz);
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   // Stage 1: Current = uploaded primitives or the Solution.
   StageArgs a1 = base_args(c);
   a1.dt = dt;
@@ -1019,6 +1048,7 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   HIP_TRY(hipGetLastError());
   // Stage 2: Current = Y.
   HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   StageArgs a2 = base_args(c);
   a2.dt = dt;
   for (int f = 0; f < 5; f++) {
@@ -1028,14 +1058,28 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   }
   stage_kernel<2><<<grid, block, 0, c->stream>>>(a2);
   HIP_TRY(hipGetLastError());
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
   // Shock flags of the next step from the new state.
   StageArgs a3 = base_args(c);
   for (int f = 0; f < 5; f++) a3.src.f[f] = c->U[f];
   shock_bits_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a3, c->bits);
   HIP_TRY(hipGetLastError());
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   flags_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
       make_int3(c->n[0], c->n[1], c->n[2]), c->bits, c->flags);
   HIP_TRY(hipGetLastError());
+  if (c->profiling) {
+    // The stage-1 interval starts before the invdt reset; the reset is a
+    // few-byte fill and is counted with stage 1.
+    HIP_TRY(hipEventRecord(c->ev[4], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[4]));
+    for (int k = 0; k < 4; k++) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
+      c->kernel_ms[k] += ms;
+      c->kernel_n[k] += 1;
+    }
+  }
   c->prims_valid = false;
   c->invdt_valid = true;
   return GX_OK;

@@ -168,6 +168,17 @@ class Step_RK2_C_ASC:
     def set_tuning(self, z_chunk: int):
         _capi.check(self.lib.gx_set_tuning(self._h, int(z_chunk)), "gx_set_tuning")
 
+    def set_profiling(self, on: bool):
+        _capi.check(self.lib.gx_set_profiling(self._h, int(bool(on))), "gx_set_profiling")
+
+    def kernel_times(self):
+        """{name: (total_ms, launches)} for stage 1, stage 2, shock bits, flags."""
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_longlong * 4)()
+        _capi.check(self.lib.gx_kernel_times(self._h, ms, n), "gx_kernel_times")
+        names = ("stage1", "stage2", "shock_bits", "flags")
+        return {k: (ms[i], n[i]) for i, k in enumerate(names)}
+
     @property
     def stream(self) -> int:
         return self.lib.gx_stream(self._h) or 0

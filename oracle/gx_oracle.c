@@ -63,7 +63,9 @@ static inline long IDX(const gxo_ctx *c, int p0, int p1, int p2) {
   return (long)p0 + (long)c->m[0] * ((long)p1 + (long)c->m[1] * (long)p2);
 }
 
-static double *alloc0(long n) { return (double *)calloc((size_t)n, sizeof(double)); }
+static double *alloc0(long n) {
+  return (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+}
 
 /* ---- geometry: CH/ChartBasics/Chart_Template.f90:463-593, 699-723, 827-841,
  * 985-987 (SetGeometryCell, SetEdgeEqual, SetCenterRectangular,
