@@ -34,7 +34,7 @@ EXPORTS = (
     "gx_step_rk2", "gx_sync", "gx_totals", "gx_set_state", "gx_set_primitive",
     "gx_get_conserved", "gx_get_primitive", "gx_get_flags", "gx_set_flags",
     "gx_set_tuning", "gx_stream", "gx_set_profiling", "gx_kernel_times",
-    "gx_last_error",
+    "gx_self_test_division", "gx_last_error",
 )
 
 
@@ -101,6 +101,7 @@ def load(path: str = LIB_PATH):
         "gx_stream": (vp, [vp]),
         "gx_set_profiling": (i, [vp, i]),
         "gx_kernel_times": (i, [vp, pd, P(ctypes.c_longlong)]),
+        "gx_self_test_division": (i, [ll, ctypes.c_ulonglong, P(ctypes.c_longlong)]),
         "gx_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

@@ -37,31 +37,45 @@ constexpr int WAVE = 64;
 
 __host__ __device__ inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-struct Field5 {
-  const double *f[5];
-};
-struct Field5W {
-  double *f[5];
-};
-
+// Field f of a five-field state lives at base + f * fs (fs = cells).  The
+// geometry is one buffer: array g of dimension d at geo + (d*N_GEO + g)*gs.
+// Few pointers keep the kernel's scalar registers free.
 struct StageArgs {
   int n[3];
   int bc[6];
   int tz;               // planes per block (z-chunk)
   int src_is_prim;      // stage input holds primitives (first stage after upload)
+  int gs;               // geometry stride (max n + 4)
+  long long fs;         // field stride (cells)
   double dt;
   Physics ph;
-  Field5 src;           // stage input: conserved (or primitive) state
-  Field5 aux;           // stage 1: U (Solution); stage 2: W = U + K1/2
-  Field5W out1;         // stage 1: Y = U + K1; stage 2: U_new
-  Field5W out2;         // stage 1: W
+  const double *src;    // stage input: conserved (or primitive) state
+  const double *aux;    // stage 1: U (Solution); stage 2: W = U + K1/2
+  double *out1;         // stage 1: Y = U + K1; stage 2: U_new
+  double *out2;         // stage 1: W
+  const double *geo;
   const uint8_t *flags; // DiffusiveFlux bits of the step
   unsigned long long *invdt_bits;  // stage 2: max CFL term (as bits)
-  Geo1D geo[3];
 };
 
-__device__ __forceinline__ long long lin(const int n[3], int x, int y, int z) {
-  return (long long)x + (long long)n[0] * ((long long)y + (long long)n[1] * z);
+// Loads and stores at a 32-bit element offset from a base pointer: the byte
+// offset stays 32-bit (charts hold < 2^29 cells, checked at creation), so the
+// address is a scalar base plus one 32-bit VGPR.
+template <typename T>
+__device__ __forceinline__ T ld(const T *base, unsigned idx) {
+  return *(const T *)((const char *)base + (size_t)(idx * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st(T *base, unsigned idx, T v) {
+  *(T *)((char *)base + (size_t)(idx * (unsigned)sizeof(T))) = v;
+}
+
+__device__ __forceinline__ double geo(const StageArgs &a, int d, int g, int p) {
+  return ld(a.geo + (d * N_GEO + g) * a.gs, (unsigned)p);
+}
+
+__device__ __forceinline__ unsigned lin(const int n[3], int x, int y, int z) {
+  return (unsigned)x + (unsigned)n[0] * ((unsigned)y + (unsigned)n[1] * (unsigned)z);
 }
 
 // Ghost index rule of one dimension: periodic wrap, or the mirror copy of
@@ -81,69 +95,155 @@ __device__ __forceinline__ int map_coord(int c, int n, int bc_in, int bc_out,
   return c;
 }
 
-// Primitive state of cell (x, y, z), at most one coordinate outside the chart
-// by at most two cells; ghost values by the boundary rule of that face.
-__device__ __forceinline__ void load_prim(const StageArgs &a, int x, int y,
-                                          int z, double V[5]) {
+// Linear index of cell (x, y, z) with at most one coordinate outside the
+// chart by at most two cells, after the ghost rule of that face; *flip = the
+// dimension whose velocity a reflecting face negates, or -1.
+__device__ __forceinline__ unsigned cell_index(const StageArgs &a, int x, int y,
+                                               int z, int *flip) {
+  *flip = -1;
+  if ((unsigned)x < (unsigned)a.n[0] && (unsigned)y < (unsigned)a.n[1] &&
+      (unsigned)z < (unsigned)a.n[2])
+    return lin(a.n, x, y, z);
   bool fx = false, fy = false, fz = false;
   const int cx = map_coord(x, a.n[0], a.bc[0], a.bc[1], &fx);
   const int cy = map_coord(y, a.n[1], a.bc[2], a.bc[3], &fy);
   const int cz = map_coord(z, a.n[2], a.bc[4], a.bc[5], &fz);
-  const long long i =
-      (long long)cx + (long long)a.n[0] * ((long long)cy + (long long)a.n[1] * cz);
-  if (a.src_is_prim) {
-#pragma unroll
-    for (int f = 0; f < 5; f++) V[f] = a.src.f[f][i];
-  } else {
-    double U[5];
-#pragma unroll
-    for (int f = 0; f < 5; f++) U[f] = a.src.f[f][i];
-    prim_from_cons(U, a.ph.mass, a.ph.nmin, V);
-  }
-  if (fx) V[1] = -V[1];
-  if (fy) V[2] = -V[2];
-  if (fz) V[3] = -V[3];
+  *flip = fx ? 0 : (fy ? 1 : (fz ? 2 : -1));
+  return lin(a.n, cx, cy, cz);
 }
 
-__device__ __forceinline__ double slope_at(const Geo1D &g, int c, double vm,
-                                           double v0, double vp,
-                                           const Physics &ph) {
-  const int p = c + 2;
-  return slope(vm, v0, vp, g.dxl[p], g.dxr[p], g.dxl2[p], g.dxr2[p], g.den[p],
-               ph.theta, ph.use_limiter);
+// Raw stage input (conserved, or primitive after an upload) of one cell.
+__device__ __forceinline__ void load_raw(const StageArgs &a, int x, int y, int z,
+                                         double R[5], int *flip) {
+  const unsigned i = cell_index(a, x, y, z, flip);
+#pragma unroll
+  for (int f = 0; f < 5; f++) R[f] = ld(a.src + f * a.fs, i);
+}
+
+// Primitives of a raw value, with the reflecting sign flip.
+__device__ __forceinline__ void raw_to_prim(const StageArgs &a, const double R[5],
+                                            int flip, double V[5]) {
+  if (a.src_is_prim) {
+#pragma unroll
+    for (int f = 0; f < 5; f++) V[f] = R[f];
+  } else {
+    prim_from_cons(R, a.ph.mass, a.ph.nmin, V);
+  }
+  // Sign flips as bit operations: a select on an array element here is
+  // turned into an indexed (scratch) store by the compiler.
+  const long long sbit = 0x8000000000000000ll;
+  V[1] = __longlong_as_double(__double_as_longlong(V[1]) ^ (flip == 0 ? sbit : 0ll));
+  V[2] = __longlong_as_double(__double_as_longlong(V[2]) ^ (flip == 1 ? sbit : 0ll));
+  V[3] = __longlong_as_double(__double_as_longlong(V[3]) ^ (flip == 2 ? sbit : 0ll));
+}
+
+__device__ __forceinline__ void load_prim(const StageArgs &a, int x, int y,
+                                          int z, double V[5]) {
+  double R[5];
+  int flip;
+  load_raw(a, x, y, z, R, &flip);
+  raw_to_prim(a, R, flip, V);
+}
+
+__device__ __forceinline__ double slope_at(const StageArgs &a, int d, int c,
+                                           double vm, double v0, double vp) {
+  const double *g = a.geo + d * N_GEO * a.gs + (c + 2);
+  const int s = a.gs;
+  return slope(vm, v0, vp, g[G_DXL * s], g[G_DXR * s], g[G_DXL2 * s], g[G_DXR2 * s],
+               g[G_DEN * s], g[G_RDXL * s], g[G_RDXR * s], g[G_RDEN * s],
+               a.ph.theta, a.ph.use_limiter);
 }
 
 __device__ __forceinline__ bool flag_bit(const StageArgs &a, int x, int y,
                                          int z, int d) {
   const int c[3] = {x, y, z};
   if (c[d] >= a.n[d]) return false;  // ghost face: cleared in the reference
-  return (a.flags[lin(a.n, x, y, z)] >> d) & 1;
+  return (ld(a.flags, lin(a.n, x, y, z)) >> d) & 1;
+}
+
+// Halo cell t of the plane tile: 4 per row for x (two each side), 4 per
+// column for y.
+__device__ __forceinline__ void halo_cell(int t, int x0, int y0, int tw, int th,
+                                          int *hx, int *hy) {
+  if (t < 4 * th) {
+    const int j = t >> 2, s = t & 3;
+    *hy = y0 + j;
+    *hx = s < 2 ? x0 - 2 + s : x0 + tw + (s - 2);
+  } else {
+    const int u = t - 4 * th;
+    const int i = u >> 2, s = u & 3;
+    *hx = x0 + i;
+    *hy = s < 2 ? y0 - 2 + s : y0 + th + (s - 2);
+  }
+}
+
+// x-face i (between cells x0+i-1 and x0+i) of row j into sFx.
+template <typename SP, typename SS, typename SF>
+__device__ __forceinline__ void x_face(const StageArgs &a, SP sP, SS sSx, SF sFx,
+                                       int i, int j, int x0, int y0, int k) {
+  const int bl = (j + 2) * HX + (i + 1);
+  const double wr = geo(a, 0, G_WR, x0 + i - 1 + 2), wl = geo(a, 0, G_WL, x0 + i + 2);
+  double WL[5], WR[5], F[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    WL[f] = sP[f][bl] + wr * sSx[f][j * SXW + i];
+    WR[f] = sP[f][bl + 1] - wl * sSx[f][j * SXW + i + 1];
+  }
+  face_flux(WL, WR, 0, flag_bit(a, x0 + i, y0 + j, k, 0), a.ph, F);
+#pragma unroll
+  for (int f = 0; f < 5; f++) sFx[f][j * FXW + i] = F[f];
+}
+
+// y-face j (between rows y0+j-1 and y0+j) of column i into sFy.
+template <typename SP, typename SS, typename SF>
+__device__ __forceinline__ void y_face(const StageArgs &a, SP sP, SS sSy, SF sFy,
+                                       int i, int j, int x0, int y0, int k) {
+  const int bl = (j + 1) * HX + (i + 2);
+  const double wr = geo(a, 1, G_WR, y0 + j - 1 + 2), wl = geo(a, 1, G_WL, y0 + j + 2);
+  double WL[5], WR[5], F[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    WL[f] = sP[f][bl] + wr * sSy[f][j * TX + i];
+    WR[f] = sP[f][bl + HX] - wl * sSy[f][(j + 1) * TX + i];
+  }
+  face_flux(WL, WR, 1, flag_bit(a, x0 + i, y0 + j, k, 1), a.ph, F);
+#pragma unroll
+  for (int f = 0; f < 5; f++) sFy[f][j * TX + i] = F[f];
 }
 
 // Fused stage: K = dt * (-div F) over the three dimensions, then the RK
 // combination.  A block owns a TX x TY column tile and marches along z over
 // a.tz planes; x/y stencils come from the plane tile in LDS, the z stencil
-// from registers.
+// from registers.  Every thread computes the x- and y-slopes and the left and
+// bottom faces of its own cell; the tile-edge slopes and faces are extra
+// tasks.  Global loads of plane k+1 (halo, z-stencil) are issued during plane
+// k, and the RK input of plane k at its start, so that their latency overlaps
+// the slope and flux work.
 template <int STAGE>
-__global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
   __shared__ double sP[5][HY * HX];
-  __shared__ double sSx[5][TY * SXW];       // x-slopes
-  __shared__ double sSy[5][(TY + 2) * TX];  // y-slopes
-  __shared__ double sFx[5][TY * FXW];       // x-face fluxes
-  __shared__ double sFy[5][(TY + 1) * TX];  // y-face fluxes
+  __shared__ double sSx[5][TY * SXW];       // x-slopes of cells x0-1 .. x0+tw
+  __shared__ double sSy[5][(TY + 2) * TX];  // y-slopes of cells y0-1 .. y0+th
+  __shared__ double sFx[5][TY * FXW];       // x-face fluxes x0 .. x0+tw
+  __shared__ double sFy[5][(TY + 1) * TX];  // y-face fluxes y0 .. y0+th
   __shared__ double sRed[NT / WAVE];
 
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * a.tz;
   const int tw = min(TX, nx - x0), th = min(TY, ny - y0);
   const int zend = min(z0 + a.tz, nz);
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
   const bool active = tx < tw && ty < th;
   const int x = x0 + tx, y = y0 + ty;
+  const int ctr = (ty + 2) * HX + tx + 2;  // own cell in the plane tile
   const Physics ph = a.ph;
-  const Geo1D &gx = a.geo[0], &gy = a.geo[1], &gz = a.geo[2];
+  const int nhalo = 4 * th + 4 * tw;
+  const int nxs = 2 * th, nys = 2 * tw;  // extra slope tasks
 
   double Vk[5], Vk1[5], sk[5], Flo[5];
+  double zr[5], hr[5];  // prefetched raw values: z-stencil k+2, halo of k
+  int zflip = -1, hflip = -1, hpos = 0;
   double tmax = 0.0;
 
   if (active) {
@@ -152,143 +252,144 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
     load_prim(a, x, y, z0 - 1, Vm1);
     load_prim(a, x, y, z0, Vk);
     load_prim(a, x, y, z0 + 1, Vk1);
+    load_raw(a, x, y, z0 + 2, zr, &zflip);
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      sm1[f] = slope_at(gz, z0 - 1, Vm2[f], Vm1[f], Vk[f], ph);
-      sk[f] = slope_at(gz, z0, Vm1[f], Vk[f], Vk1[f], ph);
-      WL[f] = Vm1[f] + gz.wr[z0 - 1 + 2] * sm1[f];
-      WR[f] = Vk[f] - gz.wl[z0 + 2] * sk[f];
+      sm1[f] = slope_at(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f]);
+      sk[f] = slope_at(a, 2, z0, Vm1[f], Vk[f], Vk1[f]);
+      WL[f] = Vm1[f] + geo(a, 2, G_WR, z0 - 1 + 2) * sm1[f];
+      WR[f] = Vk[f] - geo(a, 2, G_WL, z0 + 2) * sk[f];
     }
     face_flux(WL, WR, 2, flag_bit(a, x, y, z0, 2), ph, Flo);
   }
-
-  const int nsx = round_up((tw + 2) * th, WAVE);  // x-slope tasks, padded
-  const int nsy = tw * (th + 2);
-  const int nfx = round_up((tw + 1) * th, WAVE);  // x-face tasks, padded
-  const int nfy = tw * (th + 1);
-  const int nhalo = 4 * th + 4 * tw;
+  if (tid < nhalo) {
+    int hx, hy;
+    halo_cell(tid, x0, y0, tw, th, &hx, &hy);
+    hpos = (hy - y0 + 2) * HX + (hx - x0 + 2);
+    load_raw(a, hx, hy, z0, hr, &hflip);
+  }
 
   for (int k = z0; k < zend; ++k) {
-    double Fhi[5], sk1[5], Vk2[5];
+    const bool more = k + 1 < zend;
+    double ar[5], zn[5];
+    int znflip = -1;
     if (active) {
-      double WL[5], WR[5];
-      load_prim(a, x, y, k + 2, Vk2);
 #pragma unroll
-      for (int f = 0; f < 5; f++) {
-        sk1[f] = slope_at(gz, k + 1, Vk[f], Vk1[f], Vk2[f], ph);
-        WL[f] = Vk[f] + gz.wr[k + 2] * sk[f];
-        WR[f] = Vk1[f] - gz.wl[k + 1 + 2] * sk1[f];
-      }
-      face_flux(WL, WR, 2, flag_bit(a, x, y, k + 1, 2), ph, Fhi);
-#pragma unroll
-      for (int f = 0; f < 5; f++) sP[f][(ty + 2) * HX + tx + 2] = Vk[f];
+      for (int f = 0; f < 5; f++) sP[f][ctr] = Vk[f];
     }
-    // Ghost rows and columns of the plane tile.
-    for (int t = threadIdx.x; t < nhalo; t += NT) {
-      int hx, hy;
-      if (t < 4 * th) {
-        const int j = t >> 2, s = t & 3;
-        hy = y0 + j;
-        hx = s < 2 ? x0 - 2 + s : x0 + tw + (s - 2);
-      } else {
-        const int u = t - 4 * th;
-        const int i = u % tw, s = u / tw;
-        hx = x0 + i;
-        hy = s < 2 ? y0 - 2 + s : y0 + th + (s - 2);
-      }
+    if (tid < nhalo) {
       double V[5];
-      load_prim(a, hx, hy, k, V);
+      raw_to_prim(a, hr, hflip, V);
 #pragma unroll
-      for (int f = 0; f < 5; f++) sP[f][(hy - y0 + 2) * HX + (hx - x0 + 2)] = V[f];
+      for (int f = 0; f < 5; f++) sP[f][hpos] = V[f];
+      if (more) {
+        int hx, hy;
+        halo_cell(tid, x0, y0, tw, th, &hx, &hy);
+        load_raw(a, hx, hy, k + 1, hr, &hflip);
+      }
+    }
+    if (active) {
+      const unsigned ic = lin(a.n, x, y, k);
+#pragma unroll
+      for (int f = 0; f < 5; f++) ar[f] = ld(a.aux + f * a.fs, ic);
+      if (more) load_raw(a, x, y, k + 3, zn, &znflip);
     }
     __syncthreads();
 
-    // Limited slopes: x for cells x0-1 .. x0+tw, y for cells y0-1 .. y0+th.
-    for (int t = threadIdx.x; t < nsx + nsy; t += NT) {
-      int base, stride, c, out;
-      const Geo1D *g;
-      const bool isx = t < nsx;
+    // Slopes: own cell (x and y), then the tile-edge cells.
+    if (active) {
+      const int bx = ctr;
+#pragma unroll
+      for (int f = 0; f < 5; f++)
+        sSx[f][ty * SXW + tx + 1] =
+            slope_at(a, 0, x, sP[f][bx - 1], sP[f][bx], sP[f][bx + 1]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int f = 0; f < 5; f++)
+        sSy[f][(ty + 1) * TX + tx] =
+            slope_at(a, 1, y, sP[f][bx - HX], sP[f][bx], sP[f][bx + HX]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (tid < nxs + nys) {
+      int base, stride, c, out, gd;
+      bool isx = tid < nxs;
       if (isx) {
-        if (t >= (tw + 2) * th) continue;
-        const int i = t % (tw + 2), j = t / (tw + 2);
+        const int j = tid >> 1, side = tid & 1;
+        const int i = side ? tw + 1 : 0;  // slope index of cell x0-1+i
         base = (j + 2) * HX + (i + 1);
         stride = 1;
         c = x0 - 1 + i;
-        g = &gx;
+        gd = 0;
         out = j * SXW + i;
       } else {
-        const int u = t - nsx;
-        const int i = u % tw, j = u / tw;
+        const int u = tid - nxs;
+        const int i = u >> 1, side = u & 1;
+        const int j = side ? th + 1 : 0;  // slope index of cell y0-1+j
         base = (j + 1) * HX + (i + 2);
         stride = HX;
         c = y0 - 1 + j;
-        g = &gy;
+        gd = 1;
         out = j * TX + i;
       }
 #pragma unroll
       for (int f = 0; f < 5; f++) {
-        const double s = slope_at(*g, c, sP[f][base - stride], sP[f][base],
-                                  sP[f][base + stride], ph);
+        const double sv = slope_at(a, gd, c, sP[f][base - stride], sP[f][base],
+                                   sP[f][base + stride]);
         if (isx)
-          sSx[f][out] = s;
+          sSx[f][out] = sv;
         else
-          sSy[f][out] = s;
+          sSy[f][out] = sv;
       }
     }
     __syncthreads();
 
-    // Face fluxes: x faces x0 .. x0+tw, y faces y0 .. y0+th.  The x-task
-    // count is padded to whole waves so that every wave runs one direction.
-#pragma unroll 1
-    for (int t = threadIdx.x; t < nfx + nfy; t += NT) {
-      double WL[5], WR[5], F[5];
-      if (t < nfx) {
-        if (t >= (tw + 1) * th) continue;
-        const int i = t % (tw + 1), j = t / (tw + 1);
-        const int bl = (j + 2) * HX + (i + 1);
-        const double wr = gx.wr[x0 + i - 1 + 2], wl = gx.wl[x0 + i + 2];
-#pragma unroll
-        for (int f = 0; f < 5; f++) {
-          WL[f] = sP[f][bl] + wr * sSx[f][j * SXW + i];
-          WR[f] = sP[f][bl + 1] - wl * sSx[f][j * SXW + i + 1];
-        }
-        face_flux(WL, WR, 0, flag_bit(a, x0 + i, y0 + j, k, 0), ph, F);
-#pragma unroll
-        for (int f = 0; f < 5; f++) sFx[f][j * FXW + i] = F[f];
-      } else {
-        const int u = t - nfx;
-        const int i = u % tw, j = u / tw;
-        const int bl = (j + 1) * HX + (i + 2);
-        const double wr = gy.wr[y0 + j - 1 + 2], wl = gy.wl[y0 + j + 2];
-#pragma unroll
-        for (int f = 0; f < 5; f++) {
-          WL[f] = sP[f][bl] + wr * sSy[f][j * TX + i];
-          WR[f] = sP[f][bl + HX] - wl * sSy[f][(j + 1) * TX + i];
-        }
-        face_flux(WL, WR, 1, flag_bit(a, x0 + i, y0 + j, k, 1), ph, F);
-#pragma unroll
-        for (int f = 0; f < 5; f++) sFy[f][j * TX + i] = F[f];
-      }
+    // Faces: left and bottom faces of the own cell, then the right column of
+    // x-faces (wave 0) and the top row of y-faces (wave 1).
+    if (active) {
+      x_face(a, sP, sSx, sFx, tx, ty, x0, y0, k);
+      __builtin_amdgcn_sched_barrier(0);
+      y_face(a, sP, sSy, sFy, tx, ty, x0, y0, k);
+      __builtin_amdgcn_sched_barrier(0);
     }
+    if (tid < th) x_face(a, sP, sSx, sFx, tw, tid, x0, y0, k);
+    __builtin_amdgcn_sched_barrier(0);
+    if (tid >= WAVE && tid - WAVE < tw) y_face(a, sP, sSy, sFy, tid - WAVE, th, x0, y0, k);
     __syncthreads();
 
     if (active) {
+      // z stencil of plane k: slope at k+1, flux through the face k+1/2.
+      double Vk2[5], sk1[5], Fhi[5];
+      {
+        double WL[5], WR[5];
+        raw_to_prim(a, zr, zflip, Vk2);
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          const double vk = sP[f][ctr];
+          sk1[f] = slope_at(a, 2, k + 1, vk, Vk1[f], Vk2[f]);
+          WL[f] = vk + geo(a, 2, G_WR, k + 2) * sk[f];
+          WR[f] = Vk1[f] - geo(a, 2, G_WL, k + 1 + 2) * sk1[f];
+        }
+        face_flux(WL, WR, 2, flag_bit(a, x, y, k + 1, 2), ph, Fhi);
+      }
       // ComputeIncrement_CSL_Kernel (EV/Increments/IncrementDivergence_FV__
-      // Kernel.f90:672-752), dimensions in order 1, 2, 3, onto K = 0.
-      const double dX = gx.width[x + 2], dY = gy.width[y + 2], dZ = gz.width[k + 2];
+      // Kernel.f90:672-752), dimensions in order 1, 2, 3, onto K = 0; the
+      // division by the volume uses its reciprocal, rounded exactly.
+      const double dX = geo(a, 0, G_WIDTH, x + 2), dY = geo(a, 1, G_WIDTH, y + 2),
+                   dZ = geo(a, 2, G_WIDTH, k + 2);
       const double Vol = dX * dY * dZ;
+      const double rVol = 1.0 / Vol;
       const double A1 = dY * dZ, A2 = dZ * dX, A3 = dX * dY;
       const double dt = a.dt;
-      const long long ic = lin(a.n, x, y, k);
+      const unsigned ic = lin(a.n, x, y, k);
       double Kc[5];
 #pragma unroll
       for (int f = 0; f < 5; f++) {
         const double fxl = sFx[f][ty * FXW + tx], fxh = sFx[f][ty * FXW + tx + 1];
         const double fyl = sFy[f][ty * TX + tx], fyh = sFy[f][(ty + 1) * TX + tx];
         double K = 0.0;
-        K = K + dt * (A1 * fxl - A1 * fxh) / Vol;
-        K = K + dt * (A2 * fyl - A2 * fyh) / Vol;
-        K = K + dt * (A3 * Flo[f] - A3 * Fhi[f]) / Vol;
+        K = K + div_rcp(dt * (A1 * fxl - A1 * fxh), Vol, rVol);
+        K = K + div_rcp(dt * (A2 * fyl - A2 * fyh), Vol, rVol);
+        K = K + div_rcp(dt * (A3 * Flo[f] - A3 * Fhi[f]), Vol, rVol);
         Kc[f] = K;
       }
       if (STAGE == 1) {
@@ -296,19 +397,18 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
         // (Step_RK__Template.f90:240-278, MultiplyAdd_Command.f90:56-94).
 #pragma unroll
         for (int f = 0; f < 5; f++) {
-          const double u = a.aux.f[f][ic];
-          a.out1.f[f][ic] = u + 1.0 * Kc[f];
-          a.out2.f[f][ic] = u + 0.5 * Kc[f];
+          st(a.out1 + f * a.fs, ic, ar[f] + 1.0 * Kc[f]);
+          st(a.out2 + f * a.fs, ic, ar[f] + 0.5 * Kc[f]);
         }
       } else {
         // Solution = (Solution + 1/2 K1) + 1/2 K2, StoreSolution's conserved
         // fix, and the CFL term of the next step.
         double U[5], V[5];
 #pragma unroll
-        for (int f = 0; f < 5; f++) U[f] = a.aux.f[f][ic] + 0.5 * Kc[f];
+        for (int f = 0; f < 5; f++) U[f] = ar[f] + 0.5 * Kc[f];
         fix_conserved(U, ph.mass, ph.nmin, V);
 #pragma unroll
-        for (int f = 0; f < 5; f++) a.out1.f[f][ic] = U[f];
+        for (int f = 0; f < 5; f++) st(a.out1 + f * a.fs, ic, U[f]);
         const double tc = cfl_term(V, ph.gamma, ph.mass, dX, dY, dZ);
         tmax = tc > tmax ? tc : tmax;
       }
@@ -318,8 +418,13 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
         sk[f] = sk1[f];
         Vk[f] = Vk1[f];
         Vk1[f] = Vk2[f];
+        zr[f] = zn[f];
       }
+      zflip = znflip;
     }
+    // No barrier here: the next plane writes only the own centre and the
+    // halo cells, which nobody reads after the face barrier, and its first
+    // barrier orders the slope and flux buffers.
   }
 
   if (STAGE == 2) {
@@ -330,9 +435,9 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
       const double v = __shfl_xor(tmax, o, WAVE);
       tmax = v > tmax ? v : tmax;
     }
-    if ((threadIdx.x % WAVE) == 0) sRed[threadIdx.x / WAVE] = tmax;
+    if ((tid % WAVE) == 0) sRed[tid / WAVE] = tmax;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       double m = 0.0;
       for (int w = 0; w < NT / WAVE; w++) m = sRed[w] > m ? sRed[w] : m;
       atomicMax(a.invdt_bits, (unsigned long long)__double_as_longlong(m));
@@ -353,8 +458,8 @@ __global__ __launch_bounds__(256) void cfl_kernel(StageArgs a) {
     const int z = (int)(i / ((long long)a.n[0] * a.n[1]));
     double V[5];
     load_prim(a, x, y, z, V);
-    const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, a.geo[0].width[x + 2],
-                               a.geo[1].width[y + 2], a.geo[2].width[z + 2]);
+    const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, x + 2),
+                               geo(a, 1, G_WIDTH, y + 2), geo(a, 2, G_WIDTH, z + 2));
     tmax = tc > tmax ? tc : tmax;
   }
 #pragma unroll
@@ -451,7 +556,7 @@ __global__ __launch_bounds__(256) void current_kernel(StageArgs a, double c_v,
        i += (long long)gridDim.x * 256) {
     double U[5], V[5];
 #pragma unroll
-    for (int f = 0; f < 5; f++) U[f] = a.aux.f[f][i];
+    for (int f = 0; f < 5; f++) U[f] = a.aux[f * a.fs + i];
     fix_conserved(U, a.ph.mass, a.ph.nmin, V);
     double *o = out;
     const double g = a.ph.gamma, M = a.ph.mass;
@@ -484,22 +589,22 @@ __global__ __launch_bounds__(256) void current_kernel(StageArgs a, double c_v,
 // to the primitive field, then D, S, G.
 __global__ __launch_bounds__(256) void from_primitive_kernel(long long ntot,
                                                              double mass,
-                                                             Field5W P,
-                                                             Field5W U) {
+                                                             double *P,
+                                                             double *U) {
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
        i += (long long)gridDim.x * 256) {
-    double N = P.f[0][i];
+    double N = P[i];
     if (N < 0.0) {
       N = 0.0;
-      P.f[0][i] = N;
+      P[i] = N;
     }
-    const double V1 = P.f[1][i], V2 = P.f[2][i], V3 = P.f[3][i];
+    const double V1 = P[ntot + i], V2 = P[2 * ntot + i], V3 = P[3 * ntot + i];
     const double S1 = mass * N * V1, S2 = mass * N * V2, S3 = mass * N * V3;
-    U.f[0][i] = N;
-    U.f[1][i] = S1;
-    U.f[2][i] = S2;
-    U.f[3][i] = S3;
-    U.f[4][i] = P.f[4][i] + 0.5 * ((S1 * V1 + S2 * V2) + S3 * V3);
+    U[i] = N;
+    U[ntot + i] = S1;
+    U[2 * ntot + i] = S2;
+    U[3 * ntot + i] = S3;
+    U[4 * ntot + i] = P[4 * ntot + i] + 0.5 * ((S1 * V1 + S2 * V2) + S3 * V3);
   }
 }
 
@@ -514,10 +619,10 @@ __global__ __launch_bounds__(256) void totals_kernel(StageArgs a,
     const int x = (int)(i % a.n[0]);
     const int y = (int)((i / a.n[0]) % a.n[1]);
     const int z = (int)(i / ((long long)a.n[0] * a.n[1]));
-    const double vol = a.geo[0].width[x + 2] * a.geo[1].width[y + 2] *
-                       a.geo[2].width[z + 2];
+    const double vol = geo(a, 0, G_WIDTH, x + 2) * geo(a, 1, G_WIDTH, y + 2) *
+                       geo(a, 2, G_WIDTH, z + 2);
 #pragma unroll
-    for (int f = 0; f < 5; f++) acc[f] += a.aux.f[f][i] * vol;
+    for (int f = 0; f < 5; f++) acc[f] += a.aux[f * a.fs + i] * vol;
   }
 #pragma unroll
   for (int f = 0; f < 5; f++) s[f][threadIdx.x] = acc[f];
@@ -528,6 +633,36 @@ __global__ __launch_bounds__(256) void totals_kernel(StageArgs a,
     __syncthreads();
   }
   if (threadIdx.x < 5) partial[blockIdx.x * 5 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Self-test of div_rcp against the IEEE division on pseudo-random operands:
+// |a| in [2^-60, 2^60), b in [2^-40, 2^40), random signs and mantissas.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void division_selftest_kernel(
+    long long n, unsigned long long seed, unsigned long long *mismatches) {
+  unsigned long long bad = 0;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256) {
+    const unsigned long long r1 = mix64(seed ^ (2ull * i));
+    const unsigned long long r2 = mix64(seed ^ (2ull * i + 1));
+    const long long ea = (long long)(r1 >> 57) - 60;  // [-60, 67]
+    const long long eb = (long long)(r2 >> 58) - 32;  // [-32, 31]
+    const double ma = 1.0 + (double)(r1 & ((1ull << 52) - 1)) * 0x1p-52;
+    const double mb = 1.0 + (double)(r2 & ((1ull << 52) - 1)) * 0x1p-52;
+    double a = ldexp(ma, (int)ea), b = ldexp(mb, (int)eb);
+    if (r1 & (1ull << 52)) a = -a;
+    const double y = 1.0 / b;
+    const double q1 = div_rcp(a, b, y);
+    const double q2 = a / b;
+    if (__double_as_longlong(q1) != __double_as_longlong(q2)) bad++;
+  }
+  if (bad) atomicAdd(mismatches, bad);
 }
 
 }  // namespace gx
@@ -560,9 +695,10 @@ struct gx_ctx {
   hipStream_t stream = nullptr;
   int tz = 16;
   // device buffers
-  double *geo_buf = nullptr;  // 8 arrays x 3 dimensions x (nmax+4)
-  Geo1D geo[3];
-  double *U[5] = {}, *Y[5] = {}, *W[5] = {}, *P0[5] = {};
+  double *geo_buf = nullptr;  // 3 dimensions x N_GEO arrays x gs
+  int gs = 0;                 // geometry stride: max(n) + 4
+  // five-field states, field f at base + f * ncell
+  double *U = nullptr, *Y = nullptr, *W = nullptr, *P0 = nullptr;
   uint8_t *flags = nullptr, *bits = nullptr;
   unsigned long long *invdt = nullptr;
   double *partial = nullptr;
@@ -576,10 +712,10 @@ struct gx_ctx {
   hipEvent_t ev[5] = {};
   double kernel_ms[4] = {0, 0, 0, 0};
   long long kernel_n[4] = {0, 0, 0, 0};
-  std::vector<double> geo_host[3][8];
+  std::vector<double> geo_host[3][N_GEO];
 };
 
-static void geometry_1d(int n, double lo, double hi, std::vector<double> out[8]) {
+static void geometry_1d(int n, double lo, double hi, std::vector<double> out[N_GEO]) {
   // Same construction as the reference chart (Chart_Template.f90:463-593),
   // including single-precision ghost widths (real(KDL) at :477-478).
   const int g = 2;
@@ -596,7 +732,7 @@ static void geometry_1d(int n, double lo, double hi, std::vector<double> out[8])
   }
   const int m = n + 2 * g;
   std::vector<double> c(m);
-  for (int a = 0; a < 8; a++) out[a].assign(m, 0.0);
+  for (int a = 0; a < N_GEO; a++) out[a].assign(m, 0.0);
   for (int p = 0; p < m; p++) {
     const int ic = p - g + 1;
     c[p] = 0.5 * (E(ic) + E(ic + 1));
@@ -617,6 +753,9 @@ static void geometry_1d(int n, double lo, double hi, std::vector<double> out[8])
       out[6][p] = 1.0;
     }
     out[7][p] = out[0][p] + out[1][p];
+    out[8][p] = 1.0 / out[2][p];
+    out[9][p] = 1.0 / out[3][p];
+    out[10][p] = 1.0 / out[6][p];
   }
 }
 
@@ -633,7 +772,9 @@ static StageArgs base_args(const gx_ctx *c) {
   a.ph.st = c->params.ShockThreshold;
   a.ph.use_limiter = c->params.UseLimiter;
   a.ph.hllc = c->params.RiemannSolverType == GX_HLLC;
-  for (int d = 0; d < 3; d++) a.geo[d] = c->geo[d];
+  a.gs = c->gs;
+  a.fs = c->ncell;
+  a.geo = c->geo_buf;
   a.flags = c->flags;
   a.invdt_bits = c->invdt;
   return a;
@@ -663,18 +804,33 @@ void gx_default_params(gx_params *p) {
 
 const char *gx_last_error(void) { return g_last_error.c_str(); }
 
+int gx_self_test_division(long long n, unsigned long long seed,
+                          long long *mismatches) {
+  if (n < 0 || !mismatches) return fail(GX_E_ARG, "gx_self_test_division: bad argument");
+  unsigned long long *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
+  HIP_TRY(hipMemset(d, 0, sizeof(unsigned long long)));
+  division_selftest_kernel<<<4096, 256>>>(n, seed, d);
+  hipError_t e = hipGetLastError();
+  unsigned long long h = 0;
+  if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return fail(GX_E_HIP, std::string("gx_self_test_division: ") + hipGetErrorString(e));
+  *mismatches = (long long)h;
+  return GX_OK;
+}
+
 int gx_destroy(gx_ctx *c) {
   if (!c) return GX_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (int e = 0; e < 5; e++)
     if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
   (void)hipFree(c->geo_buf);
-  for (int f = 0; f < 5; f++) {
-    (void)hipFree(c->U[f]);
-    (void)hipFree(c->Y[f]);
-    (void)hipFree(c->W[f]);
-    (void)hipFree(c->P0[f]);
-  }
+  (void)hipFree(c->U);
+  (void)hipFree(c->Y);
+  (void)hipFree(c->W);
+  (void)hipFree(c->P0);
   (void)hipFree(c->flags);
   (void)hipFree(c->bits);
   (void)hipFree(c->invdt);
@@ -708,6 +864,13 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
     if (!(grid->MaxCoordinate[d] > grid->MinCoordinate[d]))
       return fail(GX_E_ARG, "gx_create: MaxCoordinate must exceed MinCoordinate");
   }
+  {
+    long long cells = 1;
+    for (int d = 0; d < 3; d++) cells *= grid->nCells[d];
+    // byte offsets within one field stay below 2^32
+    if (cells >= (1ll << 29))
+      return fail(GX_E_ARG, "gx_create: at most 2^29 - 1 cells per chart");
+  }
   gx_ctx *c = new (std::nothrow) gx_ctx();
   if (!c) return fail(GX_E_NOMEM, "gx_create: host allocation failed");
   c->grid = *grid;
@@ -727,35 +890,27 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
     }                                                                     \
   } while (0)
   TRY_CLEAN(hipStreamCreate(&c->stream));
-  size_t geo_len = 0;
+  c->gs = 0;
   for (int d = 0; d < 3; d++) {
     geometry_1d(c->n[d], grid->MinCoordinate[d], grid->MaxCoordinate[d], c->geo_host[d]);
-    geo_len += 8 * (size_t)(c->n[d] + 4);
+    c->gs = c->n[d] + 4 > c->gs ? c->n[d] + 4 : c->gs;
   }
+  const size_t geo_len = 3 * (size_t)N_GEO * c->gs;
   TRY_CLEAN(hipMalloc(&c->geo_buf, geo_len * sizeof(double)));
   {
-    std::vector<double> host(geo_len);
-    size_t off = 0;
-    for (int d = 0; d < 3; d++) {
-      const double *ptrs[8];
-      for (int a = 0; a < 8; a++) {
-        memcpy(&host[off], c->geo_host[d][a].data(), sizeof(double) * (c->n[d] + 4));
-        ptrs[a] = c->geo_buf + off;
-        off += c->n[d] + 4;
-      }
-      c->geo[d] = Geo1D{ptrs[0], ptrs[1], ptrs[2], ptrs[3],
-                        ptrs[4], ptrs[5], ptrs[6], ptrs[7]};
-    }
+    std::vector<double> host(geo_len, 1.0);
+    for (int d = 0; d < 3; d++)
+      for (int a = 0; a < N_GEO; a++)
+        memcpy(&host[((size_t)d * N_GEO + a) * c->gs], c->geo_host[d][a].data(),
+               sizeof(double) * (c->n[d] + 4));
     TRY_CLEAN(hipMemcpy(c->geo_buf, host.data(), geo_len * sizeof(double),
                         hipMemcpyHostToDevice));
   }
   const size_t bytes = sizeof(double) * (size_t)c->ncell;
-  for (int f = 0; f < 5; f++) {
-    TRY_CLEAN(hipMalloc(&c->U[f], bytes));
-    TRY_CLEAN(hipMalloc(&c->Y[f], bytes));
-    TRY_CLEAN(hipMalloc(&c->W[f], bytes));
-    TRY_CLEAN(hipMalloc(&c->P0[f], bytes));
-  }
+  TRY_CLEAN(hipMalloc(&c->U, 5 * bytes));
+  TRY_CLEAN(hipMalloc(&c->Y, 5 * bytes));
+  TRY_CLEAN(hipMalloc(&c->W, 5 * bytes));
+  TRY_CLEAN(hipMalloc(&c->P0, 5 * bytes));
   TRY_CLEAN(hipMalloc(&c->flags, (size_t)c->ncell));
   TRY_CLEAN(hipMalloc(&c->bits, (size_t)c->ncell));
   TRY_CLEAN(hipMemset(c->flags, 0, (size_t)c->ncell));
@@ -799,13 +954,9 @@ int gx_kernel_times(gx_ctx *c, double ms[4], long long count[4]) {
 
 int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
   if (!c || !primitive || !conserved) return fail(GX_E_ARG, "gx_set_state: null argument");
-  const size_t bytes = sizeof(double) * (size_t)c->ncell;
-  for (int f = 0; f < 5; f++) {
-    HIP_TRY(hipMemcpyAsync(c->P0[f], primitive + (size_t)f * c->ncell, bytes,
-                           hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->U[f], conserved + (size_t)f * c->ncell, bytes,
-                           hipMemcpyHostToDevice, c->stream));
-  }
+  const size_t bytes = 5 * sizeof(double) * (size_t)c->ncell;
+  HIP_TRY(hipMemcpyAsync(c->P0, primitive, bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->U, conserved, bytes, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->uploaded = true;
@@ -816,16 +967,10 @@ int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
 
 int gx_set_primitive(gx_ctx *c, const double *primitive) {
   if (!c || !primitive) return fail(GX_E_ARG, "gx_set_primitive: null argument");
-  const size_t bytes = sizeof(double) * (size_t)c->ncell;
-  Field5W P, U;
-  for (int f = 0; f < 5; f++) {
-    HIP_TRY(hipMemcpyAsync(c->P0[f], primitive + (size_t)f * c->ncell, bytes,
-                           hipMemcpyHostToDevice, c->stream));
-    P.f[f] = c->P0[f];
-    U.f[f] = c->U[f];
-  }
+  const size_t bytes = 5 * sizeof(double) * (size_t)c->ncell;
+  HIP_TRY(hipMemcpyAsync(c->P0, primitive, bytes, hipMemcpyHostToDevice, c->stream));
   from_primitive_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
-      c->ncell, c->params.BaryonMassReference, P, U);
+      c->ncell, c->params.BaryonMassReference, c->P0, c->U);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -870,10 +1015,8 @@ int gx_upload(gx_ctx *c, const double *value, long long ld, const int *map) {
 int gx_get_conserved(gx_ctx *c, double *conserved) {
   if (!c || !conserved) return fail(GX_E_ARG, "gx_get_conserved: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_get_conserved: no state uploaded");
-  const size_t bytes = sizeof(double) * (size_t)c->ncell;
-  for (int f = 0; f < 5; f++)
-    HIP_TRY(hipMemcpyAsync(conserved + (size_t)f * c->ncell, c->U[f], bytes,
-                           hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(conserved, c->U, 5 * sizeof(double) * (size_t)c->ncell,
+                         hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
 }
@@ -882,7 +1025,7 @@ static int run_current(gx_ctx *c) {
   if (!c->scratch)
     HIP_TRY(hipMalloc(&c->scratch, sizeof(double) * GX_N_FIELDS * (size_t)c->ncell));
   StageArgs a = base_args(c);
-  for (int f = 0; f < 5; f++) a.aux.f[f] = c->U[f];
+  a.aux = c->U;
   // c_v = k/(mu(gamma-1)) with the default gamma (Fluid_P_I__Form.f90:206-223)
   const double c_v = 1.0 / (1.0 * (1.4 - 1.0));
   current_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a, c_v, c->scratch);
@@ -895,9 +1038,8 @@ int gx_get_primitive(gx_ctx *c, double *primitive) {
   if (!c->uploaded) return fail(GX_E_STATE, "gx_get_primitive: no state uploaded");
   const size_t bytes = sizeof(double) * (size_t)c->ncell;
   if (c->prims_valid) {
-    for (int f = 0; f < 5; f++)
-      HIP_TRY(hipMemcpyAsync(primitive + (size_t)f * c->ncell, c->P0[f], bytes,
-                             hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(primitive, c->P0, 5 * bytes, hipMemcpyDeviceToHost,
+                           c->stream));
   } else {
     int rc = run_current(c);
     if (rc) return rc;
@@ -924,9 +1066,7 @@ int gx_download(gx_ctx *c, double *value, long long ld, const int *map) {
   if (c->prims_valid) {
     // The Current still holds the uploaded primitives.
     std::vector<double> p(5 * (size_t)c->ncell);
-    for (int f = 0; f < 5; f++)
-      HIP_TRY(hipMemcpy(p.data() + (size_t)f * c->ncell, c->P0[f],
-                        sizeof(double) * c->ncell, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(p.data(), c->P0, sizeof(double) * p.size(), hipMemcpyDeviceToHost));
     for (int f = 0; f < 5; f++)
       memcpy(host.data() + (size_t)PRIM_FIELDS[f] * c->ncell,
              p.data() + (size_t)f * c->ncell, sizeof(double) * c->ncell);
@@ -1009,7 +1149,7 @@ int gx_compute_time_step(gx_ctx *c, double *candidate) {
     HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
     StageArgs a = base_args(c);
     a.src_is_prim = c->prims_valid ? 1 : 0;
-    for (int f = 0; f < 5; f++) a.src.f[f] = c->prims_valid ? c->P0[f] : c->U[f];
+    a.src = c->prims_valid ? c->P0 : c->U;
     cfl_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a);
     HIP_TRY(hipGetLastError());
   }
@@ -1038,12 +1178,10 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   StageArgs a1 = base_args(c);
   a1.dt = dt;
   a1.src_is_prim = c->prims_valid ? 1 : 0;
-  for (int f = 0; f < 5; f++) {
-    a1.src.f[f] = c->prims_valid ? c->P0[f] : c->U[f];
-    a1.aux.f[f] = c->U[f];
-    a1.out1.f[f] = c->Y[f];
-    a1.out2.f[f] = c->W[f];
-  }
+  a1.src = c->prims_valid ? c->P0 : c->U;
+  a1.aux = c->U;
+  a1.out1 = c->Y;
+  a1.out2 = c->W;
   stage_kernel<1><<<grid, block, 0, c->stream>>>(a1);
   HIP_TRY(hipGetLastError());
   // Stage 2: Current = Y.
@@ -1051,17 +1189,15 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   StageArgs a2 = base_args(c);
   a2.dt = dt;
-  for (int f = 0; f < 5; f++) {
-    a2.src.f[f] = c->Y[f];
-    a2.aux.f[f] = c->W[f];
-    a2.out1.f[f] = c->U[f];
-  }
+  a2.src = c->Y;
+  a2.aux = c->W;
+  a2.out1 = c->U;
   stage_kernel<2><<<grid, block, 0, c->stream>>>(a2);
   HIP_TRY(hipGetLastError());
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
   // Shock flags of the next step from the new state.
   StageArgs a3 = base_args(c);
-  for (int f = 0; f < 5; f++) a3.src.f[f] = c->U[f];
+  a3.src = c->U;
   shock_bits_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a3, c->bits);
   HIP_TRY(hipGetLastError());
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
@@ -1095,7 +1231,7 @@ int gx_totals(gx_ctx *c, double out[5]) {
   if (!c || !out) return fail(GX_E_ARG, "gx_totals: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_totals: no state uploaded");
   StageArgs a = base_args(c);
-  for (int f = 0; f < 5; f++) a.aux.f[f] = c->U[f];
+  a.aux = c->U;
   totals_kernel<<<c->n_partial, 256, 0, c->stream>>>(a, c->partial);
   HIP_TRY(hipGetLastError());
   std::vector<double> h(5 * (size_t)c->n_partial);

@@ -20,16 +20,35 @@ constexpr int BC_REFLECTING = 2;
 
 // Per-index 1-D geometry of one dimension, padded index p = c + 2 for the
 // interior-based cell index c in [-2, n+2).
-struct Geo1D {
-  const double *wl;    // WidthLeft (Chart_Template.f90:985-987)
-  const double *wr;    // WidthRight
-  const double *dxl;   // Center(c) - Center(c-1) (Difference_Kernel)
-  const double *dxr;   // Center(c+1) - Center(c)
-  const double *dxl2;  // dxl*dxl
-  const double *dxr2;  // dxr*dxr
-  const double *den;   // (dxl*dxr)*(dxl+dxr)
-  const double *width; // WidthLeft + WidthRight (GeometryFlat_Kernel.f90:21-23)
+enum GeoArray {
+  G_WL = 0,  // WidthLeft (Chart_Template.f90:985-987)
+  G_WR,      // WidthRight
+  G_DXL,     // Center(c) - Center(c-1) (Difference_Kernel)
+  G_DXR,     // Center(c+1) - Center(c)
+  G_DXL2,    // dxl*dxl
+  G_DXR2,    // dxr*dxr
+  G_DEN,     // (dxl*dxr)*(dxl+dxr)
+  G_WIDTH,   // WidthLeft + WidthRight (GeometryFlat_Kernel.f90:21-23)
+  G_RDXL,    // RN(1/dxl)
+  G_RDXR,    // RN(1/dxr)
+  G_RDEN,    // RN(1/den)
+  N_GEO
 };
+
+// a/b rounded to nearest, given y = RN(1/b): q0 = RN(a y) is corrected twice
+// with the exact-remainder fma; after the first correction q is within one
+// ulp of a/b, so the remainder of the second is exact and its correction
+// rounds to RN(a/b) (Markstein, IBM J. Res. Dev. 34 (1990) 111-119).  The
+// quotient, a and the remainders must be normal: callers keep b in
+// [2^-500, 2^500] and route |a| < 2^-900 to the IEEE division.
+__device__ __forceinline__ double div_rcp(double a, double b, double y) {
+  if (fabs(a) < 0x1p-900) return a / b;
+  double q = a * y;
+  double r = fma(-b, q, a);
+  q = fma(r, y, q);
+  r = fma(-b, q, a);
+  return fma(r, y, q);
+}
 
 struct Physics {
   double gamma, theta, mass, nmin, st;
@@ -42,10 +61,18 @@ __device__ __forceinline__ void prim_from_cons(const double U[5], double mass,
                                                double nmin, double V[5]) {
   const double D = U[0];
   if (D > nmin) {
+    const double MD = mass * D;
     V[0] = D;
-    V[1] = U[1] / (mass * D);
-    V[2] = U[2] / (mass * D);
-    V[3] = U[3] / (mass * D);
+    if (MD > 0x1p-500 && MD < 0x1p500) {
+      const double y = 1.0 / MD;
+      V[1] = div_rcp(U[1], MD, y);
+      V[2] = div_rcp(U[2], MD, y);
+      V[3] = div_rcp(U[3], MD, y);
+    } else {
+      V[1] = U[1] / MD;
+      V[2] = U[2] / MD;
+      V[3] = U[3] / MD;
+    }
     V[4] = U[4] - 0.5 * ((U[1] * V[1] + U[2] * V[2]) + U[3] * V[3]);
   } else {
     V[0] = nmin;
@@ -59,16 +86,19 @@ __device__ __forceinline__ void prim_from_cons(const double U[5], double mass,
 
 // Gradient_Kernel limited slope (OP/Calculus/Derivatives/Gradient_Kernel.f90:
 // 166-190), or the unlimited centred slope when UseLimiter is false.
+// The three divisions use the precomputed reciprocals of the geometry.
 __device__ __forceinline__ double slope(double vm, double v0, double vp,
                                         double dxl, double dxr, double dxl2,
-                                        double dxr2, double den, double theta,
+                                        double dxr2, double den, double rdxl,
+                                        double rdxr, double rden, double theta,
                                         int use_limiter) {
   const double dVL = v0 - vm;
   const double dVR = vp - v0;
-  const double central = (dxr2 * dVL + dxl2 * dVR) / den;
+  const double central = div_rcp(dxr2 * dVL + dxl2 * dVR, den, rden);
   if (!use_limiter) return central;
   return (copysign(0.5, dVL) + copysign(0.5, dVR)) *
-         fmin(fmin(fabs(theta * dVL / dxl), fabs(theta * dVR / dxr)),
+         fmin(fmin(fabs(div_rcp(theta * dVL, dxl, rdxl)),
+                   fabs(div_rcp(theta * dVR, dxr, rdxr))),
               fabs(central));
 }
 

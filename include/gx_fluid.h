@@ -180,6 +180,13 @@ void *gx_stream(gx_ctx *ctx);
 int gx_set_profiling(gx_ctx *ctx, int on);
 int gx_kernel_times(gx_ctx *ctx, double ms[4], long long count[4]);
 
+/* Device self-test of the exactly rounded reciprocal division used for the
+ * divisions by geometry constants and cell volumes: n pseudo-random operand
+ * pairs, *mismatches = number of quotients whose bits differ from the IEEE
+ * division.  Synchronises. */
+int gx_self_test_division(long long n, unsigned long long seed,
+                          long long *mismatches);
+
 const char *gx_last_error(void);
 
 #ifdef __cplusplus

@@ -120,3 +120,13 @@ def test_primitive_and_totals():
     tot = g.totals()
     ref = np.array([np.sum(u[f] * vol) for f in range(5)])
     np.testing.assert_allclose(tot, ref, rtol=1e-12, atol=1e-15)
+
+
+def test_reciprocal_division_is_exactly_rounded():
+    import ctypes
+    from genasis_amd import _capi
+    lib = _capi.load()
+    bad = ctypes.c_longlong(-1)
+    _capi.check(lib.gx_self_test_division(1 << 28, 12345, ctypes.byref(bad)),
+                "gx_self_test_division")
+    assert bad.value == 0
