@@ -177,19 +177,31 @@ __device__ __forceinline__ void halo_cell(int t, int x0, int y0, int tw, int th,
   }
 }
 
+// Geometry of the tile in LDS: array g of dimension d (0 = x, 1 = y) at the
+// local index c - origin + 2 for c in [origin-2, origin+TX+2).
+typedef double TileGeo[2][N_GEO][HX];
+
+__device__ __forceinline__ double slope_lds(const TileGeo &sG, int d, int l,
+                                            double vm, double v0, double vp,
+                                            const Physics &ph) {
+  return slope(vm, v0, vp, sG[d][G_DXL][l], sG[d][G_DXR][l], sG[d][G_DXL2][l],
+               sG[d][G_DXR2][l], sG[d][G_DEN][l], sG[d][G_RDXL][l],
+               sG[d][G_RDXR][l], sG[d][G_RDEN][l], ph.theta, ph.use_limiter);
+}
+
 // x-face i (between cells x0+i-1 and x0+i) of row j into sFx.
 template <typename SP, typename SS, typename SF>
 __device__ __forceinline__ void x_face(const StageArgs &a, SP sP, SS sSx, SF sFx,
-                                       int i, int j, int x0, int y0, int k) {
+                                       const TileGeo &sG, int i, int j, bool df) {
   const int bl = (j + 2) * HX + (i + 1);
-  const double wr = geo(a, 0, G_WR, x0 + i - 1 + 2), wl = geo(a, 0, G_WL, x0 + i + 2);
+  const double wr = sG[0][G_WR][i + 1], wl = sG[0][G_WL][i + 2];
   double WL[5], WR[5], F[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     WL[f] = sP[f][bl] + wr * sSx[f][j * SXW + i];
     WR[f] = sP[f][bl + 1] - wl * sSx[f][j * SXW + i + 1];
   }
-  face_flux(WL, WR, 0, flag_bit(a, x0 + i, y0 + j, k, 0), a.ph, F);
+  face_flux(WL, WR, 0, df, a.ph, F);
 #pragma unroll
   for (int f = 0; f < 5; f++) sFx[f][j * FXW + i] = F[f];
 }
@@ -197,16 +209,16 @@ __device__ __forceinline__ void x_face(const StageArgs &a, SP sP, SS sSx, SF sFx
 // y-face j (between rows y0+j-1 and y0+j) of column i into sFy.
 template <typename SP, typename SS, typename SF>
 __device__ __forceinline__ void y_face(const StageArgs &a, SP sP, SS sSy, SF sFy,
-                                       int i, int j, int x0, int y0, int k) {
+                                       const TileGeo &sG, int i, int j, bool df) {
   const int bl = (j + 1) * HX + (i + 2);
-  const double wr = geo(a, 1, G_WR, y0 + j - 1 + 2), wl = geo(a, 1, G_WL, y0 + j + 2);
+  const double wr = sG[1][G_WR][j + 1], wl = sG[1][G_WL][j + 2];
   double WL[5], WR[5], F[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     WL[f] = sP[f][bl] + wr * sSy[f][j * TX + i];
     WR[f] = sP[f][bl + HX] - wl * sSy[f][(j + 1) * TX + i];
   }
-  face_flux(WL, WR, 1, flag_bit(a, x0 + i, y0 + j, k, 1), a.ph, F);
+  face_flux(WL, WR, 1, df, a.ph, F);
 #pragma unroll
   for (int f = 0; f < 5; f++) sFy[f][j * TX + i] = F[f];
 }
@@ -227,12 +239,19 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
   __shared__ double sFx[5][TY * FXW];       // x-face fluxes x0 .. x0+tw
   __shared__ double sFy[5][(TY + 1) * TX];  // y-face fluxes y0 .. y0+th
   __shared__ double sRed[NT / WAVE];
+  __shared__ TileGeo sG;
 
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * a.tz;
   const int tw = min(TX, nx - x0), th = min(TY, ny - y0);
   const int zend = min(z0 + a.tz, nz);
   const int tid = threadIdx.x;
+  // Tile geometry: x for cells x0-2 .. x0+tw+1, y for y0-2 .. y0+th+1.
+  for (int t = tid; t < 2 * N_GEO * HX; t += NT) {
+    const int d = t / (N_GEO * HX), g = (t / HX) % N_GEO, l = t % HX;
+    const int o = d == 0 ? x0 : y0, w = d == 0 ? tw : th;
+    sG[d][g][l] = l < w + 4 ? geo(a, d, g, o + l) : 0.0;
+  }
   const int tx = tid % TX, ty = tid / TX;
   const bool active = tx < tw && ty < th;
   const int x = x0 + tx, y = y0 + ty;
@@ -244,6 +263,7 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
   double Vk[5], Vk1[5], sk[5], Flo[5];
   double zr[5], hr[5];  // prefetched raw values: z-stencil k+2, halo of k
   int zflip = -1, hflip = -1, hpos = 0;
+  unsigned flk = 0;     // flag byte of the own cell at plane k (then k+1)
   double tmax = 0.0;
 
   if (active) {
@@ -253,6 +273,7 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
     load_prim(a, x, y, z0, Vk);
     load_prim(a, x, y, z0 + 1, Vk1);
     load_raw(a, x, y, z0 + 2, zr, &zflip);
+    flk = ld(a.flags, lin(a.n, x, y, z0));
 #pragma unroll
     for (int f = 0; f < 5; f++) {
       sm1[f] = slope_at(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f]);
@@ -260,7 +281,7 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
       WL[f] = Vm1[f] + geo(a, 2, G_WR, z0 - 1 + 2) * sm1[f];
       WR[f] = Vk[f] - geo(a, 2, G_WL, z0 + 2) * sk[f];
     }
-    face_flux(WL, WR, 2, flag_bit(a, x, y, z0, 2), ph, Flo);
+    face_flux(WL, WR, 2, (flk >> 2) & 1, ph, Flo);
   }
   if (tid < nhalo) {
     int hx, hy;
@@ -273,6 +294,7 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
     const bool more = k + 1 < zend;
     double ar[5], zn[5];
     int znflip = -1;
+    unsigned fln = 0, flx = 0;  // own flags at k+1; flag of this thread's edge face
     if (active) {
 #pragma unroll
       for (int f = 0; f < 5; f++) sP[f][ctr] = Vk[f];
@@ -293,6 +315,12 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
 #pragma unroll
       for (int f = 0; f < 5; f++) ar[f] = ld(a.aux + f * a.fs, ic);
       if (more) load_raw(a, x, y, k + 3, zn, &znflip);
+      if (k + 1 < nz) fln = ld(a.flags, lin(a.n, x, y, k + 1));
+    }
+    if (tid < th) {
+      if (x0 + tw < nx) flx = ld(a.flags, lin(a.n, x0 + tw, y0 + tid, k)) & 1u;
+    } else if (tid >= WAVE && tid - WAVE < tw) {
+      if (y0 + th < ny) flx = (ld(a.flags, lin(a.n, x0 + tid - WAVE, y0 + th, k)) >> 1) & 1u;
     }
     __syncthreads();
 
@@ -302,23 +330,23 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
 #pragma unroll
       for (int f = 0; f < 5; f++)
         sSx[f][ty * SXW + tx + 1] =
-            slope_at(a, 0, x, sP[f][bx - 1], sP[f][bx], sP[f][bx + 1]);
+            slope_lds(sG, 0, tx + 2, sP[f][bx - 1], sP[f][bx], sP[f][bx + 1], ph);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int f = 0; f < 5; f++)
         sSy[f][(ty + 1) * TX + tx] =
-            slope_at(a, 1, y, sP[f][bx - HX], sP[f][bx], sP[f][bx + HX]);
+            slope_lds(sG, 1, ty + 2, sP[f][bx - HX], sP[f][bx], sP[f][bx + HX], ph);
       __builtin_amdgcn_sched_barrier(0);
     }
     if (tid < nxs + nys) {
-      int base, stride, c, out, gd;
+      int base, stride, l, out, gd;
       bool isx = tid < nxs;
       if (isx) {
         const int j = tid >> 1, side = tid & 1;
         const int i = side ? tw + 1 : 0;  // slope index of cell x0-1+i
         base = (j + 2) * HX + (i + 1);
         stride = 1;
-        c = x0 - 1 + i;
+        l = i + 1;  // local geometry index of cell x0-1+i
         gd = 0;
         out = j * SXW + i;
       } else {
@@ -327,14 +355,14 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
         const int j = side ? th + 1 : 0;  // slope index of cell y0-1+j
         base = (j + 1) * HX + (i + 2);
         stride = HX;
-        c = y0 - 1 + j;
+        l = j + 1;
         gd = 1;
         out = j * TX + i;
       }
 #pragma unroll
       for (int f = 0; f < 5; f++) {
-        const double sv = slope_at(a, gd, c, sP[f][base - stride], sP[f][base],
-                                   sP[f][base + stride]);
+        const double sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
+                                    sP[f][base + stride], ph);
         if (isx)
           sSx[f][out] = sv;
         else
@@ -346,14 +374,14 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
     // Faces: left and bottom faces of the own cell, then the right column of
     // x-faces (wave 0) and the top row of y-faces (wave 1).
     if (active) {
-      x_face(a, sP, sSx, sFx, tx, ty, x0, y0, k);
+      x_face(a, sP, sSx, sFx, sG, tx, ty, flk & 1u);
       __builtin_amdgcn_sched_barrier(0);
-      y_face(a, sP, sSy, sFy, tx, ty, x0, y0, k);
+      y_face(a, sP, sSy, sFy, sG, tx, ty, (flk >> 1) & 1u);
       __builtin_amdgcn_sched_barrier(0);
     }
-    if (tid < th) x_face(a, sP, sSx, sFx, tw, tid, x0, y0, k);
+    if (tid < th) x_face(a, sP, sSx, sFx, sG, tw, tid, flx);
     __builtin_amdgcn_sched_barrier(0);
-    if (tid >= WAVE && tid - WAVE < tw) y_face(a, sP, sSy, sFy, tid - WAVE, th, x0, y0, k);
+    if (tid >= WAVE && tid - WAVE < tw) y_face(a, sP, sSy, sFy, sG, tid - WAVE, th, flx);
     __syncthreads();
 
     if (active) {
@@ -369,12 +397,12 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
           WL[f] = vk + geo(a, 2, G_WR, k + 2) * sk[f];
           WR[f] = Vk1[f] - geo(a, 2, G_WL, k + 1 + 2) * sk1[f];
         }
-        face_flux(WL, WR, 2, flag_bit(a, x, y, k + 1, 2), ph, Fhi);
+        face_flux(WL, WR, 2, (fln >> 2) & 1u, ph, Fhi);
       }
       // ComputeIncrement_CSL_Kernel (EV/Increments/IncrementDivergence_FV__
       // Kernel.f90:672-752), dimensions in order 1, 2, 3, onto K = 0; the
       // division by the volume uses its reciprocal, rounded exactly.
-      const double dX = geo(a, 0, G_WIDTH, x + 2), dY = geo(a, 1, G_WIDTH, y + 2),
+      const double dX = sG[0][G_WIDTH][tx + 2], dY = sG[1][G_WIDTH][ty + 2],
                    dZ = geo(a, 2, G_WIDTH, k + 2);
       const double Vol = dX * dY * dZ;
       const double rVol = 1.0 / Vol;
@@ -421,6 +449,7 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
         zr[f] = zn[f];
       }
       zflip = znflip;
+      flk = fln;
     }
     // No barrier here: the next plane writes only the own centre and the
     // halo cells, which nobody reads after the face barrier, and its first
