@@ -13,6 +13,9 @@ kernel) against HBM: algorithmic bytes per launch (120 B per cell: read Y and
 U + K1/2, write U_new, FP64, five fields) over its average duration from HIP
 events in the timed region.  ``cpu_baseline`` times the oracle (the CPU
 restatement of the reference step) on the host cores on a bounded sample.
+``fp32`` runs the same seeded problem at 512^3 for 200 steps with the state
+and sweeps in FP32 and in FP64, with the same time steps, and reports the
+speed ratio and the per-field L1 / Linf error of FP32 against FP64.
 """
 from __future__ import annotations
 
@@ -70,6 +73,62 @@ def cpu_baseline(n: int, steps: int, threads: int):
     return chart.n_proper * steps / (t1 - t0)
 
 
+def _ramped(candidate: float, cycle: int, n_ramp: int = 100) -> float:
+    """Integrator_C_PS time-step ramp (min((iCycle+1)/nRampCycles, 1))."""
+    ramp = min(float(cycle + 1) / n_ramp, 1.0)
+    return ramp * candidate if ramp < 1.0 else candidate
+
+
+def precision_check(n: int, steps: int, z_chunk: int):
+    """FP64 and FP32 GPU runs of the seeded Kelvin-Helmholtz problem with the
+    FP64 run's time steps; speed and per-field error of FP32 against FP64."""
+    from genasis_amd import problems
+    from genasis_amd.chart import Chart
+    from genasis_amd.compare import field_errors
+    from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
+
+    chart = Chart((n, n, n))
+    prim = problems.kelvin_helmholtz(chart, seed=1234, padded=False)
+    dts, runs, states = [], {}, {}
+    for prec in ("FP64", "FP32"):
+        st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, Precision=prec))
+        st.set_tuning(z_chunk)
+        st.set_primitive(*prim)
+        st.set_profiling(True)
+        st.sync()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            c = st.time_step_candidate()
+            if prec == "FP64":
+                dts.append(_ramped(c, s))
+            st.step(0.0, dts[s])
+        st.sync()
+        t1 = time.perf_counter()
+        kt = st.kernel_times()
+        runs[prec] = {"ms_per_step": (t1 - t0) / steps * 1e3,
+                      "kernel_ms_per_step": sum(v[0] for v in kt.values()) / steps,
+                      "stage2_ms": kt["stage2"][0] / max(kt["stage2"][1], 1)}
+        states[prec] = st.conserved()
+        st.close()
+        print(f"# precision check {prec}: {runs[prec]['ms_per_step']:.3f} ms/step",
+              file=sys.stderr, flush=True)
+    del prim
+    err = field_errors(states["FP32"], states["FP64"])
+    return {
+        "workload": f"Kelvin-Helmholtz {n}^3 periodic, seed 1234, {steps} RK2 steps, "
+                    "same time steps (FP64 CFL) for both runs",
+        "value": chart.n_proper / (runs["FP32"]["ms_per_step"] * 1e-3),
+        "unit": "cell-updates/s",
+        "speedup_vs_fp64": runs["FP64"]["ms_per_step"] / runs["FP32"]["ms_per_step"],
+        "kernel_speedup_vs_fp64": runs["FP64"]["kernel_ms_per_step"]
+        / runs["FP32"]["kernel_ms_per_step"],
+        "fp64": runs["FP64"],
+        "fp32": runs["FP32"],
+        "density_L1": err["D"]["L1"],
+        "errors_vs_fp64": err,
+    }
+
+
 def copy_calibration():
     """A plain 1 GiB device copy (torch), for scaling the PMC byte counters."""
     import torch
@@ -95,6 +154,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--precision", choices=("FP64", "FP32"), default="FP64")
+    ap.add_argument("--fp32-check", choices=("auto", "off"), default="auto")
+    ap.add_argument("--fp32-n", type=int, default=512)
+    ap.add_argument("--fp32-steps", type=int, default=200)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -112,10 +175,11 @@ def main():
 
     n = args.n
     chart = Chart((n, n, n))
-    params = FluidParameters(CourantFactor=0.8)
+    params = FluidParameters(CourantFactor=0.8, Precision=args.precision)
+    esz = 8 if args.precision == "FP64" else 4
     stepper = Step_RK2_C_ASC(chart, params)
     stepper.set_tuning(args.z_chunk)
-    stepper.set_primitive(*problems.kelvin_helmholtz(chart, seed=1234))
+    stepper.set_primitive(*problems.kelvin_helmholtz(chart, seed=1234, padded=False))
     total0 = stepper.totals()
     integ = Integrator_C_PS(stepper, IntegratorParameters(
         FinishTime=1.0e30, FinishCycle=args.warmup + args.steps, nWrite=1))
@@ -139,9 +203,17 @@ def main():
     value = chart.n_proper * args.steps / elapsed
     s2_ms, s2_n = kt["stage2"]
     s2_avg = s2_ms / max(s2_n, 1) * 1e-3
-    achieved = ALG_BYTES_STAGE2 * chart.n_proper / s2_avg / 1e9
+    alg_stage2 = ALG_BYTES_STAGE2 * esz / 8
+    alg_step = ALG_BYTES_STEP * esz / 8
+    achieved = alg_stage2 * chart.n_proper / s2_avg / 1e9
     step_kernel_s = sum(v[0] for v in kt.values()) / max(s2_n, 1) * 1e-3
-    traffic = _read_traffic()
+    traffic = _read_traffic() if args.precision == "FP64" else None
+    stepper.close()
+    print(f"# main run: {elapsed / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+
+    fp32 = None
+    if args.fp32_check == "auto":
+        fp32 = precision_check(args.fp32_n, args.fp32_steps, args.z_chunk)
 
     cpu = None
     if args.cpu_baseline == "auto":
@@ -165,21 +237,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "FP64" else "f32",
         "data": "synthetic (seeded Kelvin-Helmholtz initial data)",
         "config": {"workload": f"Kelvin-Helmholtz {n}^3 periodic, PLM(MC theta=1.4)"
-                               f"+HLLC, RK2, CFL 0.8, FP64",
+                               f"+HLLC, RK2, CFL 0.8, {args.precision}",
                    "cells": chart.n_proper, "z_chunk": args.z_chunk},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},
-        "step_roofline": {"alg_bytes_per_cell_update": ALG_BYTES_STEP,
+        "step_roofline": {"alg_bytes_per_cell_update": alg_step,
                           "kernel_s_per_step": step_kernel_s,
-                          "frac": ALG_BYTES_STEP * chart.n_proper / step_kernel_s
+                          "frac": alg_step * chart.n_proper / step_kernel_s
                           / 1e9 / HBM_PEAK_GBS},
         "kernel_ms_avg": {k: v[0] / max(v[1], 1) for k, v in kt.items()},
         "conserved_totals_change": [float(a - b) for a, b in zip(total1, total0)],
         "cpu_baseline": cpu,
+        "fp32": fp32,
+        "adjoint": None,
     }
     print(json.dumps(out))
 

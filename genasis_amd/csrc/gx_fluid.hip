@@ -39,7 +39,10 @@ __host__ __device__ inline int round_up(int v, int m) { return (v + m - 1) / m *
 
 // Field f of a five-field state lives at base + f * fs (fs = cells).  The
 // geometry is one buffer: array g of dimension d at geo + (d*N_GEO + g)*gs.
-// Few pointers keep the kernel's scalar registers free.
+// Few pointers keep the kernel's scalar registers free.  R is the real type
+// of the state and the sweep arithmetic (double, or float for the FP32
+// variant); physics constants stay double and are converted on use.
+template <typename R>
 struct StageArgs {
   int n[3];
   int bc[6];
@@ -49,14 +52,19 @@ struct StageArgs {
   long long fs;         // field stride (cells)
   double dt;
   Physics ph;
-  const double *src;    // stage input: conserved (or primitive) state
-  const double *aux;    // stage 1: U (Solution); stage 2: W = U + K1/2
-  double *out1;         // stage 1: Y = U + K1; stage 2: U_new
-  double *out2;         // stage 1: W
-  const double *geo;
+  const R *src;         // stage input: conserved (or primitive) state
+  const R *aux;         // stage 1: U (Solution); stage 2: W = U + K1/2
+  R *out1;              // stage 1: Y = U + K1; stage 2: U_new
+  R *out2;              // stage 1: W
+  const R *geo;
   const uint8_t *flags; // DiffusiveFlux bits of the step
-  unsigned long long *invdt_bits;  // stage 2: max CFL term (as bits)
+  unsigned long long *invdt_bits;  // stage 2: max CFL term (as FP64 bits)
 };
+
+// Waves per SIMD the stage kernel is compiled for: the FP32 tile uses half
+// the registers and LDS of the FP64 one.
+template <typename R> struct StageOccupancy { static constexpr int waves = 2; };
+template <> struct StageOccupancy<float> { static constexpr int waves = 3; };
 
 // Loads and stores at a 32-bit element offset from a base pointer: the byte
 // offset stays 32-bit (charts hold < 2^29 cells, checked at creation), so the
@@ -70,7 +78,8 @@ __device__ __forceinline__ void st(T *base, unsigned idx, T v) {
   *(T *)((char *)base + (size_t)(idx * (unsigned)sizeof(T))) = v;
 }
 
-__device__ __forceinline__ double geo(const StageArgs &a, int d, int g, int p) {
+template <typename R>
+__device__ __forceinline__ R geo(const StageArgs<R> &a, int d, int g, int p) {
   return ld(a.geo + (d * N_GEO + g) * a.gs, (unsigned)p);
 }
 
@@ -98,8 +107,9 @@ __device__ __forceinline__ int map_coord(int c, int n, int bc_in, int bc_out,
 // Linear index of cell (x, y, z) with at most one coordinate outside the
 // chart by at most two cells, after the ghost rule of that face; *flip = the
 // dimension whose velocity a reflecting face negates, or -1.
-__device__ __forceinline__ unsigned cell_index(const StageArgs &a, int x, int y,
-                                               int z, int *flip) {
+template <typename A>
+__device__ __forceinline__ unsigned cell_index(const A &a, int x, int y, int z,
+                                               int *flip) {
   *flip = -1;
   if ((unsigned)x < (unsigned)a.n[0] && (unsigned)y < (unsigned)a.n[1] &&
       (unsigned)z < (unsigned)a.n[2])
@@ -113,52 +123,56 @@ __device__ __forceinline__ unsigned cell_index(const StageArgs &a, int x, int y,
 }
 
 // Raw stage input (conserved, or primitive after an upload) of one cell.
-__device__ __forceinline__ void load_raw(const StageArgs &a, int x, int y, int z,
-                                         double R[5], int *flip) {
+template <typename R>
+__device__ __forceinline__ void load_raw(const StageArgs<R> &a, int x, int y, int z,
+                                         R raw[5], int *flip) {
   const unsigned i = cell_index(a, x, y, z, flip);
 #pragma unroll
-  for (int f = 0; f < 5; f++) R[f] = ld(a.src + f * a.fs, i);
+  for (int f = 0; f < 5; f++) raw[f] = ld(a.src + f * a.fs, i);
+}
+
+// -v when f, as a sign-bit operation: a select on an array element is turned
+// into an indexed (scratch) store by the compiler.
+__device__ __forceinline__ double negate_if(double v, bool f) {
+  return __longlong_as_double(__double_as_longlong(v) ^
+                              (f ? (long long)0x8000000000000000ull : 0ll));
+}
+__device__ __forceinline__ float negate_if(float v, bool f) {
+  return __int_as_float(__float_as_int(v) ^ (f ? (int)0x80000000u : 0));
 }
 
 // Primitives of a raw value, with the reflecting sign flip.
-__device__ __forceinline__ void raw_to_prim(const StageArgs &a, const double R[5],
-                                            int flip, double V[5]) {
+template <typename R>
+__device__ __forceinline__ void raw_to_prim(const StageArgs<R> &a, const R raw[5],
+                                            int flip, R V[5]) {
   if (a.src_is_prim) {
 #pragma unroll
-    for (int f = 0; f < 5; f++) V[f] = R[f];
+    for (int f = 0; f < 5; f++) V[f] = raw[f];
   } else {
-    prim_from_cons(R, a.ph.mass, a.ph.nmin, V);
+    prim_from_cons(raw, R(a.ph.mass), R(a.ph.nmin), V);
   }
-  // Sign flips as bit operations: a select on an array element here is
-  // turned into an indexed (scratch) store by the compiler.
-  const long long sbit = 0x8000000000000000ll;
-  V[1] = __longlong_as_double(__double_as_longlong(V[1]) ^ (flip == 0 ? sbit : 0ll));
-  V[2] = __longlong_as_double(__double_as_longlong(V[2]) ^ (flip == 1 ? sbit : 0ll));
-  V[3] = __longlong_as_double(__double_as_longlong(V[3]) ^ (flip == 2 ? sbit : 0ll));
+  V[1] = negate_if(V[1], flip == 0);
+  V[2] = negate_if(V[2], flip == 1);
+  V[3] = negate_if(V[3], flip == 2);
 }
 
-__device__ __forceinline__ void load_prim(const StageArgs &a, int x, int y,
-                                          int z, double V[5]) {
-  double R[5];
+template <typename R>
+__device__ __forceinline__ void load_prim(const StageArgs<R> &a, int x, int y,
+                                          int z, R V[5]) {
+  R raw[5];
   int flip;
-  load_raw(a, x, y, z, R, &flip);
-  raw_to_prim(a, R, flip, V);
+  load_raw(a, x, y, z, raw, &flip);
+  raw_to_prim(a, raw, flip, V);
 }
 
-__device__ __forceinline__ double slope_at(const StageArgs &a, int d, int c,
-                                           double vm, double v0, double vp) {
-  const double *g = a.geo + d * N_GEO * a.gs + (c + 2);
+template <typename R>
+__device__ __forceinline__ R slope_at(const StageArgs<R> &a, int d, int c, R vm,
+                                      R v0, R vp) {
+  const R *g = a.geo + d * N_GEO * a.gs + (c + 2);
   const int s = a.gs;
   return slope(vm, v0, vp, g[G_DXL * s], g[G_DXR * s], g[G_DXL2 * s], g[G_DXR2 * s],
                g[G_DEN * s], g[G_RDXL * s], g[G_RDXR * s], g[G_RDEN * s],
-               a.ph.theta, a.ph.use_limiter);
-}
-
-__device__ __forceinline__ bool flag_bit(const StageArgs &a, int x, int y,
-                                         int z, int d) {
-  const int c[3] = {x, y, z};
-  if (c[d] >= a.n[d]) return false;  // ghost face: cleared in the reference
-  return (ld(a.flags, lin(a.n, x, y, z)) >> d) & 1;
+               R(a.ph.theta), a.ph.use_limiter);
 }
 
 // Halo cell t of the plane tile: 4 per row for x (two each side), 4 per
@@ -179,46 +193,46 @@ __device__ __forceinline__ void halo_cell(int t, int x0, int y0, int tw, int th,
 
 // Geometry of the tile in LDS: array g of dimension d (0 = x, 1 = y) at the
 // local index c - origin + 2 for c in [origin-2, origin+TX+2).
-typedef double TileGeo[2][N_GEO][HX];
+template <typename R> using TileGeo = R[2][N_GEO][HX];
 
-__device__ __forceinline__ double slope_lds(const TileGeo &sG, int d, int l,
-                                            double vm, double v0, double vp,
-                                            const Physics &ph) {
+template <typename R>
+__device__ __forceinline__ R slope_lds(const TileGeo<R> &sG, int d, int l, R vm,
+                                       R v0, R vp, R theta, int use_limiter) {
   return slope(vm, v0, vp, sG[d][G_DXL][l], sG[d][G_DXR][l], sG[d][G_DXL2][l],
                sG[d][G_DXR2][l], sG[d][G_DEN][l], sG[d][G_RDXL][l],
-               sG[d][G_RDXR][l], sG[d][G_RDEN][l], ph.theta, ph.use_limiter);
+               sG[d][G_RDXR][l], sG[d][G_RDEN][l], theta, use_limiter);
 }
 
 // x-face i (between cells x0+i-1 and x0+i) of row j into sFx.
-template <typename SP, typename SS, typename SF>
-__device__ __forceinline__ void x_face(const StageArgs &a, SP sP, SS sSx, SF sFx,
-                                       const TileGeo &sG, int i, int j, bool df) {
+template <typename R, typename SP, typename SS, typename SF>
+__device__ __forceinline__ void x_face(const Physics &ph, SP sP, SS sSx, SF sFx,
+                                       const TileGeo<R> &sG, int i, int j, bool df) {
   const int bl = (j + 2) * HX + (i + 1);
-  const double wr = sG[0][G_WR][i + 1], wl = sG[0][G_WL][i + 2];
-  double WL[5], WR[5], F[5];
+  const R wr = sG[0][G_WR][i + 1], wl = sG[0][G_WL][i + 2];
+  R WL[5], WR[5], F[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     WL[f] = sP[f][bl] + wr * sSx[f][j * SXW + i];
     WR[f] = sP[f][bl + 1] - wl * sSx[f][j * SXW + i + 1];
   }
-  face_flux(WL, WR, 0, df, a.ph, F);
+  face_flux(WL, WR, 0, df, ph, F);
 #pragma unroll
   for (int f = 0; f < 5; f++) sFx[f][j * FXW + i] = F[f];
 }
 
 // y-face j (between rows y0+j-1 and y0+j) of column i into sFy.
-template <typename SP, typename SS, typename SF>
-__device__ __forceinline__ void y_face(const StageArgs &a, SP sP, SS sSy, SF sFy,
-                                       const TileGeo &sG, int i, int j, bool df) {
+template <typename R, typename SP, typename SS, typename SF>
+__device__ __forceinline__ void y_face(const Physics &ph, SP sP, SS sSy, SF sFy,
+                                       const TileGeo<R> &sG, int i, int j, bool df) {
   const int bl = (j + 1) * HX + (i + 2);
-  const double wr = sG[1][G_WR][j + 1], wl = sG[1][G_WL][j + 2];
-  double WL[5], WR[5], F[5];
+  const R wr = sG[1][G_WR][j + 1], wl = sG[1][G_WL][j + 2];
+  R WL[5], WR[5], F[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     WL[f] = sP[f][bl] + wr * sSy[f][j * TX + i];
     WR[f] = sP[f][bl + HX] - wl * sSy[f][(j + 1) * TX + i];
   }
-  face_flux(WL, WR, 1, df, a.ph, F);
+  face_flux(WL, WR, 1, df, ph, F);
 #pragma unroll
   for (int f = 0; f < 5; f++) sFy[f][j * TX + i] = F[f];
 }
@@ -230,16 +244,18 @@ __device__ __forceinline__ void y_face(const StageArgs &a, SP sP, SS sSy, SF sFy
 // bottom faces of its own cell; the tile-edge slopes and faces are extra
 // tasks.  Global loads of plane k+1 (halo, z-stencil) are issued during plane
 // k, and the RK input of plane k at its start, so that their latency overlaps
-// the slope and flux work.
-template <int STAGE>
-__global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
-  __shared__ double sP[5][HY * HX];
-  __shared__ double sSx[5][TY * SXW];       // x-slopes of cells x0-1 .. x0+tw
-  __shared__ double sSy[5][(TY + 2) * TX];  // y-slopes of cells y0-1 .. y0+th
-  __shared__ double sFx[5][TY * FXW];       // x-face fluxes x0 .. x0+tw
-  __shared__ double sFy[5][(TY + 1) * TX];  // y-face fluxes y0 .. y0+th
+// the slope and flux work.  Stage 2 applies the conserved fix and the CFL term
+// in FP64 whatever R is.
+template <typename R, int STAGE>
+__global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel(
+    StageArgs<R> a) {
+  __shared__ R sP[5][HY * HX];
+  __shared__ R sSx[5][TY * SXW];       // x-slopes of cells x0-1 .. x0+tw
+  __shared__ R sSy[5][(TY + 2) * TX];  // y-slopes of cells y0-1 .. y0+th
+  __shared__ R sFx[5][TY * FXW];       // x-face fluxes x0 .. x0+tw
+  __shared__ R sFy[5][(TY + 1) * TX];  // y-face fluxes y0 .. y0+th
   __shared__ double sRed[NT / WAVE];
-  __shared__ TileGeo sG;
+  __shared__ TileGeo<R> sG;
 
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * a.tz;
@@ -250,24 +266,25 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
   for (int t = tid; t < 2 * N_GEO * HX; t += NT) {
     const int d = t / (N_GEO * HX), g = (t / HX) % N_GEO, l = t % HX;
     const int o = d == 0 ? x0 : y0, w = d == 0 ? tw : th;
-    sG[d][g][l] = l < w + 4 ? geo(a, d, g, o + l) : 0.0;
+    sG[d][g][l] = l < w + 4 ? geo(a, d, g, o + l) : R(0);
   }
   const int tx = tid % TX, ty = tid / TX;
   const bool active = tx < tw && ty < th;
   const int x = x0 + tx, y = y0 + ty;
   const int ctr = (ty + 2) * HX + tx + 2;  // own cell in the plane tile
   const Physics ph = a.ph;
+  const R theta = R(ph.theta);
   const int nhalo = 4 * th + 4 * tw;
   const int nxs = 2 * th, nys = 2 * tw;  // extra slope tasks
 
-  double Vk[5], Vk1[5], sk[5], Flo[5];
-  double zr[5], hr[5];  // prefetched raw values: z-stencil k+2, halo of k
+  R Vk[5], Vk1[5], sk[5], Flo[5];
+  R zr[5], hr[5];  // prefetched raw values: z-stencil k+2, halo of k
   int zflip = -1, hflip = -1, hpos = 0;
   unsigned flk = 0;     // flag byte of the own cell at plane k (then k+1)
   double tmax = 0.0;
 
   if (active) {
-    double Vm2[5], Vm1[5], sm1[5], WL[5], WR[5];
+    R Vm2[5], Vm1[5], sm1[5], WL[5], WR[5];
     load_prim(a, x, y, z0 - 2, Vm2);
     load_prim(a, x, y, z0 - 1, Vm1);
     load_prim(a, x, y, z0, Vk);
@@ -292,7 +309,7 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
 
   for (int k = z0; k < zend; ++k) {
     const bool more = k + 1 < zend;
-    double ar[5], zn[5];
+    R ar[5], zn[5];
     int znflip = -1;
     unsigned fln = 0, flx = 0;  // own flags at k+1; flag of this thread's edge face
     if (active) {
@@ -300,7 +317,7 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
       for (int f = 0; f < 5; f++) sP[f][ctr] = Vk[f];
     }
     if (tid < nhalo) {
-      double V[5];
+      R V[5];
       raw_to_prim(a, hr, hflip, V);
 #pragma unroll
       for (int f = 0; f < 5; f++) sP[f][hpos] = V[f];
@@ -329,13 +346,13 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
       const int bx = ctr;
 #pragma unroll
       for (int f = 0; f < 5; f++)
-        sSx[f][ty * SXW + tx + 1] =
-            slope_lds(sG, 0, tx + 2, sP[f][bx - 1], sP[f][bx], sP[f][bx + 1], ph);
+        sSx[f][ty * SXW + tx + 1] = slope_lds(sG, 0, tx + 2, sP[f][bx - 1], sP[f][bx],
+                                              sP[f][bx + 1], theta, ph.use_limiter);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int f = 0; f < 5; f++)
-        sSy[f][(ty + 1) * TX + tx] =
-            slope_lds(sG, 1, ty + 2, sP[f][bx - HX], sP[f][bx], sP[f][bx + HX], ph);
+        sSy[f][(ty + 1) * TX + tx] = slope_lds(sG, 1, ty + 2, sP[f][bx - HX], sP[f][bx],
+                                               sP[f][bx + HX], theta, ph.use_limiter);
       __builtin_amdgcn_sched_barrier(0);
     }
     if (tid < nxs + nys) {
@@ -361,8 +378,8 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
       }
 #pragma unroll
       for (int f = 0; f < 5; f++) {
-        const double sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
-                                    sP[f][base + stride], ph);
+        const R sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
+                               sP[f][base + stride], theta, ph.use_limiter);
         if (isx)
           sSx[f][out] = sv;
         else
@@ -374,25 +391,25 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
     // Faces: left and bottom faces of the own cell, then the right column of
     // x-faces (wave 0) and the top row of y-faces (wave 1).
     if (active) {
-      x_face(a, sP, sSx, sFx, sG, tx, ty, flk & 1u);
+      x_face(ph, sP, sSx, sFx, sG, tx, ty, flk & 1u);
       __builtin_amdgcn_sched_barrier(0);
-      y_face(a, sP, sSy, sFy, sG, tx, ty, (flk >> 1) & 1u);
+      y_face(ph, sP, sSy, sFy, sG, tx, ty, (flk >> 1) & 1u);
       __builtin_amdgcn_sched_barrier(0);
     }
-    if (tid < th) x_face(a, sP, sSx, sFx, sG, tw, tid, flx);
+    if (tid < th) x_face(ph, sP, sSx, sFx, sG, tw, tid, flx);
     __builtin_amdgcn_sched_barrier(0);
-    if (tid >= WAVE && tid - WAVE < tw) y_face(a, sP, sSy, sFy, sG, tid - WAVE, th, flx);
+    if (tid >= WAVE && tid - WAVE < tw) y_face(ph, sP, sSy, sFy, sG, tid - WAVE, th, flx);
     __syncthreads();
 
     if (active) {
       // z stencil of plane k: slope at k+1, flux through the face k+1/2.
-      double Vk2[5], sk1[5], Fhi[5];
+      R Vk2[5], sk1[5], Fhi[5];
       {
-        double WL[5], WR[5];
+        R WL[5], WR[5];
         raw_to_prim(a, zr, zflip, Vk2);
 #pragma unroll
         for (int f = 0; f < 5; f++) {
-          const double vk = sP[f][ctr];
+          const R vk = sP[f][ctr];
           sk1[f] = slope_at(a, 2, k + 1, vk, Vk1[f], Vk2[f]);
           WL[f] = vk + geo(a, 2, G_WR, k + 2) * sk[f];
           WR[f] = Vk1[f] - geo(a, 2, G_WL, k + 1 + 2) * sk1[f];
@@ -401,23 +418,23 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
       }
       // ComputeIncrement_CSL_Kernel (EV/Increments/IncrementDivergence_FV__
       // Kernel.f90:672-752), dimensions in order 1, 2, 3, onto K = 0; the
-      // division by the volume uses its reciprocal, rounded exactly.
-      const double dX = sG[0][G_WIDTH][tx + 2], dY = sG[1][G_WIDTH][ty + 2],
-                   dZ = geo(a, 2, G_WIDTH, k + 2);
-      const double Vol = dX * dY * dZ;
-      const double rVol = 1.0 / Vol;
-      const double A1 = dY * dZ, A2 = dZ * dX, A3 = dX * dY;
-      const double dt = a.dt;
+      // division by the volume uses its reciprocal (FP64: rounded exactly).
+      const R dX = sG[0][G_WIDTH][tx + 2], dY = sG[1][G_WIDTH][ty + 2],
+              dZ = geo(a, 2, G_WIDTH, k + 2);
+      const R Vol = dX * dY * dZ;
+      const R rVol = Arith<R>::recip(Vol);
+      const R A1 = dY * dZ, A2 = dZ * dX, A3 = dX * dY;
+      const R dt = R(a.dt);
       const unsigned ic = lin(a.n, x, y, k);
-      double Kc[5];
+      R Kc[5];
 #pragma unroll
       for (int f = 0; f < 5; f++) {
-        const double fxl = sFx[f][ty * FXW + tx], fxh = sFx[f][ty * FXW + tx + 1];
-        const double fyl = sFy[f][ty * TX + tx], fyh = sFy[f][(ty + 1) * TX + tx];
-        double K = 0.0;
-        K = K + div_rcp(dt * (A1 * fxl - A1 * fxh), Vol, rVol);
-        K = K + div_rcp(dt * (A2 * fyl - A2 * fyh), Vol, rVol);
-        K = K + div_rcp(dt * (A3 * Flo[f] - A3 * Fhi[f]), Vol, rVol);
+        const R fxl = sFx[f][ty * FXW + tx], fxh = sFx[f][ty * FXW + tx + 1];
+        const R fyl = sFy[f][ty * TX + tx], fyh = sFy[f][(ty + 1) * TX + tx];
+        R K = R(0);
+        K = K + Arith<R>::rdiv(dt * (A1 * fxl - A1 * fxh), Vol, rVol);
+        K = K + Arith<R>::rdiv(dt * (A2 * fyl - A2 * fyh), Vol, rVol);
+        K = K + Arith<R>::rdiv(dt * (A3 * Flo[f] - A3 * Fhi[f]), Vol, rVol);
         Kc[f] = K;
       }
       if (STAGE == 1) {
@@ -425,19 +442,20 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
         // (Step_RK__Template.f90:240-278, MultiplyAdd_Command.f90:56-94).
 #pragma unroll
         for (int f = 0; f < 5; f++) {
-          st(a.out1 + f * a.fs, ic, ar[f] + 1.0 * Kc[f]);
-          st(a.out2 + f * a.fs, ic, ar[f] + 0.5 * Kc[f]);
+          st(a.out1 + f * a.fs, ic, ar[f] + R(1) * Kc[f]);
+          st(a.out2 + f * a.fs, ic, ar[f] + R(0.5) * Kc[f]);
         }
       } else {
-        // Solution = (Solution + 1/2 K1) + 1/2 K2, StoreSolution's conserved
-        // fix, and the CFL term of the next step.
+        // Solution = (Solution + 1/2 K1) + 1/2 K2, then StoreSolution's
+        // conserved fix and the CFL term of the next step in FP64.
         double U[5], V[5];
 #pragma unroll
-        for (int f = 0; f < 5; f++) U[f] = ar[f] + 0.5 * Kc[f];
+        for (int f = 0; f < 5; f++) U[f] = (double)(ar[f] + R(0.5) * Kc[f]);
         fix_conserved(U, ph.mass, ph.nmin, V);
 #pragma unroll
-        for (int f = 0; f < 5; f++) st(a.out1 + f * a.fs, ic, U[f]);
-        const double tc = cfl_term(V, ph.gamma, ph.mass, dX, dY, dZ);
+        for (int f = 0; f < 5; f++) st(a.out1 + f * a.fs, ic, (R)U[f]);
+        const double tc = cfl_term(V, ph.gamma, ph.mass, (double)dX, (double)dY,
+                                   (double)dZ);
         tmax = tc > tmax ? tc : tmax;
       }
 #pragma unroll
@@ -475,8 +493,9 @@ __global__ __launch_bounds__(NT, 2) void stage_kernel(StageArgs a) {
 }
 
 // Max CFL term over the current state (the first step after an upload, or
-// whenever the stage-2 value is not available).
-__global__ __launch_bounds__(256) void cfl_kernel(StageArgs a) {
+// whenever the stage-2 value is not available), in FP64.
+template <typename R>
+__global__ __launch_bounds__(256) void cfl_kernel(StageArgs<R> a) {
   __shared__ double sRed[256 / WAVE];
   const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
   double tmax = 0.0;
@@ -485,10 +504,14 @@ __global__ __launch_bounds__(256) void cfl_kernel(StageArgs a) {
     const int x = (int)(i % a.n[0]);
     const int y = (int)((i / a.n[0]) % a.n[1]);
     const int z = (int)(i / ((long long)a.n[0] * a.n[1]));
+    R Vr[5];
+    load_prim(a, x, y, z, Vr);
     double V[5];
-    load_prim(a, x, y, z, V);
-    const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, x + 2),
-                               geo(a, 1, G_WIDTH, y + 2), geo(a, 2, G_WIDTH, z + 2));
+#pragma unroll
+    for (int f = 0; f < 5; f++) V[f] = (double)Vr[f];
+    const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, (double)geo(a, 0, G_WIDTH, x + 2),
+                               (double)geo(a, 1, G_WIDTH, y + 2),
+                               (double)geo(a, 2, G_WIDTH, z + 2));
     tmax = tc > tmax ? tc : tmax;
   }
 #pragma unroll
@@ -510,29 +533,31 @@ __global__ __launch_bounds__(256) void cfl_kernel(StageArgs a) {
 // the stored state: bit d set when |dP|/max(min(P), sqrt(tiny)) > threshold
 // and dV_d <= 0.  Cells on the lower face of a dimension get no bit there
 // (no flag that survives the boundary clearing reads it).
-__global__ __launch_bounds__(256) void shock_bits_kernel(StageArgs a,
+template <typename R>
+__global__ __launch_bounds__(256) void shock_bits_kernel(StageArgs<R> a,
                                                          uint8_t *bits) {
   const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
-  const double sqrt_tiny = 1.4916681462400413e-154;
+  const R sqrt_tiny = Arith<R>::sqrt_tiny;
+  const R gm1 = R(a.ph.gamma) - R(1), st = R(a.ph.st);
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
        i += (long long)gridDim.x * 256) {
     const int c[3] = {(int)(i % a.n[0]), (int)((i / a.n[0]) % a.n[1]),
                       (int)(i / ((long long)a.n[0] * a.n[1]))};
-    double V[5];
+    R V[5];
     load_prim(a, c[0], c[1], c[2], V);
-    const double P = (a.ph.gamma - 1.0) * V[4];
+    const R P = gm1 * V[4];
     uint8_t b = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       if (c[d] == 0) continue;
-      double W[5];
+      R W[5];
       load_prim(a, c[0] - (d == 0), c[1] - (d == 1), c[2] - (d == 2), W);
-      const double Pm = (a.ph.gamma - 1.0) * W[4];
-      const double dP = fabs(P - Pm);
-      const double P_Min = fmax(fmin(P, Pm), sqrt_tiny);
-      const double dLnP = dP / P_Min;
-      const double dV = V[1 + d] - W[1 + d];
-      if (dLnP > a.ph.st && dV <= 0.0) b |= (uint8_t)(1u << d);
+      const R Pm = gm1 * W[4];
+      const R dP = fabs(P - Pm);
+      const R P_Min = fmax(fmin(P, Pm), sqrt_tiny);
+      const R dLnP = Arith<R>::div(dP, P_Min);
+      const R dV = V[1 + d] - W[1 + d];
+      if (dLnP > st && dV <= R(0)) b |= (uint8_t)(1u << d);
     }
     bits[i] = b;
   }
@@ -576,8 +601,9 @@ __global__ __launch_bounds__(256) void flags_kernel(const int3 n,
 }
 
 // Current fields of ComputeFromConservedCommon for download
-// (F/Fluid_P_I__Form.f90:604-722), proper cells, on the stored state.
-__global__ __launch_bounds__(256) void current_kernel(StageArgs a, double c_v,
+// (F/Fluid_P_I__Form.f90:604-722), proper cells, on the stored state, in FP64.
+template <typename R>
+__global__ __launch_bounds__(256) void current_kernel(StageArgs<R> a, double c_v,
                                                       double *out) {
   const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
   const double sqrt_huge = 1.3407807929942596e+154;  // sqrt(DBL_MAX)
@@ -585,7 +611,7 @@ __global__ __launch_bounds__(256) void current_kernel(StageArgs a, double c_v,
        i += (long long)gridDim.x * 256) {
     double U[5], V[5];
 #pragma unroll
-    for (int f = 0; f < 5; f++) U[f] = a.aux[f * a.fs + i];
+    for (int f = 0; f < 5; f++) U[f] = (double)a.aux[f * a.fs + i];
     fix_conserved(U, a.ph.mass, a.ph.nmin, V);
     double *o = out;
     const double g = a.ph.gamma, M = a.ph.mass;
@@ -616,29 +642,31 @@ __global__ __launch_bounds__(256) void current_kernel(StageArgs a, double c_v,
 // ComputeFromInitial = ComputeFromPrimitiveCommon on the Current
 // (F/Fluid_P_I__Form.f90:509-601): the density floor N<0 -> 0 is written back
 // to the primitive field, then D, S, G.
+template <typename R>
 __global__ __launch_bounds__(256) void from_primitive_kernel(long long ntot,
-                                                             double mass,
-                                                             double *P,
-                                                             double *U) {
+                                                             double mass_d, R *P,
+                                                             R *U) {
+  const R mass = R(mass_d);
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
        i += (long long)gridDim.x * 256) {
-    double N = P[i];
-    if (N < 0.0) {
-      N = 0.0;
+    R N = P[i];
+    if (N < R(0)) {
+      N = R(0);
       P[i] = N;
     }
-    const double V1 = P[ntot + i], V2 = P[2 * ntot + i], V3 = P[3 * ntot + i];
-    const double S1 = mass * N * V1, S2 = mass * N * V2, S3 = mass * N * V3;
+    const R V1 = P[ntot + i], V2 = P[2 * ntot + i], V3 = P[3 * ntot + i];
+    const R S1 = mass * N * V1, S2 = mass * N * V2, S3 = mass * N * V3;
     U[i] = N;
     U[ntot + i] = S1;
     U[2 * ntot + i] = S2;
     U[3 * ntot + i] = S3;
-    U[4 * ntot + i] = P[4 * ntot + i] + 0.5 * ((S1 * V1 + S2 * V2) + S3 * V3);
+    U[4 * ntot + i] = P[4 * ntot + i] + R(0.5) * ((S1 * V1 + S2 * V2) + S3 * V3);
   }
 }
 
-// Per-block partial sums of U * volume over proper cells.
-__global__ __launch_bounds__(256) void totals_kernel(StageArgs a,
+// Per-block partial sums of U * volume over proper cells, in FP64.
+template <typename R>
+__global__ __launch_bounds__(256) void totals_kernel(StageArgs<R> a,
                                                      double *partial) {
   __shared__ double s[5][256];
   const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
@@ -648,10 +676,11 @@ __global__ __launch_bounds__(256) void totals_kernel(StageArgs a,
     const int x = (int)(i % a.n[0]);
     const int y = (int)((i / a.n[0]) % a.n[1]);
     const int z = (int)(i / ((long long)a.n[0] * a.n[1]));
-    const double vol = geo(a, 0, G_WIDTH, x + 2) * geo(a, 1, G_WIDTH, y + 2) *
-                       geo(a, 2, G_WIDTH, z + 2);
+    const double vol = (double)geo(a, 0, G_WIDTH, x + 2) *
+                       (double)geo(a, 1, G_WIDTH, y + 2) *
+                       (double)geo(a, 2, G_WIDTH, z + 2);
 #pragma unroll
-    for (int f = 0; f < 5; f++) acc[f] += a.aux[f * a.fs + i] * vol;
+    for (int f = 0; f < 5; f++) acc[f] += (double)a.aux[f * a.fs + i] * vol;
   }
 #pragma unroll
   for (int f = 0; f < 5; f++) s[f][threadIdx.x] = acc[f];
@@ -723,11 +752,13 @@ struct gx_ctx {
   long long ncell;
   hipStream_t stream = nullptr;
   int tz = 16;
-  // device buffers
-  double *geo_buf = nullptr;  // 3 dimensions x N_GEO arrays x gs
+  bool fp32 = false;          // state and sweeps in float (Precision = GX_FP32)
+  size_t esz = sizeof(double);  // bytes per state value
+  // device buffers, of the state's real type
+  void *geo_buf = nullptr;    // 3 dimensions x N_GEO arrays x gs
   int gs = 0;                 // geometry stride: max(n) + 4
   // five-field states, field f at base + f * ncell
-  double *U = nullptr, *Y = nullptr, *W = nullptr, *P0 = nullptr;
+  void *U = nullptr, *Y = nullptr, *W = nullptr, *P0 = nullptr;
   uint8_t *flags = nullptr, *bits = nullptr;
   unsigned long long *invdt = nullptr;
   double *partial = nullptr;
@@ -788,8 +819,9 @@ static void geometry_1d(int n, double lo, double hi, std::vector<double> out[N_G
   }
 }
 
-static StageArgs base_args(const gx_ctx *c) {
-  StageArgs a;
+template <typename R>
+static StageArgs<R> base_args(const gx_ctx *c) {
+  StageArgs<R> a;
   memset(&a, 0, sizeof(a));
   for (int d = 0; d < 3; d++) a.n[d] = c->n[d];
   for (int i = 0; i < 6; i++) a.bc[i] = c->grid.BoundaryCondition[i];
@@ -803,7 +835,7 @@ static StageArgs base_args(const gx_ctx *c) {
   a.ph.hllc = c->params.RiemannSolverType == GX_HLLC;
   a.gs = c->gs;
   a.fs = c->ncell;
-  a.geo = c->geo_buf;
+  a.geo = (const R *)c->geo_buf;
   a.flags = c->flags;
   a.invdt_bits = c->invdt;
   return a;
@@ -814,6 +846,42 @@ static int grid_blocks(long long ncell) {
   if (b > 65536) b = 65536;
   if (b < 1) b = 1;
   return (int)b;
+}
+
+// Runs f(R()) with R the context's real type.
+template <typename F>
+static int by_precision(const gx_ctx *c, F &&f) {
+  return c->fp32 ? f(float()) : f(double());
+}
+
+// n host doubles into a device state buffer of the context's real type, and
+// back.  Both synchronise the stream.
+static int put_values(gx_ctx *c, void *dst, const double *src, size_t n) {
+  if (!c->fp32) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> t(n);
+    for (size_t i = 0; i < n; i++) t[i] = (float)src[i];
+    HIP_TRY(hipMemcpyAsync(dst, t.data(), n * sizeof(float), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+static int get_values(gx_ctx *c, double *dst, const void *src, size_t n) {
+  if (!c->fp32) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> t(n);
+    HIP_TRY(hipMemcpyAsync(t.data(), src, n * sizeof(float), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++) dst[i] = (double)t[i];
+  }
+  return GX_OK;
 }
 
 extern "C" {
@@ -878,8 +946,8 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
     p = *params;
   else
     gx_default_params(&p);
-  if (p.Precision != GX_FP64)
-    return fail(GX_E_ARG, "gx_create: only Precision = GX_FP64 is built");
+  if (p.Precision != GX_FP64 && p.Precision != GX_FP32)
+    return fail(GX_E_ARG, "gx_create: Precision must be GX_FP64 or GX_FP32");
   if (p.RiemannSolverType != GX_HLL && p.RiemannSolverType != GX_HLLC)
     return fail(GX_E_ARG, "gx_create: RiemannSolverType must be GX_HLL or GX_HLLC");
   for (int d = 0; d < 3; d++) {
@@ -904,6 +972,8 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
   if (!c) return fail(GX_E_NOMEM, "gx_create: host allocation failed");
   c->grid = *grid;
   c->params = p;
+  c->fp32 = p.Precision == GX_FP32;
+  c->esz = c->fp32 ? sizeof(float) : sizeof(double);
   c->ncell = 1;
   for (int d = 0; d < 3; d++) {
     c->n[d] = grid->nCells[d];
@@ -925,17 +995,31 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
     c->gs = c->n[d] + 4 > c->gs ? c->n[d] + 4 : c->gs;
   }
   const size_t geo_len = 3 * (size_t)N_GEO * c->gs;
-  TRY_CLEAN(hipMalloc(&c->geo_buf, geo_len * sizeof(double)));
+  TRY_CLEAN(hipMalloc(&c->geo_buf, geo_len * c->esz));
   {
     std::vector<double> host(geo_len, 1.0);
     for (int d = 0; d < 3; d++)
       for (int a = 0; a < N_GEO; a++)
         memcpy(&host[((size_t)d * N_GEO + a) * c->gs], c->geo_host[d][a].data(),
                sizeof(double) * (c->n[d] + 4));
-    TRY_CLEAN(hipMemcpy(c->geo_buf, host.data(), geo_len * sizeof(double),
-                        hipMemcpyHostToDevice));
+    if (!c->fp32) {
+      TRY_CLEAN(hipMemcpy(c->geo_buf, host.data(), geo_len * sizeof(double),
+                          hipMemcpyHostToDevice));
+    } else {
+      // Rounded geometry, with the reciprocals of the rounded values.
+      std::vector<float> h32(geo_len);
+      for (size_t i = 0; i < geo_len; i++) h32[i] = (float)host[i];
+      const int recip[3][2] = {{G_RDXL, G_DXL}, {G_RDXR, G_DXR}, {G_RDEN, G_DEN}};
+      for (int d = 0; d < 3; d++)
+        for (int r = 0; r < 3; r++)
+          for (int q = 0; q < c->gs; q++)
+            h32[((size_t)d * N_GEO + recip[r][0]) * c->gs + q] =
+                1.0f / h32[((size_t)d * N_GEO + recip[r][1]) * c->gs + q];
+      TRY_CLEAN(hipMemcpy(c->geo_buf, h32.data(), geo_len * sizeof(float),
+                          hipMemcpyHostToDevice));
+    }
   }
-  const size_t bytes = sizeof(double) * (size_t)c->ncell;
+  const size_t bytes = c->esz * (size_t)c->ncell;
   TRY_CLEAN(hipMalloc(&c->U, 5 * bytes));
   TRY_CLEAN(hipMalloc(&c->Y, 5 * bytes));
   TRY_CLEAN(hipMalloc(&c->W, 5 * bytes));
@@ -983,9 +1067,11 @@ int gx_kernel_times(gx_ctx *c, double ms[4], long long count[4]) {
 
 int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
   if (!c || !primitive || !conserved) return fail(GX_E_ARG, "gx_set_state: null argument");
-  const size_t bytes = 5 * sizeof(double) * (size_t)c->ncell;
-  HIP_TRY(hipMemcpyAsync(c->P0, primitive, bytes, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->U, conserved, bytes, hipMemcpyHostToDevice, c->stream));
+  const size_t n = 5 * (size_t)c->ncell;
+  int rc = put_values(c, c->P0, primitive, n);
+  if (rc) return rc;
+  rc = put_values(c, c->U, conserved, n);
+  if (rc) return rc;
   HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->uploaded = true;
@@ -996,10 +1082,14 @@ int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
 
 int gx_set_primitive(gx_ctx *c, const double *primitive) {
   if (!c || !primitive) return fail(GX_E_ARG, "gx_set_primitive: null argument");
-  const size_t bytes = 5 * sizeof(double) * (size_t)c->ncell;
-  HIP_TRY(hipMemcpyAsync(c->P0, primitive, bytes, hipMemcpyHostToDevice, c->stream));
-  from_primitive_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
-      c->ncell, c->params.BaryonMassReference, c->P0, c->U);
+  const int rc = put_values(c, c->P0, primitive, 5 * (size_t)c->ncell);
+  if (rc) return rc;
+  by_precision(c, [&](auto tag) {
+    using R = decltype(tag);
+    from_primitive_kernel<R><<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
+        c->ncell, c->params.BaryonMassReference, (R *)c->P0, (R *)c->U);
+    return 0;
+  });
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1044,20 +1134,21 @@ int gx_upload(gx_ctx *c, const double *value, long long ld, const int *map) {
 int gx_get_conserved(gx_ctx *c, double *conserved) {
   if (!c || !conserved) return fail(GX_E_ARG, "gx_get_conserved: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_get_conserved: no state uploaded");
-  HIP_TRY(hipMemcpyAsync(conserved, c->U, 5 * sizeof(double) * (size_t)c->ncell,
-                         hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  return GX_OK;
+  return get_values(c, conserved, c->U, 5 * (size_t)c->ncell);
 }
 
 static int run_current(gx_ctx *c) {
   if (!c->scratch)
     HIP_TRY(hipMalloc(&c->scratch, sizeof(double) * GX_N_FIELDS * (size_t)c->ncell));
-  StageArgs a = base_args(c);
-  a.aux = c->U;
   // c_v = k/(mu(gamma-1)) with the default gamma (Fluid_P_I__Form.f90:206-223)
   const double c_v = 1.0 / (1.0 * (1.4 - 1.0));
-  current_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a, c_v, c->scratch);
+  by_precision(c, [&](auto tag) {
+    using R = decltype(tag);
+    StageArgs<R> a = base_args<R>(c);
+    a.aux = (const R *)c->U;
+    current_kernel<R><<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a, c_v, c->scratch);
+    return 0;
+  });
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }
@@ -1067,8 +1158,7 @@ int gx_get_primitive(gx_ctx *c, double *primitive) {
   if (!c->uploaded) return fail(GX_E_STATE, "gx_get_primitive: no state uploaded");
   const size_t bytes = sizeof(double) * (size_t)c->ncell;
   if (c->prims_valid) {
-    HIP_TRY(hipMemcpyAsync(primitive, c->P0, 5 * bytes, hipMemcpyDeviceToHost,
-                           c->stream));
+    return get_values(c, primitive, c->P0, 5 * (size_t)c->ncell);
   } else {
     int rc = run_current(c);
     if (rc) return rc;
@@ -1095,7 +1185,8 @@ int gx_download(gx_ctx *c, double *value, long long ld, const int *map) {
   if (c->prims_valid) {
     // The Current still holds the uploaded primitives.
     std::vector<double> p(5 * (size_t)c->ncell);
-    HIP_TRY(hipMemcpy(p.data(), c->P0, sizeof(double) * p.size(), hipMemcpyDeviceToHost));
+    rc = get_values(c, p.data(), c->P0, p.size());
+    if (rc) return rc;
     for (int f = 0; f < 5; f++)
       memcpy(host.data() + (size_t)PRIM_FIELDS[f] * c->ncell,
              p.data() + (size_t)f * c->ncell, sizeof(double) * c->ncell);
@@ -1176,10 +1267,14 @@ int gx_compute_time_step(gx_ctx *c, double *candidate) {
   if (!c->uploaded) return fail(GX_E_STATE, "gx_compute_time_step: no state uploaded");
   if (!c->invdt_valid) {
     HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
-    StageArgs a = base_args(c);
-    a.src_is_prim = c->prims_valid ? 1 : 0;
-    a.src = c->prims_valid ? c->P0 : c->U;
-    cfl_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a);
+    by_precision(c, [&](auto tag) {
+      using R = decltype(tag);
+      StageArgs<R> a = base_args<R>(c);
+      a.src_is_prim = c->prims_valid ? 1 : 0;
+      a.src = (const R *)(c->prims_valid ? c->P0 : c->U);
+      cfl_kernel<R><<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a);
+      return 0;
+    });
     HIP_TRY(hipGetLastError());
   }
   unsigned long long bits = 0;
@@ -1203,32 +1298,37 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   const dim3 grid((c->n[0] + TX - 1) / TX, (c->n[1] + TY - 1) / TY,
                   (c->n[2] + c->tz - 1) / c->tz);
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  // Stage 1: Current = uploaded primitives or the Solution.
-  StageArgs a1 = base_args(c);
-  a1.dt = dt;
-  a1.src_is_prim = c->prims_valid ? 1 : 0;
-  a1.src = c->prims_valid ? c->P0 : c->U;
-  a1.aux = c->U;
-  a1.out1 = c->Y;
-  a1.out2 = c->W;
-  stage_kernel<1><<<grid, block, 0, c->stream>>>(a1);
-  HIP_TRY(hipGetLastError());
-  // Stage 2: Current = Y.
-  HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-  StageArgs a2 = base_args(c);
-  a2.dt = dt;
-  a2.src = c->Y;
-  a2.aux = c->W;
-  a2.out1 = c->U;
-  stage_kernel<2><<<grid, block, 0, c->stream>>>(a2);
-  HIP_TRY(hipGetLastError());
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-  // Shock flags of the next step from the new state.
-  StageArgs a3 = base_args(c);
-  a3.src = c->U;
-  shock_bits_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a3, c->bits);
-  HIP_TRY(hipGetLastError());
+  const int rc = by_precision(c, [&](auto tag) {
+    using R = decltype(tag);
+    // Stage 1: Current = uploaded primitives or the Solution.
+    StageArgs<R> a1 = base_args<R>(c);
+    a1.dt = dt;
+    a1.src_is_prim = c->prims_valid ? 1 : 0;
+    a1.src = (const R *)(c->prims_valid ? c->P0 : c->U);
+    a1.aux = (const R *)c->U;
+    a1.out1 = (R *)c->Y;
+    a1.out2 = (R *)c->W;
+    stage_kernel<R, 1><<<grid, block, 0, c->stream>>>(a1);
+    HIP_TRY(hipGetLastError());
+    // Stage 2: Current = Y.
+    HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
+    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    StageArgs<R> a2 = base_args<R>(c);
+    a2.dt = dt;
+    a2.src = (const R *)c->Y;
+    a2.aux = (const R *)c->W;
+    a2.out1 = (R *)c->U;
+    stage_kernel<R, 2><<<grid, block, 0, c->stream>>>(a2);
+    HIP_TRY(hipGetLastError());
+    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    // Shock flags of the next step from the new state.
+    StageArgs<R> a3 = base_args<R>(c);
+    a3.src = (const R *)c->U;
+    shock_bits_kernel<R><<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a3, c->bits);
+    HIP_TRY(hipGetLastError());
+    return GX_OK;
+  });
+  if (rc) return rc;
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   flags_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
       make_int3(c->n[0], c->n[1], c->n[2]), c->bits, c->flags);
@@ -1259,9 +1359,13 @@ int gx_sync(gx_ctx *c) {
 int gx_totals(gx_ctx *c, double out[5]) {
   if (!c || !out) return fail(GX_E_ARG, "gx_totals: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_totals: no state uploaded");
-  StageArgs a = base_args(c);
-  a.aux = c->U;
-  totals_kernel<<<c->n_partial, 256, 0, c->stream>>>(a, c->partial);
+  by_precision(c, [&](auto tag) {
+    using R = decltype(tag);
+    StageArgs<R> a = base_args<R>(c);
+    a.aux = (const R *)c->U;
+    totals_kernel<R><<<c->n_partial, 256, 0, c->stream>>>(a, c->partial);
+    return 0;
+  });
   HIP_TRY(hipGetLastError());
   std::vector<double> h(5 * (size_t)c->n_partial);
   HIP_TRY(hipMemcpyAsync(h.data(), c->partial, sizeof(double) * h.size(),

@@ -55,91 +55,117 @@ struct Physics {
   int use_limiter, hllc;
 };
 
+// Arithmetic of one precision.  FP64 keeps the reference's IEEE operations
+// (divisions by precomputed reciprocals are corrected to the exact quotient).
+// FP32 multiplies by the hardware reciprocal (v_rcp_f32, 1 ulp) and uses the
+// hardware square root (v_sqrt_f32, 1 ulp): a quotient is within 2 ulp.
+template <typename R> struct Arith;
+template <> struct Arith<double> {
+  static constexpr double tiny = DBL_MIN;
+  static constexpr double sqrt_tiny = 1.4916681462400413e-154;  // sqrt(DBL_MIN)
+  __device__ static __forceinline__ double rdiv(double a, double b, double y) {
+    return div_rcp(a, b, y);
+  }
+  __device__ static __forceinline__ double recip(double b) { return 1.0 / b; }
+  __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+  __device__ static __forceinline__ double root(double a) { return sqrt(a); }
+};
+template <> struct Arith<float> {
+  static constexpr float tiny = FLT_MIN;
+  static constexpr float sqrt_tiny = 1.0842021724855044e-19f;  // sqrt(FLT_MIN)
+  __device__ static __forceinline__ float rdiv(float a, float, float y) { return a * y; }
+  __device__ static __forceinline__ float recip(float b) { return __builtin_amdgcn_rcpf(b); }
+  __device__ static __forceinline__ float div(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+  }
+  __device__ static __forceinline__ float root(float a) { return __builtin_amdgcn_sqrtf(a); }
+};
+
 // Fluid_P_I % ComputeFromConservedCommon, primitive part (F/Fluid_P__Kernel.
 // f90:94-192): N, V, E from D, S, G with the density floor and E >= 0.
-__device__ __forceinline__ void prim_from_cons(const double U[5], double mass,
-                                               double nmin, double V[5]) {
-  const double D = U[0];
+template <typename R>
+__device__ __forceinline__ void prim_from_cons(const R U[5], R mass, R nmin, R V[5]) {
+  const R D = U[0];
   if (D > nmin) {
-    const double MD = mass * D;
+    const R MD = mass * D;
     V[0] = D;
-    if (MD > 0x1p-500 && MD < 0x1p500) {
-      const double y = 1.0 / MD;
-      V[1] = div_rcp(U[1], MD, y);
-      V[2] = div_rcp(U[2], MD, y);
-      V[3] = div_rcp(U[3], MD, y);
+    if (MD > R(0x1p-100) && MD < R(0x1p100)) {
+      const R y = Arith<R>::recip(MD);
+      V[1] = Arith<R>::rdiv(U[1], MD, y);
+      V[2] = Arith<R>::rdiv(U[2], MD, y);
+      V[3] = Arith<R>::rdiv(U[3], MD, y);
     } else {
       V[1] = U[1] / MD;
       V[2] = U[2] / MD;
       V[3] = U[3] / MD;
     }
-    V[4] = U[4] - 0.5 * ((U[1] * V[1] + U[2] * V[2]) + U[3] * V[3]);
+    V[4] = U[4] - R(0.5) * ((U[1] * V[1] + U[2] * V[2]) + U[3] * V[3]);
   } else {
     V[0] = nmin;
-    V[1] = 0.0;
-    V[2] = 0.0;
-    V[3] = 0.0;
-    V[4] = 0.0;
+    V[1] = R(0);
+    V[2] = R(0);
+    V[3] = R(0);
+    V[4] = R(0);
   }
-  if (V[4] < 0.0) V[4] = 0.0;
+  if (V[4] < R(0)) V[4] = R(0);
 }
 
 // Gradient_Kernel limited slope (OP/Calculus/Derivatives/Gradient_Kernel.f90:
 // 166-190), or the unlimited centred slope when UseLimiter is false.
 // The three divisions use the precomputed reciprocals of the geometry.
-__device__ __forceinline__ double slope(double vm, double v0, double vp,
-                                        double dxl, double dxr, double dxl2,
-                                        double dxr2, double den, double rdxl,
-                                        double rdxr, double rden, double theta,
-                                        int use_limiter) {
-  const double dVL = v0 - vm;
-  const double dVR = vp - v0;
-  const double central = div_rcp(dxr2 * dVL + dxl2 * dVR, den, rden);
+template <typename R>
+__device__ __forceinline__ R slope(R vm, R v0, R vp, R dxl, R dxr, R dxl2, R dxr2,
+                                   R den, R rdxl, R rdxr, R rden, R theta,
+                                   int use_limiter) {
+  const R dVL = v0 - vm;
+  const R dVR = vp - v0;
+  const R central = Arith<R>::rdiv(dxr2 * dVL + dxl2 * dVR, den, rden);
   if (!use_limiter) return central;
-  return (copysign(0.5, dVL) + copysign(0.5, dVR)) *
-         fmin(fmin(fabs(div_rcp(theta * dVL, dxl, rdxl)),
-                   fabs(div_rcp(theta * dVR, dxr, rdxr))),
+  return (copysign(R(0.5), dVL) + copysign(R(0.5), dVR)) *
+         fmin(fmin(fabs(Arith<R>::rdiv(theta * dVL, dxl, rdxl)),
+                   fabs(Arith<R>::rdiv(theta * dVR, dxr, rdxr))),
               fabs(central));
 }
 
 // One reconstructed face state after ComputeFromPrimitiveCommon
 // (F/Fluid_P_I__Form.f90:509-601): EOS (Fluid_P_I__Kernel.f90:197-271) on the
 // unclipped density, then the N<0 floor (Fluid_D__Kernel.f90:87-88), D, S, G.
-struct FaceState {
-  double N, V1, V2, V3, E, P, CS, D, S1, S2, S3, G;
+template <typename R> struct FaceState {
+  R N, V1, V2, V3, E, P, CS, D, S1, S2, S3, G;
 };
 
-__device__ __forceinline__ FaceState face_state(const double W[5],
-                                                const Physics &ph) {
-  FaceState s;
+template <typename R>
+__device__ __forceinline__ FaceState<R> face_state(const R W[5], R gamma, R mass) {
+  FaceState<R> s;
   s.N = W[0];
   s.V1 = W[1];
   s.V2 = W[2];
   s.V3 = W[3];
   s.E = W[4];
-  s.P = (ph.gamma - 1.0) * s.E;
-  if (s.N > 0.0 && s.P > 0.0)
-    s.CS = sqrt(ph.gamma * s.P / (ph.mass * s.N));
+  s.P = (gamma - R(1)) * s.E;
+  if (s.N > R(0) && s.P > R(0))
+    s.CS = Arith<R>::root(Arith<R>::div(gamma * s.P, mass * s.N));
   else
-    s.CS = 0.0;
-  if (s.N < 0.0) s.N = 0.0;
+    s.CS = R(0);
+  if (s.N < R(0)) s.N = R(0);
   s.D = s.N;
-  s.S1 = ph.mass * s.N * s.V1;
-  s.S2 = ph.mass * s.N * s.V2;
-  s.S3 = ph.mass * s.N * s.V3;
-  s.G = s.E + 0.5 * ((s.S1 * s.V1 + s.S2 * s.V2) + s.S3 * s.V3);
+  s.S1 = mass * s.N * s.V1;
+  s.S2 = mass * s.N * s.V2;
+  s.S3 = mass * s.N * s.V3;
+  s.G = s.E + R(0.5) * ((s.S1 * s.V1 + s.S2 * s.V2) + s.S3 * s.V3);
   return s;
 }
 
-__device__ __forceinline__ double sel3(int d, double a, double b, double c) {
+template <typename R>
+__device__ __forceinline__ R sel3(int d, R a, R b, R c) {
   return d == 0 ? a : (d == 1 ? b : c);
 }
 
 // Raw flux in dimension d (F/Fluid_D__Kernel.f90:220-262 then
 // F/Fluid_P__Kernel.f90:671-709): D V, S_i V, S_d V + P, (G + P) V.
-__device__ __forceinline__ void raw_flux(double D, double S1, double S2,
-                                         double S3, double G, double P,
-                                         double Vd, int d, double F[5]) {
+template <typename R>
+__device__ __forceinline__ void raw_flux(R D, R S1, R S2, R S3, R G, R P, R Vd,
+                                         int d, R F[5]) {
   F[0] = D * Vd;
   F[1] = S1 * Vd;
   F[2] = S2 * Vd;
@@ -157,73 +183,74 @@ __device__ __forceinline__ void raw_flux(double D, double S1, double S2,
 // (F/Fluid_P__Kernel.f90:319-668).  Only the flux the reference keeps is
 // evaluated: the HLL flux is skipped where HLLC overwrites it, and only the
 // star state that is selected is built.
-__device__ __forceinline__ void face_flux(const double WL[5],
-                                          const double WR[5], int d, bool df,
-                                          const Physics &ph, double F[5]) {
-  const double tiny = DBL_MIN;
-  const double sqrt_tiny = 1.4916681462400413e-154;  // sqrt(DBL_MIN), exact
-  const FaceState L = face_state(WL, ph);
-  const FaceState R = face_state(WR, ph);
-  const double vdL = sel3(d, L.V1, L.V2, L.V3);
-  const double vdR = sel3(d, R.V1, R.V2, R.V3);
-  const double AP = fmax(fmax(0.0, vdL + L.CS), vdR + R.CS);
-  const double AM = fmax(fmax(0.0, -(vdL - L.CS)), -(vdR - R.CS));
-  double FL[5], FR[5];
+template <typename R>
+__device__ __forceinline__ void face_flux(const R WL[5], const R WR[5], int d, bool df,
+                                          const Physics &ph, R F[5]) {
+  const R tiny = Arith<R>::tiny;
+  const R sqrt_tiny = Arith<R>::sqrt_tiny;
+  const R gamma = R(ph.gamma), mass = R(ph.mass);
+  const FaceState<R> L = face_state(WL, gamma, mass);
+  const FaceState<R> Rt = face_state(WR, gamma, mass);
+  const R vdL = sel3(d, L.V1, L.V2, L.V3);
+  const R vdR = sel3(d, Rt.V1, Rt.V2, Rt.V3);
+  const R AP = fmax(fmax(R(0), vdL + L.CS), vdR + Rt.CS);
+  const R AM = fmax(fmax(R(0), -(vdL - L.CS)), -(vdR - Rt.CS));
+  R FL[5], FR[5];
   raw_flux(L.D, L.S1, L.S2, L.S3, L.G, L.P, vdL, d, FL);
-  raw_flux(R.D, R.S1, R.S2, R.S3, R.G, R.P, vdR, d, FR);
-  if (ph.hllc && !df && AP != 0.0 && AM != 0.0) {
-    const double SdL = sel3(d, L.S1, L.S2, L.S3);
-    const double SdR = sel3(d, R.S1, R.S2, R.S3);
-    const double MD = AP * ph.mass * R.D + AM * ph.mass * L.D - FR[0] + FL[0];
-    const double FSdL = sel3(d, FL[1], FL[2], FL[3]);
-    const double FSdR = sel3(d, FR[1], FR[2], FR[3]);
-    const double SN = AP * SdR + AM * SdL - FSdR + FSdL;
-    const double MDinv = fmax(MD, 0.0) / fmax(MD * MD, tiny);
-    const double AC = 1.0 * SN * MDinv;
-    if (AC >= 0.0) {
-      const double AM_VL = AM + vdL;
-      const double AM_AC = AM + AC;
-      const double inv = 1.0 / fmax(fabs(AM_AC), sqrt_tiny);
-      const double Ds = L.D * AM_VL * inv;
-      const double V1 = d == 0 ? AC : L.V1;
-      const double V2 = d == 1 ? AC : L.V2;
-      const double V3 = d == 2 ? AC : L.V3;
-      const double S1 = ph.mass * Ds * V1;
-      const double S2 = ph.mass * Ds * V2;
-      const double S3 = ph.mass * Ds * V3;
-      const double Sds = sel3(d, S1, S2, S3);
-      const double Ps = L.P + SdL * AM_VL - Sds * AM_AC;
-      const double Gs = (L.G * AM_VL + vdL * L.P - AC * Ps) * inv;
+  raw_flux(Rt.D, Rt.S1, Rt.S2, Rt.S3, Rt.G, Rt.P, vdR, d, FR);
+  if (ph.hllc && !df && AP != R(0) && AM != R(0)) {
+    const R SdL = sel3(d, L.S1, L.S2, L.S3);
+    const R SdR = sel3(d, Rt.S1, Rt.S2, Rt.S3);
+    const R MD = AP * mass * Rt.D + AM * mass * L.D - FR[0] + FL[0];
+    const R FSdL = sel3(d, FL[1], FL[2], FL[3]);
+    const R FSdR = sel3(d, FR[1], FR[2], FR[3]);
+    const R SN = AP * SdR + AM * SdL - FSdR + FSdL;
+    const R MDinv = Arith<R>::div(fmax(MD, R(0)), fmax(MD * MD, tiny));
+    const R AC = R(1) * SN * MDinv;
+    if (AC >= R(0)) {
+      const R AM_VL = AM + vdL;
+      const R AM_AC = AM + AC;
+      const R inv = Arith<R>::div(R(1), fmax(fabs(AM_AC), sqrt_tiny));
+      const R Ds = L.D * AM_VL * inv;
+      const R V1 = d == 0 ? AC : L.V1;
+      const R V2 = d == 1 ? AC : L.V2;
+      const R V3 = d == 2 ? AC : L.V3;
+      const R S1 = mass * Ds * V1;
+      const R S2 = mass * Ds * V2;
+      const R S3 = mass * Ds * V3;
+      const R Sds = sel3(d, S1, S2, S3);
+      const R Ps = L.P + SdL * AM_VL - Sds * AM_AC;
+      const R Gs = (L.G * AM_VL + vdL * L.P - AC * Ps) * inv;
       raw_flux(Ds, S1, S2, S3, Gs, Ps, AC, d, F);
     } else {
-      const double AP_VR = AP - vdR;
-      const double AP_AC = AP - AC;
-      const double inv = 1.0 / fmax(fabs(AP_AC), sqrt_tiny);
-      const double Ds = R.D * AP_VR * inv;
-      const double V1 = d == 0 ? AC : R.V1;
-      const double V2 = d == 1 ? AC : R.V2;
-      const double V3 = d == 2 ? AC : R.V3;
-      const double S1 = ph.mass * Ds * V1;
-      const double S2 = ph.mass * Ds * V2;
-      const double S3 = ph.mass * Ds * V3;
-      const double Sds = sel3(d, S1, S2, S3);
-      const double Ps = R.P - SdR * AP_VR + Sds * AP_AC;
-      const double Gs = (R.G * AP_VR - vdR * R.P + AC * Ps) * inv;
+      const R AP_VR = AP - vdR;
+      const R AP_AC = AP - AC;
+      const R inv = Arith<R>::div(R(1), fmax(fabs(AP_AC), sqrt_tiny));
+      const R Ds = Rt.D * AP_VR * inv;
+      const R V1 = d == 0 ? AC : Rt.V1;
+      const R V2 = d == 1 ? AC : Rt.V2;
+      const R V3 = d == 2 ? AC : Rt.V3;
+      const R S1 = mass * Ds * V1;
+      const R S2 = mass * Ds * V2;
+      const R S3 = mass * Ds * V3;
+      const R Sds = sel3(d, S1, S2, S3);
+      const R Ps = Rt.P - SdR * AP_VR + Sds * AP_AC;
+      const R Gs = (Rt.G * AP_VR - vdR * Rt.P + AC * Ps) * inv;
       raw_flux(Ds, S1, S2, S3, Gs, Ps, AC, d, F);
     }
     return;
   }
-  const double UL[5] = {L.D, L.S1, L.S2, L.S3, L.G};
-  const double UR[5] = {R.D, R.S1, R.S2, R.S3, R.G};
-  const double den = fmax(AP + AM, tiny);
+  const R UL[5] = {L.D, L.S1, L.S2, L.S3, L.G};
+  const R UR[5] = {Rt.D, Rt.S1, Rt.S2, Rt.S3, Rt.G};
+  const R den = fmax(AP + AM, tiny);
 #pragma unroll
   for (int f = 0; f < 5; f++)
-    F[f] = (AP * FL[f] + AM * FR[f] - 1.0 * AP * AM * (UR[f] - UL[f])) / den;
+    F[f] = Arith<R>::div(AP * FL[f] + AM * FR[f] - R(1) * AP * AM * (UR[f] - UL[f]), den);
 }
 
-// Conserved fix of ComputeFromConserved on the new state, returning the
-// primitive velocity and internal energy it implies (F/Fluid_P__Kernel.f90:
-// 94-192).
+// Conserved fix of ComputeFromConserved on the new state (FP64), returning
+// the primitive velocity and internal energy it implies (F/Fluid_P__Kernel.
+// f90:94-192).
 __device__ __forceinline__ void fix_conserved(double U[5], double mass,
                                               double nmin, double V[5]) {
   const double D = U[0];
@@ -252,7 +279,7 @@ __device__ __forceinline__ void fix_conserved(double U[5], double mass,
 }
 
 // Per-cell CFL term (Integrator_C_PS__Kernel.f90:12-144, nDimensions 3,
-// Crsn = 1) from primitives, with the EOS sound speed.
+// Crsn = 1) from primitives, with the EOS sound speed, in FP64.
 __device__ __forceinline__ double cfl_term(const double V[5], double gamma,
                                           double mass, double w1, double w2,
                                           double w3) {

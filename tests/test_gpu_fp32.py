@@ -1,0 +1,107 @@
+"""FP32 variant of the step: state and sweep arithmetic in float, with the
+CFL term, the conserved totals and the conserved fix in FP64.  Checked
+against the FP64 HIP path (itself bit-identical to the oracle) and against
+the oracle directly, with the same time steps; the tolerance is the density
+L1 error bound 1e-5 (volume-weighted mean of |N_fp32 - N_fp64|)."""
+import numpy as np
+import pytest
+
+from genasis_amd import problems
+from genasis_amd.chart import Chart
+from genasis_amd.compare import field_errors
+from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
+
+from ._helpers import make_oracle, oracle_conserved
+
+pytestmark = pytest.mark.gpu
+
+PER = (("PERIODIC", "PERIODIC"),) * 3
+OUT = (("OUTFLOW", "OUTFLOW"),) * 3
+DENSITY_L1 = 1e-5
+
+
+def _ramped(c, s):
+    ramp = min(float(s + 1) / 100.0, 1.0)
+    return ramp * c if ramp < 1.0 else c
+
+
+def test_fp32_kelvin_helmholtz_vs_fp64():
+    chart = Chart((48, 48, 48), bc=PER)
+    prim = problems.kelvin_helmholtz(chart, seed=7)
+    g64 = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8))
+    g32 = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, Precision="FP32"))
+    g64.set_primitive(*prim)
+    g32.set_primitive(*prim)
+    t64_0, t32_0 = g64.totals(), g32.totals()
+    for s in range(30):
+        c64 = g64.time_step_candidate()
+        c32 = g32.time_step_candidate()
+        # FP64 CFL term of a float state: candidates agree to float rounding
+        assert abs(c32 - c64) <= 1e-5 * c64, (s, c32, c64)
+        dt = _ramped(c64, s)
+        g64.step(0.0, dt)
+        g32.step(0.0, dt)
+    u64, u32 = g64.conserved(), g32.conserved()
+    err = field_errors(u32, u64)
+    assert err["D"]["L1"] <= DENSITY_L1, err["D"]
+    # fields are O(1) (S_2 is the 1e-2 perturbation): absolute bounds
+    for name, e in err.items():
+        assert e["L1"] <= 1e-6 and e["Linf"] <= 1e-4, (name, e)
+    # periodic box: mass, momentum and energy totals conserved (FP64 sums of
+    # the float state) to float rounding
+    t64, t32 = g64.totals(), g32.totals()
+    scale = np.abs(t64_0).max()
+    assert np.all(np.abs(t64 - t64_0) <= 1e-12 * scale), (t64, t64_0)
+    assert np.all(np.abs(t32 - t32_0) <= 1e-6 * scale), (t32, t32_0)
+
+
+def test_fp32_sod_vs_oracle():
+    chart = Chart((64, 8, 8), bc=OUT)
+    prim = problems.sod(chart)
+    o = make_oracle(chart, prim, courant=0.8)
+    g = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, Precision="FP32"))
+    g.set_primitive(*prim)
+    for s in range(20):
+        dt = _ramped(o.time_step_candidate()[0], s)
+        o.step(dt)
+        g.step(0.0, dt)
+    err = field_errors(g.conserved(), oracle_conserved(o, chart))
+    assert err["D"]["L1"] <= DENSITY_L1, err["D"]
+    assert err["G"]["L1_rel"] <= 1e-5, err["G"]
+
+
+def test_fp32_hll_reflecting_vs_fp64():
+    bc = (("REFLECTING", "OUTFLOW"), ("OUTFLOW", "REFLECTING"), ("PERIODIC", "PERIODIC"))
+    chart = Chart((37, 11, 19), bc=bc)
+    rng = np.random.default_rng(5)
+    shape = chart.m[::-1]
+    N = np.exp(rng.uniform(np.log(0.5), np.log(2.0), shape))
+    V = [rng.uniform(-0.3, 0.3, shape) for _ in range(3)]
+    E = np.exp(rng.uniform(np.log(0.5), np.log(2.0), shape)) / 0.4
+    prim = (N, V[0], V[1], V[2], E)
+    runs = []
+    for prec in ("FP64", "FP32"):
+        g = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.5, RiemannSolverType="HLL",
+                                                  Precision=prec))
+        g.set_tuning(5)
+        g.set_primitive(*prim)
+        runs.append(g)
+    for s in range(10):
+        dt = _ramped(runs[0].time_step_candidate(), s)
+        for g in runs:
+            g.step(0.0, dt)
+    err = field_errors(runs[1].conserved(), runs[0].conserved())
+    assert err["D"]["L1"] <= DENSITY_L1, err["D"]
+
+
+def test_fp32_state_round_trip():
+    chart = Chart((16, 12, 10), bc=PER)
+    rng = np.random.default_rng(3)
+    shape = chart.n[::-1]
+    prim = [rng.uniform(0.5, 2.0, shape) for _ in range(5)]
+    cons = [rng.uniform(0.5, 2.0, shape) for _ in range(5)]
+    g = Step_RK2_C_ASC(chart, FluidParameters(Precision="FP32"))
+    g.set_state(prim, cons)
+    want = np.stack(cons).astype(np.float32).astype(np.float64)
+    assert np.array_equal(g.conserved(), want)
+    assert np.array_equal(g.primitive(), np.stack(prim).astype(np.float32).astype(np.float64))
