@@ -34,6 +34,7 @@ constexpr int HY = TY + 4;
 constexpr int SXW = TX + 2;      // x-slope row length
 constexpr int FXW = TX + 1;      // x-flux row length
 constexpr int WAVE = 64;
+constexpr int NXCD = 8;          // accelerator complex dies, one L2 each
 
 __host__ __device__ inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
@@ -49,6 +50,7 @@ struct StageArgs {
   int tz;               // planes per block (z-chunk)
   int src_is_prim;      // stage input holds primitives (first stage after upload)
   int gs;               // geometry stride (max n + 4)
+  int nbx, nby, nblk;   // stage-kernel tiles in x, in y, in total
   long long fs;         // field stride (cells)
   double dt;
   Physics ph;
@@ -175,6 +177,17 @@ __device__ __forceinline__ R slope_at(const StageArgs<R> &a, int d, int c, R vm,
                R(a.ph.theta), a.ph.use_limiter);
 }
 
+// XCD-aware tile order.  The dispatcher deals consecutive blocks to the XCDs
+// in turn; block b is given tile t so that every XCD owns one contiguous run
+// of tiles (x fastest, then y, then z-chunk).  Tiles that share halo cache
+// lines then run on the same L2 at about the same time.  A bijection of
+// [0, nblk): XCD j holds q + (j < r) blocks and tiles from j*q + min(j, r).
+__device__ __forceinline__ int tile_of_block(int b, int nblk) {
+  const int q = nblk / NXCD, r = nblk % NXCD;
+  const int xcd = b % NXCD, slot = b / NXCD;
+  return xcd * q + min(xcd, r) + slot;
+}
+
 // Halo cell t of the plane tile: 4 per row for x (two each side), 4 per
 // column for y.
 __device__ __forceinline__ void halo_cell(int t, int x0, int y0, int tw, int th,
@@ -258,7 +271,9 @@ __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel(
   __shared__ TileGeo<R> sG;
 
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * a.tz;
+  const int tile = tile_of_block(blockIdx.x, a.nblk);
+  const int x0 = (tile % a.nbx) * TX, y0 = ((tile / a.nbx) % a.nby) * TY,
+            z0 = (tile / (a.nbx * a.nby)) * a.tz;
   const int tw = min(TX, nx - x0), th = min(TY, ny - y0);
   const int zend = min(z0 + a.tz, nz);
   const int tid = threadIdx.x;
@@ -694,7 +709,10 @@ __global__ __launch_bounds__(256) void totals_kernel(StageArgs<R> a,
 }
 
 // Self-test of div_rcp against the IEEE division on pseudo-random operands:
-// |a| in [2^-60, 2^60), b in [2^-40, 2^40), random signs and mantissas.
+// |a| in [2^-60, 2^67], b in [2^-32, 2^31], random signs and mantissas; one
+// pair in 64 each has a numerator of +-0, +-inf, NaN, a normal below the fast
+// range (2^-950), a subnormal or a large value (2^600).  NaN results match
+// when both are NaN.
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -714,11 +732,21 @@ __global__ __launch_bounds__(256) void division_selftest_kernel(
     const double ma = 1.0 + (double)(r1 & ((1ull << 52) - 1)) * 0x1p-52;
     const double mb = 1.0 + (double)(r2 & ((1ull << 52) - 1)) * 0x1p-52;
     double a = ldexp(ma, (int)ea), b = ldexp(mb, (int)eb);
+    switch ((int)((r2 >> 52) & 63)) {
+      case 0: a = 0.0; break;
+      case 1: a = ldexp(ma, 900); break;
+      case 2: a = ldexp(ma, -950); break;
+      case 3: a = ldexp(ma, -1060); break;
+      case 4: a = ldexp(ma, 600); break;
+      case 5: a = __longlong_as_double(0x7ff8000000000001ll); break;
+      default: break;
+    }
     if (r1 & (1ull << 52)) a = -a;
     const double y = 1.0 / b;
     const double q1 = div_rcp(a, b, y);
     const double q2 = a / b;
-    if (__double_as_longlong(q1) != __double_as_longlong(q2)) bad++;
+    const bool both_nan = q1 != q1 && q2 != q2;
+    if (__double_as_longlong(q1) != __double_as_longlong(q2) && !both_nan) bad++;
   }
   if (bad) atomicAdd(mismatches, bad);
 }
@@ -988,12 +1016,37 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
       return fail(GX_E_HIP, m_);                                          \
     }                                                                     \
   } while (0)
-  TRY_CLEAN(hipStreamCreate(&c->stream));
   c->gs = 0;
   for (int d = 0; d < 3; d++) {
     geometry_1d(c->n[d], grid->MinCoordinate[d], grid->MaxCoordinate[d], c->geo_host[d]);
     c->gs = c->n[d] + 4 > c->gs ? c->n[d] + 4 : c->gs;
   }
+  {
+    // Divisors of the exactly rounded reciprocal division: centre spacings,
+    // slope denominators and cell volumes must lie in [2^-500, 2^122].
+    const double lo = 0x1p-500, hi = 0x1p122;
+    double vmin = 1.0, vmax = 1.0;
+    bool ok = true;
+    for (int d = 0; d < 3; d++) {
+      const std::vector<double> *g = c->geo_host[d];
+      double wmin = HUGE_VAL, wmax = 0.0;
+      for (int q = 0; q < c->n[d] + 4; q++) {
+        for (int k : {G_DXL, G_DXR, G_DEN})
+          ok = ok && g[k][q] >= lo && g[k][q] <= hi;
+        if (q >= 2 && q < c->n[d] + 2) {
+          wmin = fmin(wmin, g[G_WIDTH][q]);
+          wmax = fmax(wmax, g[G_WIDTH][q]);
+        }
+      }
+      vmin *= wmin;
+      vmax *= wmax;
+    }
+    if (!ok || !(vmin >= lo) || !(vmax <= hi)) {
+      gx_destroy(c);
+      return fail(GX_E_ARG, "gx_create: cell spacings and volumes must lie in [2^-500, 2^122]");
+    }
+  }
+  TRY_CLEAN(hipStreamCreate(&c->stream));
   const size_t geo_len = 3 * (size_t)N_GEO * c->gs;
   TRY_CLEAN(hipMalloc(&c->geo_buf, geo_len * c->esz));
   {
@@ -1295,8 +1348,10 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   if (!c) return fail(GX_E_ARG, "gx_step_rk2: null context");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_step_rk2: no state uploaded");
   const dim3 block(NT);
-  const dim3 grid((c->n[0] + TX - 1) / TX, (c->n[1] + TY - 1) / TY,
-                  (c->n[2] + c->tz - 1) / c->tz);
+  const int nbx = (c->n[0] + TX - 1) / TX, nby = (c->n[1] + TY - 1) / TY,
+            nbz = (c->n[2] + c->tz - 1) / c->tz;
+  const int nblk = nbx * nby * nbz;
+  const dim3 grid(nblk);
   if (c->profiling) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   const int rc = by_precision(c, [&](auto tag) {
     using R = decltype(tag);
@@ -1308,6 +1363,9 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     a1.aux = (const R *)c->U;
     a1.out1 = (R *)c->Y;
     a1.out2 = (R *)c->W;
+    a1.nbx = nbx;
+    a1.nby = nby;
+    a1.nblk = nblk;
     stage_kernel<R, 1><<<grid, block, 0, c->stream>>>(a1);
     HIP_TRY(hipGetLastError());
     // Stage 2: Current = Y.
@@ -1318,6 +1376,9 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     a2.src = (const R *)c->Y;
     a2.aux = (const R *)c->W;
     a2.out1 = (R *)c->U;
+    a2.nbx = nbx;
+    a2.nby = nby;
+    a2.nblk = nblk;
     stage_kernel<R, 2><<<grid, block, 0, c->stream>>>(a2);
     HIP_TRY(hipGetLastError());
     if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], c->stream));

@@ -35,12 +35,16 @@ enum GeoArray {
   N_GEO
 };
 
-// a/b rounded to nearest, given y = RN(1/b): q0 = RN(a y) is corrected twice
-// with the exact-remainder fma; after the first correction q is within one
-// ulp of a/b, so the remainder of the second is exact and its correction
-// rounds to RN(a/b) (Markstein, IBM J. Res. Dev. 34 (1990) 111-119).  The
-// quotient, a and the remainders must be normal: callers keep b in
-// [2^-500, 2^500] and route |a| < 2^-900 to the IEEE division.
+// a/b rounded to nearest for b > 0, given y = RN(1/b): q0 = RN(a y) is
+// corrected twice with the exact-remainder fma; after the first correction q
+// is within one ulp of a/b, so the remainder of the second is exact and its
+// correction rounds to RN(a/b) (Markstein, IBM J. Res. Dev. 34 (1990)
+// 111-119).  Callers keep b in [2^-500, 2^122] and |a| < 2^-900 (zero and
+// subnormals included) goes to the IEEE division, so for finite a with a
+// finite quotient q is normal and the remainders (exponent >= that of a minus
+// 105) are exact.  Outside that -- a = +-inf, or |a/b| overflowing -- the
+// result is NaN where the IEEE division gives +-inf: a state that has
+// already overflowed.
 __device__ __forceinline__ double div_rcp(double a, double b, double y) {
   if (fabs(a) < 0x1p-900) return a / b;
   double q = a * y;

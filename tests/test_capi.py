@@ -49,4 +49,15 @@ def test_default_params_and_argument_errors():
     g.BoundaryCondition[1] = _capi.GX_BC_OUTFLOW
     rc = lib.gx_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
     assert rc == -1 and b"PERIODIC" in lib.gx_last_error()
+    g.BoundaryCondition[1] = _capi.GX_BC_PERIODIC
+    p.Precision = 2
+    rc = lib.gx_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
+    assert rc == -1 and b"Precision" in lib.gx_last_error()
+    p.Precision = _capi.GX_FP64
+    # spacings outside the range of the exactly rounded reciprocal division
+    for big in (1.0e40, 1.0e-160):
+        for d in range(3):
+            g.MaxCoordinate[d] = big
+        rc = lib.gx_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
+        assert rc == -1 and b"spacings" in lib.gx_last_error(), big
     assert lib.gx_step_rk2(None, 0.0, 1.0) == -1
