@@ -543,75 +543,107 @@ __global__ __launch_bounds__(256) void cfl_kernel(StageArgs<R> a) {
   }
 }
 
-// Shock test of DetectShocks_CSL_Kernel (F/FluidFeatures_P__Kernel.f90:
-// 12-186) between each cell and its lower neighbour in every dimension, on
-// the stored state: bit d set when |dP|/max(min(P), sqrt(tiny)) > threshold
-// and dV_d <= 0.  Cells on the lower face of a dimension get no bit there
-// (no flag that survives the boundary clearing reads it).
-template <typename R>
-__global__ __launch_bounds__(256) void shock_bits_kernel(StageArgs<R> a,
-                                                         uint8_t *bits) {
-  const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
-  const R sqrt_tiny = Arith<R>::sqrt_tiny;
-  const R gm1 = R(a.ph.gamma) - R(1), st = R(a.ph.st);
-  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
-       i += (long long)gridDim.x * 256) {
-    const int c[3] = {(int)(i % a.n[0]), (int)((i / a.n[0]) % a.n[1]),
-                      (int)(i / ((long long)a.n[0] * a.n[1]))};
-    R V[5];
-    load_prim(a, c[0], c[1], c[2], V);
-    const R P = gm1 * V[4];
-    uint8_t b = 0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      if (c[d] == 0) continue;
-      R W[5];
-      load_prim(a, c[0] - (d == 0), c[1] - (d == 1), c[2] - (d == 2), W);
-      const R Pm = gm1 * W[4];
-      const R dP = fabs(P - Pm);
-      const R P_Min = fmax(fmin(P, Pm), sqrt_tiny);
-      const R dLnP = Arith<R>::div(dP, P_Min);
-      const R dV = V[1 + d] - W[1 + d];
-      if (dLnP > st && dV <= R(0)) b |= (uint8_t)(1u << d);
-    }
-    bits[i] = b;
-  }
-}
+// Shock flags of the next step from the stored state, in one pass per (x, y)
+// tile marching along z.  DetectShocks_CSL_Kernel (F/FluidFeatures_P__Kernel.
+// f90:12-186) tests each cell against its lower neighbour in every dimension
+// i -- |dP|/max(min(P), sqrt(tiny)) > ShockThreshold and dV_i <= 0 -- and a
+// shock marks the DiffusiveFlux flags of the transverse faces on both sides
+// of the two cells it separates (:137-147); ClearBoundary_CSL_Kernel
+// (:189-313) then clears the first and last two layers of every dimension.
+// So flags live on cells 1 .. n-2 and read shock tests of chart cells only:
+// flag_d(c) = OR over i != d of test_i at c, c + e_i, c - e_d, c + e_i - e_d.
+// Per plane k: P and V of the tile with two cells of halo below and one
+// above go to LDS (ping-pong with plane k-1 for the z test); the tests of
+// the tile with a one-cell halo go to a ring of three planes; the flags of
+// plane k-1 follow from the ring.
+constexpr int QW = TX + 3, QH = TY + 3;  // primitives of x0-2 .. x0+TX, y0-2 .. y0+TY
+constexpr int BW = TX + 2, BH = TY + 2;  // tests of x0-1 .. x0+TX, y0-1 .. y0+TY
+constexpr int FLAG_TZ = 32;              // planes per block
 
-// DiffusiveFlux_I flags from the shock bits: a shock in dimension i marks the
-// transverse faces on both sides of the two cells it separates
-// (F/FluidFeatures_P__Kernel.f90:137-147); ClearBoundary_CSL_Kernel
-// (:189-313) then clears the first and last two layers of every dimension, so
-// flags live on cells 1 .. n-2 only.
-__global__ __launch_bounds__(256) void flags_kernel(const int3 n,
-                                                    const uint8_t *bits,
-                                                    uint8_t *flags) {
-  const long long ntot = (long long)n.x * n.y * n.z;
-  const long long s[3] = {1, n.x, (long long)n.x * n.y};
-  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
-       i += (long long)gridDim.x * 256) {
-    const int c[3] = {(int)(i % n.x), (int)((i / n.x) % n.y),
-                      (int)(i / ((long long)n.x * n.y))};
-    const int nn[3] = {n.x, n.y, n.z};
-    uint8_t out = 0;
-    bool inside = true;
+template <typename R>
+__global__ __launch_bounds__(NT) void shock_flags_kernel(StageArgs<R> a, uint8_t *flags) {
+  __shared__ R sQ[2][4][QH * QW];     // P, V_1, V_2, V_3 of planes k and k-1
+  __shared__ uint8_t sB[3][BH * BW];  // tests of plane p at (p + 3) % 3
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+  const int tile = tile_of_block(blockIdx.x, a.nblk);
+  const int x0 = (tile % a.nbx) * TX, y0 = ((tile / a.nbx) % a.nby) * TY,
+            z0 = (tile / (a.nbx * a.nby)) * a.tz;
+  const int zend = min(z0 + a.tz, nz);
+  const int tid = threadIdx.x;
+  const R gm1 = R(a.ph.gamma) - R(1), st = R(a.ph.st);
+  const R mass = R(a.ph.mass), nmin = R(a.ph.nmin);
+  const R sqrt_tiny = Arith<R>::sqrt_tiny;
+
+  for (int k = z0 - 2; k <= zend; ++k) {
+    R(*Q)[QH * QW] = sQ[(k + 2) & 1];
+    R(*Qm)[QH * QW] = sQ[(k + 3) & 1];
+    const bool plane = k >= 0 && k < nz;
+    if (plane) {
+      for (int t = tid; t < QH * QW; t += NT) {
+        const int x = x0 - 2 + t % QW, y = y0 - 2 + t / QW;
+        R V[5] = {R(0), R(0), R(0), R(0), R(0)};
+        if ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny) {
+          const unsigned i = lin(a.n, x, y, k);
+          R U[5];
 #pragma unroll
-    for (int e = 0; e < 3; e++) inside = inside && c[e] >= 1 && c[e] <= nn[e] - 2;
-    if (inside) {
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        bool f = false;
-#pragma unroll
-        for (int id = 0; id < 3; id++) {
-          if (id == d) continue;
-          const uint8_t m = (uint8_t)(1u << id);
-          f = f || (bits[i] & m) || (bits[i + s[id]] & m) ||
-              (bits[i - s[d]] & m) || (bits[i + s[id] - s[d]] & m);
+          for (int f = 0; f < 5; f++) U[f] = ld(a.src + f * a.fs, i);
+          prim_from_cons(U, mass, nmin, V);
         }
-        if (f) out |= (uint8_t)(1u << d);
+        Q[0][t] = gm1 * V[4];
+        Q[1][t] = V[1];
+        Q[2][t] = V[2];
+        Q[3][t] = V[3];
       }
     }
-    flags[i] = out;
+    __syncthreads();
+    if (plane && k >= z0 - 1) {
+      uint8_t *B = sB[(k + 3) % 3];
+      for (int t = tid; t < BH * BW; t += NT) {
+        const int bx = t % BW, by = t / BW;
+        const int x = x0 - 1 + bx, y = y0 - 1 + by;
+        const int q = (by + 1) * QW + (bx + 1);
+        uint8_t b = 0;
+        if ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny) {
+          const R P = Q[0][q];
+          // lower neighbour in x, y (same plane) and z (plane k-1)
+          const int cs[3] = {x, y, k};
+          const R Pm[3] = {Q[0][q - 1], Q[0][q - QW], Qm[0][q]};
+          const R dV[3] = {Q[1][q] - Q[1][q - 1], Q[2][q] - Q[2][q - QW],
+                           Q[3][q] - Qm[3][q]};
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            if (cs[d] == 0) continue;
+            const R dP = fabs(P - Pm[d]);
+            const R P_Min = fmax(fmin(P, Pm[d]), sqrt_tiny);
+            const R dLnP = Arith<R>::div(dP, P_Min);
+            if (dLnP > st && dV[d] <= R(0)) b |= (uint8_t)(1u << d);
+          }
+        }
+        B[t] = b;
+      }
+    }
+    __syncthreads();
+    const int kf = k - 1;  // flags of plane k - 1
+    if (kf >= z0 && kf < zend) {
+      const int tx = tid % TX, ty = tid / TX;
+      const int x = x0 + tx, y = y0 + ty;
+      if (x < nx && y < ny) {
+        uint8_t out = 0;
+        if (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && kf >= 1 && kf <= nz - 2) {
+          const uint8_t *Bm = sB[(kf + 2) % 3], *B0 = sB[(kf + 3) % 3],
+                        *Bp = sB[(kf + 4) % 3];
+          const int i = (ty + 1) * BW + (tx + 1);
+          const unsigned fx = (B0[i] | B0[i + BW] | B0[i - 1] | B0[i + BW - 1]) & 2u |
+                              (B0[i] | Bp[i] | B0[i - 1] | Bp[i - 1]) & 4u;
+          const unsigned fy = (B0[i] | B0[i + 1] | B0[i - BW] | B0[i + 1 - BW]) & 1u |
+                              (B0[i] | Bp[i] | B0[i - BW] | Bp[i - BW]) & 4u;
+          const unsigned fz = (B0[i] | B0[i + 1] | Bm[i] | Bm[i + 1]) & 1u |
+                              (B0[i] | B0[i + BW] | Bm[i] | Bm[i + BW]) & 2u;
+          out = (uint8_t)((fx ? 1u : 0u) | (fy ? 2u : 0u) | (fz ? 4u : 0u));
+        }
+        flags[lin(a.n, x, y, kf)] = out;
+      }
+    }
   }
 }
 
@@ -787,7 +819,7 @@ struct gx_ctx {
   int gs = 0;                 // geometry stride: max(n) + 4
   // five-field states, field f at base + f * ncell
   void *U = nullptr, *Y = nullptr, *W = nullptr, *P0 = nullptr;
-  uint8_t *flags = nullptr, *bits = nullptr;
+  uint8_t *flags = nullptr;
   unsigned long long *invdt = nullptr;
   double *partial = nullptr;
   int n_partial = 0;
@@ -795,7 +827,7 @@ struct gx_ctx {
   bool uploaded = false;
   bool prims_valid = false;   // next stage 1 reads P0
   bool invdt_valid = false;   // invdt holds the max for the current state
-  // per-kernel timing (gx_set_profiling): stage 1, stage 2, shock bits, flags
+  // per-kernel timing (gx_set_profiling): stage 1, stage 2, shock flags, (unused)
   bool profiling = false;
   hipEvent_t ev[5] = {};
   double kernel_ms[4] = {0, 0, 0, 0};
@@ -957,7 +989,6 @@ int gx_destroy(gx_ctx *c) {
   (void)hipFree(c->W);
   (void)hipFree(c->P0);
   (void)hipFree(c->flags);
-  (void)hipFree(c->bits);
   (void)hipFree(c->invdt);
   (void)hipFree(c->partial);
   (void)hipFree(c->scratch);
@@ -1078,7 +1109,6 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
   TRY_CLEAN(hipMalloc(&c->W, 5 * bytes));
   TRY_CLEAN(hipMalloc(&c->P0, 5 * bytes));
   TRY_CLEAN(hipMalloc(&c->flags, (size_t)c->ncell));
-  TRY_CLEAN(hipMalloc(&c->bits, (size_t)c->ncell));
   TRY_CLEAN(hipMemset(c->flags, 0, (size_t)c->ncell));
   TRY_CLEAN(hipMalloc(&c->invdt, sizeof(unsigned long long)));
   c->n_partial = grid_blocks(c->ncell);
@@ -1385,21 +1415,21 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     // Shock flags of the next step from the new state.
     StageArgs<R> a3 = base_args<R>(c);
     a3.src = (const R *)c->U;
-    shock_bits_kernel<R><<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a3, c->bits);
+    a3.tz = FLAG_TZ;
+    a3.nbx = nbx;
+    a3.nby = nby;
+    a3.nblk = nbx * nby * ((c->n[2] + FLAG_TZ - 1) / FLAG_TZ);
+    shock_flags_kernel<R><<<a3.nblk, NT, 0, c->stream>>>(a3, c->flags);
     HIP_TRY(hipGetLastError());
     return GX_OK;
   });
   if (rc) return rc;
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-  flags_kernel<<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
-      make_int3(c->n[0], c->n[1], c->n[2]), c->bits, c->flags);
-  HIP_TRY(hipGetLastError());
   if (c->profiling) {
-    // The stage-1 interval starts before the invdt reset; the reset is a
+    // The stage-1 interval ends after the invdt reset; the reset is a
     // few-byte fill and is counted with stage 1.
-    HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-    HIP_TRY(hipEventSynchronize(c->ev[4]));
-    for (int k = 0; k < 4; k++) {
+    HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[3]));
+    for (int k = 0; k < 3; k++) {
       float ms = 0.0f;
       HIP_TRY(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
       c->kernel_ms[k] += ms;

@@ -172,11 +172,11 @@ class Step_RK2_C_ASC:
         _capi.check(self.lib.gx_set_profiling(self._h, int(bool(on))), "gx_set_profiling")
 
     def kernel_times(self):
-        """{name: (total_ms, launches)} for stage 1, stage 2, shock bits, flags."""
+        """{name: (total_ms, launches)} for stage 1, stage 2 and the shock flags."""
         ms = (ctypes.c_double * 4)()
         n = (ctypes.c_longlong * 4)()
         _capi.check(self.lib.gx_kernel_times(self._h, ms, n), "gx_kernel_times")
-        names = ("stage1", "stage2", "shock_bits", "flags")
+        names = ("stage1", "stage2", "shock_flags")
         return {k: (ms[i], n[i]) for i, k in enumerate(names)}
 
     @property
