@@ -35,6 +35,11 @@ EXPORTS = (
     "gx_get_conserved", "gx_get_primitive", "gx_get_flags", "gx_set_flags",
     "gx_set_tuning", "gx_stream", "gx_set_profiling", "gx_kernel_times",
     "gx_self_test_division", "gx_last_error",
+    "gx_batch_create", "gx_batch_destroy", "gx_batch_set_primitive",
+    "gx_batch_set_integrator", "gx_batch_step", "gx_batch_sync", "gx_batch_cfl",
+    "gx_batch_times", "gx_batch_get_conserved", "gx_batch_get_flags",
+    "gx_batch_set_tuning", "gx_batch_get_tuning", "gx_batch_autotune",
+    "gx_batch_cells", "gx_batch_stream",
 )
 
 
@@ -103,6 +108,22 @@ def load(path: str = LIB_PATH):
         "gx_kernel_times": (i, [vp, pd, P(ctypes.c_longlong)]),
         "gx_self_test_division": (i, [ll, ctypes.c_ulonglong, P(ctypes.c_longlong)]),
         "gx_last_error": (ctypes.c_char_p, []),
+        "gx_batch_create": (i, [i, P(GxGrid), P(GxParams), P(vp)]),
+        "gx_batch_destroy": (i, [vp]),
+        "gx_batch_set_primitive": (i, [vp, i, pd]),
+        "gx_batch_set_integrator": (i, [vp, d, i, i]),
+        "gx_batch_step": (i, [vp, i]),
+        "gx_batch_sync": (i, [vp]),
+        "gx_batch_cfl": (i, [vp, pd]),
+        "gx_batch_times": (i, [vp, pd, pd, P(ctypes.c_int)]),
+        "gx_batch_get_conserved": (i, [vp, i, pd]),
+        "gx_batch_get_flags": (i, [vp, i, P(ctypes.c_ubyte)]),
+        "gx_batch_set_tuning": (i, [vp, i, P(ctypes.c_int), P(ctypes.c_int), i]),
+        "gx_batch_get_tuning": (i, [vp, i, P(ctypes.c_int), P(ctypes.c_int),
+                                    P(ctypes.c_int)]),
+        "gx_batch_autotune": (i, [vp, i]),
+        "gx_batch_cells": (ll, [vp]),
+        "gx_batch_stream": (vp, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

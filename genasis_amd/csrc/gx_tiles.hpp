@@ -1,0 +1,764 @@
+// gx_tiles.hpp -- device side of the Fluid_P_I finite-volume step: launch
+// geometry, ghost rule, the fused stage tile, the shock-flag tile and the
+// small whole-chart kernels, templated on the real type of the state.
+//
+// Device layout: the proper cells only (no ghost storage), five conserved
+// fields D, S_1, S_2, S_3, G as separate slabs of n0*n1*n2 values, x fastest.
+// Ghost values are produced on load by index mapping: periodic wrap, or the
+// mirror copy of OUTFLOW / REFLECTING (CH/SingleLevelCharts/
+// Chart_SL__Template.f90:486-538, Chart_SL__Kernel.f90:12-113).  This equals
+// the reference's ghost handling for every value its sweeps read: the swept
+// dimension's ghosts over the transverse interior.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/gx_fluid.h"
+#include "gx_kernels.hpp"
+
+namespace gx {
+
+constexpr int TX = 32;           // tile width in x (cells)
+constexpr int TY = 8;            // tile height in y
+constexpr int NT = TX * TY;      // threads per block
+constexpr int HX = TX + 4;       // plane tile with two ghost layers
+constexpr int HY = TY + 4;
+constexpr int SXW = TX + 2;      // x-slope row length
+constexpr int FXW = TX + 1;      // x-flux row length
+constexpr int WAVE = 64;
+constexpr int NXCD = 8;          // accelerator complex dies, one L2 each
+
+__host__ __device__ inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Field f of a five-field state lives at base + f * fs (fs = cells).  The
+// geometry is one buffer: array g of dimension d at geo + (d*N_GEO + g)*gs.
+// Few pointers keep the kernel's scalar registers free.  R is the real type
+// of the state and the sweep arithmetic (double, or float for the FP32
+// variant); physics constants stay double and are converted on use.
+template <typename R>
+struct StageArgs {
+  int n[3];
+  int bc[6];
+  int tz;               // planes per block (z-chunk)
+  int src_is_prim;      // stage input holds primitives (first stage after upload)
+  int gs;               // geometry stride (max n + 4)
+  int nbx, nby, nblk;   // stage-kernel tiles in x, in y, in total
+  long long fs;         // field stride (cells)
+  double dt;
+  Physics ph;
+  const R *src;         // stage input: conserved (or primitive) state
+  const R *aux;         // stage 1: U (Solution); stage 2: W = U + K1/2
+  R *out1;              // stage 1: Y = U + K1; stage 2: U_new
+  R *out2;              // stage 1: W
+  const R *geo;
+  const uint8_t *flags; // DiffusiveFlux bits of the step
+  unsigned long long *invdt_bits;  // stage 2: max CFL term (as FP64 bits)
+};
+
+// Waves per SIMD the stage kernel is compiled for: the FP32 tile uses half
+// the registers and LDS of the FP64 one.
+template <typename R> struct StageOccupancy { static constexpr int waves = 2; };
+template <> struct StageOccupancy<float> { static constexpr int waves = 3; };
+
+// Loads and stores at a 32-bit element offset from a base pointer: the byte
+// offset stays 32-bit (charts hold < 2^29 cells, checked at creation), so the
+// address is a scalar base plus one 32-bit VGPR.
+template <typename T>
+__device__ __forceinline__ T ld(const T *base, unsigned idx) {
+  return *(const T *)((const char *)base + (size_t)(idx * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st(T *base, unsigned idx, T v) {
+  *(T *)((char *)base + (size_t)(idx * (unsigned)sizeof(T))) = v;
+}
+
+template <typename R>
+__device__ __forceinline__ R geo(const StageArgs<R> &a, int d, int g, int p) {
+  return ld(a.geo + (d * N_GEO + g) * a.gs, (unsigned)p);
+}
+
+__device__ __forceinline__ unsigned lin(const int n[3], int x, int y, int z) {
+  return (unsigned)x + (unsigned)n[0] * ((unsigned)y + (unsigned)n[1] * (unsigned)z);
+}
+
+// Ghost index rule of one dimension: periodic wrap, or the mirror copy of
+// OUTFLOW / REFLECTING; *flip is set for a reflecting face.
+__device__ __forceinline__ int map_coord(int c, int n, int bc_in, int bc_out,
+                                         bool *flip) {
+  if (c < 0) {
+    if (bc_in == BC_PERIODIC) return c + n;
+    *flip = bc_in == BC_REFLECTING;
+    return -1 - c;
+  }
+  if (c >= n) {
+    if (bc_out == BC_PERIODIC) return c - n;
+    *flip = bc_out == BC_REFLECTING;
+    return 2 * n - 1 - c;
+  }
+  return c;
+}
+
+// Linear index of cell (x, y, z) with at most one coordinate outside the
+// chart by at most two cells, after the ghost rule of that face; *flip = the
+// dimension whose velocity a reflecting face negates, or -1.
+template <typename A>
+__device__ __forceinline__ unsigned cell_index(const A &a, int x, int y, int z,
+                                               int *flip) {
+  *flip = -1;
+  if ((unsigned)x < (unsigned)a.n[0] && (unsigned)y < (unsigned)a.n[1] &&
+      (unsigned)z < (unsigned)a.n[2])
+    return lin(a.n, x, y, z);
+  bool fx = false, fy = false, fz = false;
+  const int cx = map_coord(x, a.n[0], a.bc[0], a.bc[1], &fx);
+  const int cy = map_coord(y, a.n[1], a.bc[2], a.bc[3], &fy);
+  const int cz = map_coord(z, a.n[2], a.bc[4], a.bc[5], &fz);
+  *flip = fx ? 0 : (fy ? 1 : (fz ? 2 : -1));
+  return lin(a.n, cx, cy, cz);
+}
+
+// Raw stage input (conserved, or primitive after an upload) of one cell.
+template <typename R>
+__device__ __forceinline__ void load_raw(const StageArgs<R> &a, int x, int y, int z,
+                                         R raw[5], int *flip) {
+  const unsigned i = cell_index(a, x, y, z, flip);
+#pragma unroll
+  for (int f = 0; f < 5; f++) raw[f] = ld(a.src + f * a.fs, i);
+}
+
+// -v when f, as a sign-bit operation: a select on an array element is turned
+// into an indexed (scratch) store by the compiler.
+__device__ __forceinline__ double negate_if(double v, bool f) {
+  return __longlong_as_double(__double_as_longlong(v) ^
+                              (f ? (long long)0x8000000000000000ull : 0ll));
+}
+__device__ __forceinline__ float negate_if(float v, bool f) {
+  return __int_as_float(__float_as_int(v) ^ (f ? (int)0x80000000u : 0));
+}
+
+// Primitives of a raw value, with the reflecting sign flip.
+template <typename R>
+__device__ __forceinline__ void raw_to_prim(const StageArgs<R> &a, const R raw[5],
+                                            int flip, R V[5]) {
+  if (a.src_is_prim) {
+#pragma unroll
+    for (int f = 0; f < 5; f++) V[f] = raw[f];
+  } else {
+    prim_from_cons(raw, R(a.ph.mass), R(a.ph.nmin), V);
+  }
+  V[1] = negate_if(V[1], flip == 0);
+  V[2] = negate_if(V[2], flip == 1);
+  V[3] = negate_if(V[3], flip == 2);
+}
+
+template <typename R>
+__device__ __forceinline__ void load_prim(const StageArgs<R> &a, int x, int y,
+                                          int z, R V[5]) {
+  R raw[5];
+  int flip;
+  load_raw(a, x, y, z, raw, &flip);
+  raw_to_prim(a, raw, flip, V);
+}
+
+template <typename R>
+__device__ __forceinline__ R slope_at(const StageArgs<R> &a, int d, int c, R vm,
+                                      R v0, R vp) {
+  const R *g = a.geo + d * N_GEO * a.gs + (c + 2);
+  const int s = a.gs;
+  return slope(vm, v0, vp, g[G_DXL * s], g[G_DXR * s], g[G_DXL2 * s], g[G_DXR2 * s],
+               g[G_DEN * s], g[G_RDXL * s], g[G_RDXR * s], g[G_RDEN * s],
+               R(a.ph.theta), a.ph.use_limiter);
+}
+
+// XCD-aware tile order.  The dispatcher deals consecutive blocks to the XCDs
+// in turn; block b is given tile t so that every XCD owns one contiguous run
+// of tiles (x fastest, then y, then z-chunk).  Tiles that share halo cache
+// lines then run on the same L2 at about the same time.  A bijection of
+// [0, nblk): XCD j holds q + (j < r) blocks and tiles from j*q + min(j, r).
+__device__ __forceinline__ int tile_of_block(int b, int nblk) {
+  const int q = nblk / NXCD, r = nblk % NXCD;
+  const int xcd = b % NXCD, slot = b / NXCD;
+  return xcd * q + min(xcd, r) + slot;
+}
+
+// Halo cell t of the plane tile: 4 per row for x (two each side), 4 per
+// column for y.
+__device__ __forceinline__ void halo_cell(int t, int x0, int y0, int tw, int th,
+                                          int *hx, int *hy) {
+  if (t < 4 * th) {
+    const int j = t >> 2, s = t & 3;
+    *hy = y0 + j;
+    *hx = s < 2 ? x0 - 2 + s : x0 + tw + (s - 2);
+  } else {
+    const int u = t - 4 * th;
+    const int i = u >> 2, s = u & 3;
+    *hx = x0 + i;
+    *hy = s < 2 ? y0 - 2 + s : y0 + th + (s - 2);
+  }
+}
+
+// Geometry of the tile in LDS: array g of dimension d (0 = x, 1 = y) at the
+// local index c - origin + 2 for c in [origin-2, origin+TX+2).
+template <typename R> using TileGeo = R[2][N_GEO][HX];
+
+template <typename R>
+__device__ __forceinline__ R slope_lds(const TileGeo<R> &sG, int d, int l, R vm,
+                                       R v0, R vp, R theta, int use_limiter) {
+  return slope(vm, v0, vp, sG[d][G_DXL][l], sG[d][G_DXR][l], sG[d][G_DXL2][l],
+               sG[d][G_DXR2][l], sG[d][G_DEN][l], sG[d][G_RDXL][l],
+               sG[d][G_RDXR][l], sG[d][G_RDEN][l], theta, use_limiter);
+}
+
+// x-face i (between cells x0+i-1 and x0+i) of row j into sFx.
+template <typename R, typename SP, typename SS, typename SF>
+__device__ __forceinline__ void x_face(const Physics &ph, SP sP, SS sSx, SF sFx,
+                                       const TileGeo<R> &sG, int i, int j, bool df) {
+  const int bl = (j + 2) * HX + (i + 1);
+  const R wr = sG[0][G_WR][i + 1], wl = sG[0][G_WL][i + 2];
+  R WL[5], WR[5], F[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    WL[f] = sP[f][bl] + wr * sSx[f][j * SXW + i];
+    WR[f] = sP[f][bl + 1] - wl * sSx[f][j * SXW + i + 1];
+  }
+  face_flux(WL, WR, 0, df, ph, F);
+#pragma unroll
+  for (int f = 0; f < 5; f++) sFx[f][j * FXW + i] = F[f];
+}
+
+// y-face j (between rows y0+j-1 and y0+j) of column i into sFy.
+template <typename R, typename SP, typename SS, typename SF>
+__device__ __forceinline__ void y_face(const Physics &ph, SP sP, SS sSy, SF sFy,
+                                       const TileGeo<R> &sG, int i, int j, bool df) {
+  const int bl = (j + 1) * HX + (i + 2);
+  const R wr = sG[1][G_WR][j + 1], wl = sG[1][G_WL][j + 2];
+  R WL[5], WR[5], F[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    WL[f] = sP[f][bl] + wr * sSy[f][j * TX + i];
+    WR[f] = sP[f][bl + HX] - wl * sSy[f][(j + 1) * TX + i];
+  }
+  face_flux(WL, WR, 1, df, ph, F);
+#pragma unroll
+  for (int f = 0; f < 5; f++) sFy[f][j * TX + i] = F[f];
+}
+
+// Fused stage: K = dt * (-div F) over the three dimensions, then the RK
+// combination.  A block owns a TX x TY column tile and marches along z over
+// a.tz planes; x/y stencils come from the plane tile in LDS, the z stencil
+// from registers.  Every thread computes the x- and y-slopes and the left and
+// bottom faces of its own cell; the tile-edge slopes and faces are extra
+// tasks.  Global loads of plane k+1 (halo, z-stencil) are issued during plane
+// k, and the RK input of plane k at its start, so that their latency overlaps
+// the slope and flux work.  Stage 2 applies the conserved fix and the CFL term
+// in FP64 whatever R is.
+// LDS of one stage tile.
+template <typename R> struct StageShared {
+  R sP[5][HY * HX];
+  R sSx[5][TY * SXW];       // x-slopes of cells x0-1 .. x0+tw
+  R sSy[5][(TY + 2) * TX];  // y-slopes of cells y0-1 .. y0+th
+  R sFx[5][TY * FXW];       // x-face fluxes x0 .. x0+tw
+  R sFy[5][(TY + 1) * TX];  // y-face fluxes y0 .. y0+th
+  double sRed[NT / WAVE];
+  TileGeo<R> sG;
+};
+
+template <typename R, int STAGE>
+__device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0, int z0,
+                                           StageShared<R> &L) {
+  auto &sP = L.sP;
+  auto &sSx = L.sSx;
+  auto &sSy = L.sSy;
+  auto &sFx = L.sFx;
+  auto &sFy = L.sFy;
+  auto &sRed = L.sRed;
+  auto &sG = L.sG;
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+  const int tw = min(TX, nx - x0), th = min(TY, ny - y0);
+  const int zend = min(z0 + a.tz, nz);
+  const int tid = threadIdx.x;
+  // Tile geometry: x for cells x0-2 .. x0+tw+1, y for y0-2 .. y0+th+1.
+  for (int t = tid; t < 2 * N_GEO * HX; t += NT) {
+    const int d = t / (N_GEO * HX), g = (t / HX) % N_GEO, l = t % HX;
+    const int o = d == 0 ? x0 : y0, w = d == 0 ? tw : th;
+    sG[d][g][l] = l < w + 4 ? geo(a, d, g, o + l) : R(0);
+  }
+  const int tx = tid % TX, ty = tid / TX;
+  const bool active = tx < tw && ty < th;
+  const int x = x0 + tx, y = y0 + ty;
+  const int ctr = (ty + 2) * HX + tx + 2;  // own cell in the plane tile
+  const Physics ph = a.ph;
+  const R theta = R(ph.theta);
+  const int nhalo = 4 * th + 4 * tw;
+  const int nxs = 2 * th, nys = 2 * tw;  // extra slope tasks
+
+  R Vk[5], Vk1[5], sk[5], Flo[5];
+  R zr[5], hr[5];  // prefetched raw values: z-stencil k+2, halo of k
+  int zflip = -1, hflip = -1, hpos = 0;
+  unsigned flk = 0;     // flag byte of the own cell at plane k (then k+1)
+  double tmax = 0.0;
+
+  if (active) {
+    R Vm2[5], Vm1[5], sm1[5], WL[5], WR[5];
+    load_prim(a, x, y, z0 - 2, Vm2);
+    load_prim(a, x, y, z0 - 1, Vm1);
+    load_prim(a, x, y, z0, Vk);
+    load_prim(a, x, y, z0 + 1, Vk1);
+    load_raw(a, x, y, z0 + 2, zr, &zflip);
+    flk = ld(a.flags, lin(a.n, x, y, z0));
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      sm1[f] = slope_at(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f]);
+      sk[f] = slope_at(a, 2, z0, Vm1[f], Vk[f], Vk1[f]);
+      WL[f] = Vm1[f] + geo(a, 2, G_WR, z0 - 1 + 2) * sm1[f];
+      WR[f] = Vk[f] - geo(a, 2, G_WL, z0 + 2) * sk[f];
+    }
+    face_flux(WL, WR, 2, (flk >> 2) & 1, ph, Flo);
+  }
+  if (tid < nhalo) {
+    int hx, hy;
+    halo_cell(tid, x0, y0, tw, th, &hx, &hy);
+    hpos = (hy - y0 + 2) * HX + (hx - x0 + 2);
+    load_raw(a, hx, hy, z0, hr, &hflip);
+  }
+
+  for (int k = z0; k < zend; ++k) {
+    const bool more = k + 1 < zend;
+    R ar[5], zn[5];
+    int znflip = -1;
+    unsigned fln = 0, flx = 0;  // own flags at k+1; flag of this thread's edge face
+    if (active) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) sP[f][ctr] = Vk[f];
+    }
+    if (tid < nhalo) {
+      R V[5];
+      raw_to_prim(a, hr, hflip, V);
+#pragma unroll
+      for (int f = 0; f < 5; f++) sP[f][hpos] = V[f];
+      if (more) {
+        int hx, hy;
+        halo_cell(tid, x0, y0, tw, th, &hx, &hy);
+        load_raw(a, hx, hy, k + 1, hr, &hflip);
+      }
+    }
+    if (active) {
+      const unsigned ic = lin(a.n, x, y, k);
+#pragma unroll
+      for (int f = 0; f < 5; f++) ar[f] = ld(a.aux + f * a.fs, ic);
+      if (more) load_raw(a, x, y, k + 3, zn, &znflip);
+      if (k + 1 < nz) fln = ld(a.flags, lin(a.n, x, y, k + 1));
+    }
+    if (tid < th) {
+      if (x0 + tw < nx) flx = ld(a.flags, lin(a.n, x0 + tw, y0 + tid, k)) & 1u;
+    } else if (tid >= WAVE && tid - WAVE < tw) {
+      if (y0 + th < ny) flx = (ld(a.flags, lin(a.n, x0 + tid - WAVE, y0 + th, k)) >> 1) & 1u;
+    }
+    __syncthreads();
+
+    // Slopes: own cell (x and y), then the tile-edge cells.
+    if (active) {
+      const int bx = ctr;
+#pragma unroll
+      for (int f = 0; f < 5; f++)
+        sSx[f][ty * SXW + tx + 1] = slope_lds(sG, 0, tx + 2, sP[f][bx - 1], sP[f][bx],
+                                              sP[f][bx + 1], theta, ph.use_limiter);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int f = 0; f < 5; f++)
+        sSy[f][(ty + 1) * TX + tx] = slope_lds(sG, 1, ty + 2, sP[f][bx - HX], sP[f][bx],
+                                               sP[f][bx + HX], theta, ph.use_limiter);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (tid < nxs + nys) {
+      int base, stride, l, out, gd;
+      bool isx = tid < nxs;
+      if (isx) {
+        const int j = tid >> 1, side = tid & 1;
+        const int i = side ? tw + 1 : 0;  // slope index of cell x0-1+i
+        base = (j + 2) * HX + (i + 1);
+        stride = 1;
+        l = i + 1;  // local geometry index of cell x0-1+i
+        gd = 0;
+        out = j * SXW + i;
+      } else {
+        const int u = tid - nxs;
+        const int i = u >> 1, side = u & 1;
+        const int j = side ? th + 1 : 0;  // slope index of cell y0-1+j
+        base = (j + 1) * HX + (i + 2);
+        stride = HX;
+        l = j + 1;
+        gd = 1;
+        out = j * TX + i;
+      }
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        const R sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
+                               sP[f][base + stride], theta, ph.use_limiter);
+        if (isx)
+          sSx[f][out] = sv;
+        else
+          sSy[f][out] = sv;
+      }
+    }
+    __syncthreads();
+
+    // Faces: left and bottom faces of the own cell, then the right column of
+    // x-faces (wave 0) and the top row of y-faces (wave 1).
+    if (active) {
+      x_face(ph, sP, sSx, sFx, sG, tx, ty, flk & 1u);
+      __builtin_amdgcn_sched_barrier(0);
+      y_face(ph, sP, sSy, sFy, sG, tx, ty, (flk >> 1) & 1u);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (tid < th) x_face(ph, sP, sSx, sFx, sG, tw, tid, flx);
+    __builtin_amdgcn_sched_barrier(0);
+    if (tid >= WAVE && tid - WAVE < tw) y_face(ph, sP, sSy, sFy, sG, tid - WAVE, th, flx);
+    __syncthreads();
+
+    if (active) {
+      // z stencil of plane k: slope at k+1, flux through the face k+1/2.
+      R Vk2[5], sk1[5], Fhi[5];
+      {
+        R WL[5], WR[5];
+        raw_to_prim(a, zr, zflip, Vk2);
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          const R vk = sP[f][ctr];
+          sk1[f] = slope_at(a, 2, k + 1, vk, Vk1[f], Vk2[f]);
+          WL[f] = vk + geo(a, 2, G_WR, k + 2) * sk[f];
+          WR[f] = Vk1[f] - geo(a, 2, G_WL, k + 1 + 2) * sk1[f];
+        }
+        face_flux(WL, WR, 2, (fln >> 2) & 1u, ph, Fhi);
+      }
+      // ComputeIncrement_CSL_Kernel (EV/Increments/IncrementDivergence_FV__
+      // Kernel.f90:672-752), dimensions in order 1, 2, 3, onto K = 0; the
+      // division by the volume uses its reciprocal (FP64: rounded exactly).
+      const R dX = sG[0][G_WIDTH][tx + 2], dY = sG[1][G_WIDTH][ty + 2],
+              dZ = geo(a, 2, G_WIDTH, k + 2);
+      const R Vol = dX * dY * dZ;
+      const R rVol = Arith<R>::recip(Vol);
+      const R A1 = dY * dZ, A2 = dZ * dX, A3 = dX * dY;
+      const R dt = R(a.dt);
+      const unsigned ic = lin(a.n, x, y, k);
+      R Kc[5];
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        const R fxl = sFx[f][ty * FXW + tx], fxh = sFx[f][ty * FXW + tx + 1];
+        const R fyl = sFy[f][ty * TX + tx], fyh = sFy[f][(ty + 1) * TX + tx];
+        R K = R(0);
+        K = K + Arith<R>::rdiv(dt * (A1 * fxl - A1 * fxh), Vol, rVol);
+        K = K + Arith<R>::rdiv(dt * (A2 * fyl - A2 * fyh), Vol, rVol);
+        K = K + Arith<R>::rdiv(dt * (A3 * Flo[f] - A3 * Fhi[f]), Vol, rVol);
+        Kc[f] = K;
+      }
+      if (STAGE == 1) {
+        // Y = Solution + 1*K1; Solution + 1/2 K1 kept for the final combination
+        // (Step_RK__Template.f90:240-278, MultiplyAdd_Command.f90:56-94).
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          st(a.out1 + f * a.fs, ic, ar[f] + R(1) * Kc[f]);
+          st(a.out2 + f * a.fs, ic, ar[f] + R(0.5) * Kc[f]);
+        }
+      } else {
+        // Solution = (Solution + 1/2 K1) + 1/2 K2, then StoreSolution's
+        // conserved fix and the CFL term of the next step in FP64.
+        double U[5], V[5];
+#pragma unroll
+        for (int f = 0; f < 5; f++) U[f] = (double)(ar[f] + R(0.5) * Kc[f]);
+        fix_conserved(U, ph.mass, ph.nmin, V);
+#pragma unroll
+        for (int f = 0; f < 5; f++) st(a.out1 + f * a.fs, ic, (R)U[f]);
+        const double tc = cfl_term(V, ph.gamma, ph.mass, (double)dX, (double)dY,
+                                   (double)dZ);
+        tmax = tc > tmax ? tc : tmax;
+      }
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        Flo[f] = Fhi[f];
+        sk[f] = sk1[f];
+        Vk[f] = Vk1[f];
+        Vk1[f] = Vk2[f];
+        zr[f] = zn[f];
+      }
+      zflip = znflip;
+      flk = fln;
+    }
+    // No barrier here: the next plane writes only the own centre and the
+    // halo cells, which nobody reads after the face barrier, and its first
+    // barrier orders the slope and flux buffers.
+  }
+
+  if (STAGE == 2) {
+    // Block max of the non-negative CFL terms; IEEE order of non-negative
+    // doubles is the order of their bit patterns.
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+      const double v = __shfl_xor(tmax, o, WAVE);
+      tmax = v > tmax ? v : tmax;
+    }
+    if ((tid % WAVE) == 0) sRed[tid / WAVE] = tmax;
+    __syncthreads();
+    if (tid == 0) {
+      double m = 0.0;
+      for (int w = 0; w < NT / WAVE; w++) m = sRed[w] > m ? sRed[w] : m;
+      atomicMax(a.invdt_bits, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+template <typename R, int STAGE>
+__global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel(
+    StageArgs<R> a) {
+  __shared__ StageShared<R> L;
+  const int tile = tile_of_block(blockIdx.x, a.nblk);
+  stage_tile<R, STAGE>(a, (tile % a.nbx) * TX, ((tile / a.nbx) % a.nby) * TY,
+                       (tile / (a.nbx * a.nby)) * a.tz, L);
+}
+
+// Max CFL term over the current state (the first step after an upload, or
+// whenever the stage-2 value is not available), in FP64.
+template <typename R>
+__global__ __launch_bounds__(256) void cfl_kernel(StageArgs<R> a) {
+  __shared__ double sRed[256 / WAVE];
+  const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
+  double tmax = 0.0;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    const int x = (int)(i % a.n[0]);
+    const int y = (int)((i / a.n[0]) % a.n[1]);
+    const int z = (int)(i / ((long long)a.n[0] * a.n[1]));
+    R Vr[5];
+    load_prim(a, x, y, z, Vr);
+    double V[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) V[f] = (double)Vr[f];
+    const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, (double)geo(a, 0, G_WIDTH, x + 2),
+                               (double)geo(a, 1, G_WIDTH, y + 2),
+                               (double)geo(a, 2, G_WIDTH, z + 2));
+    tmax = tc > tmax ? tc : tmax;
+  }
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    const double v = __shfl_xor(tmax, o, WAVE);
+    tmax = v > tmax ? v : tmax;
+  }
+  if ((threadIdx.x % WAVE) == 0) sRed[threadIdx.x / WAVE] = tmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < 256 / WAVE; w++) m = sRed[w] > m ? sRed[w] : m;
+    atomicMax(a.invdt_bits, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// Shock flags of the next step from the stored state, in one pass per (x, y)
+// tile marching along z.  DetectShocks_CSL_Kernel (F/FluidFeatures_P__Kernel.
+// f90:12-186) tests each cell against its lower neighbour in every dimension
+// i -- |dP|/max(min(P), sqrt(tiny)) > ShockThreshold and dV_i <= 0 -- and a
+// shock marks the DiffusiveFlux flags of the transverse faces on both sides
+// of the two cells it separates (:137-147); ClearBoundary_CSL_Kernel
+// (:189-313) then clears the first and last two layers of every dimension.
+// So flags live on cells 1 .. n-2 and read shock tests of chart cells only:
+// flag_d(c) = OR over i != d of test_i at c, c + e_i, c - e_d, c + e_i - e_d.
+// Per plane k: P and V of the tile with two cells of halo below and one
+// above go to LDS (ping-pong with plane k-1 for the z test); the tests of
+// the tile with a one-cell halo go to a ring of three planes; the flags of
+// plane k-1 follow from the ring.
+constexpr int QW = TX + 3, QH = TY + 3;  // primitives of x0-2 .. x0+TX, y0-2 .. y0+TY
+constexpr int BW = TX + 2, BH = TY + 2;  // tests of x0-1 .. x0+TX, y0-1 .. y0+TY
+constexpr int FLAG_TZ = 32;              // planes per block
+
+template <typename R> struct FlagShared {
+  R sQ[2][4][QH * QW];     // P, V_1, V_2, V_3 of planes k and k-1
+  uint8_t sB[3][BH * BW];  // tests of plane p at (p + 3) % 3
+};
+
+template <typename R>
+__device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, int y0,
+                                                 int z0, uint8_t *flags, FlagShared<R> &L) {
+  auto &sQ = L.sQ;
+  auto &sB = L.sB;
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+  const int zend = min(z0 + a.tz, nz);
+  const int tid = threadIdx.x;
+  const R gm1 = R(a.ph.gamma) - R(1), st = R(a.ph.st);
+  const R mass = R(a.ph.mass), nmin = R(a.ph.nmin);
+  const R sqrt_tiny = Arith<R>::sqrt_tiny;
+
+  for (int k = z0 - 2; k <= zend; ++k) {
+    R(*Q)[QH * QW] = sQ[(k + 2) & 1];
+    R(*Qm)[QH * QW] = sQ[(k + 3) & 1];
+    const bool plane = k >= 0 && k < nz;
+    if (plane) {
+      for (int t = tid; t < QH * QW; t += NT) {
+        const int x = x0 - 2 + t % QW, y = y0 - 2 + t / QW;
+        R V[5] = {R(0), R(0), R(0), R(0), R(0)};
+        if ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny) {
+          const unsigned i = lin(a.n, x, y, k);
+          R U[5];
+#pragma unroll
+          for (int f = 0; f < 5; f++) U[f] = ld(a.src + f * a.fs, i);
+          prim_from_cons(U, mass, nmin, V);
+        }
+        Q[0][t] = gm1 * V[4];
+        Q[1][t] = V[1];
+        Q[2][t] = V[2];
+        Q[3][t] = V[3];
+      }
+    }
+    __syncthreads();
+    if (plane && k >= z0 - 1) {
+      uint8_t *B = sB[(k + 3) % 3];
+      for (int t = tid; t < BH * BW; t += NT) {
+        const int bx = t % BW, by = t / BW;
+        const int x = x0 - 1 + bx, y = y0 - 1 + by;
+        const int q = (by + 1) * QW + (bx + 1);
+        uint8_t b = 0;
+        if ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny) {
+          const R P = Q[0][q];
+          // lower neighbour in x, y (same plane) and z (plane k-1)
+          const int cs[3] = {x, y, k};
+          const R Pm[3] = {Q[0][q - 1], Q[0][q - QW], Qm[0][q]};
+          const R dV[3] = {Q[1][q] - Q[1][q - 1], Q[2][q] - Q[2][q - QW],
+                           Q[3][q] - Qm[3][q]};
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            if (cs[d] == 0) continue;
+            const R dP = fabs(P - Pm[d]);
+            const R P_Min = fmax(fmin(P, Pm[d]), sqrt_tiny);
+            const R dLnP = Arith<R>::div(dP, P_Min);
+            if (dLnP > st && dV[d] <= R(0)) b |= (uint8_t)(1u << d);
+          }
+        }
+        B[t] = b;
+      }
+    }
+    __syncthreads();
+    const int kf = k - 1;  // flags of plane k - 1
+    if (kf >= z0 && kf < zend) {
+      const int tx = tid % TX, ty = tid / TX;
+      const int x = x0 + tx, y = y0 + ty;
+      if (x < nx && y < ny) {
+        uint8_t out = 0;
+        if (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && kf >= 1 && kf <= nz - 2) {
+          const uint8_t *Bm = sB[(kf + 2) % 3], *B0 = sB[(kf + 3) % 3],
+                        *Bp = sB[(kf + 4) % 3];
+          const int i = (ty + 1) * BW + (tx + 1);
+          const unsigned fx = (B0[i] | B0[i + BW] | B0[i - 1] | B0[i + BW - 1]) & 2u |
+                              (B0[i] | Bp[i] | B0[i - 1] | Bp[i - 1]) & 4u;
+          const unsigned fy = (B0[i] | B0[i + 1] | B0[i - BW] | B0[i + 1 - BW]) & 1u |
+                              (B0[i] | Bp[i] | B0[i - BW] | Bp[i - BW]) & 4u;
+          const unsigned fz = (B0[i] | B0[i + 1] | Bm[i] | Bm[i + 1]) & 1u |
+                              (B0[i] | B0[i + BW] | Bm[i] | Bm[i + BW]) & 2u;
+          out = (uint8_t)((fx ? 1u : 0u) | (fy ? 2u : 0u) | (fz ? 4u : 0u));
+        }
+        flags[lin(a.n, x, y, kf)] = out;
+      }
+    }
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(NT) void shock_flags_kernel(StageArgs<R> a, uint8_t *flags) {
+  __shared__ FlagShared<R> L;
+  const int tile = tile_of_block(blockIdx.x, a.nblk);
+  shock_flags_tile(a, (tile % a.nbx) * TX, ((tile / a.nbx) % a.nby) * TY,
+                   (tile / (a.nbx * a.nby)) * a.tz, flags, L);
+}
+
+// Current fields of ComputeFromConservedCommon for download
+// (F/Fluid_P_I__Form.f90:604-722), proper cells, on the stored state, in FP64.
+template <typename R>
+__global__ __launch_bounds__(256) void current_kernel(StageArgs<R> a, double c_v,
+                                                      double *out) {
+  const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
+  const double sqrt_huge = 1.3407807929942596e+154;  // sqrt(DBL_MAX)
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    double U[5], V[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) U[f] = (double)a.aux[f * a.fs + i];
+    fix_conserved(U, a.ph.mass, a.ph.nmin, V);
+    double *o = out;
+    const double g = a.ph.gamma, M = a.ph.mass;
+    const double N = V[0], E = V[4];
+    const double P = (g - 1.0) * E;
+    double T, SB, CS;
+    if (N > 0.0 && P > 0.0) {
+      T = E / (c_v * N);
+      SB = c_v * log(P / 1.0 * pow(1.0 / N, g));
+      CS = sqrt(g * P / (M * N));
+    } else {
+      T = 0.0;
+      SB = -sqrt_huge;
+      CS = 0.0;
+    }
+    double MN = 0.0;
+    if (CS > 0.0) MN = sqrt((V[1] * V[1] + V[2] * V[2]) + V[3] * V[3]) / CS;
+    const double vals[GX_N_FIELDS] = {
+        V[1] + CS, V[2] + CS, V[3] + CS, V[1] - CS, V[2] - CS, V[3] - CS,
+        N,         U[0],      M,         V[1],      V[2],      V[3],
+        U[1],      U[2],      U[3],      E,         U[4],      P,
+        T,         SB,        SB * N,    CS,        MN};
+#pragma unroll
+    for (int f = 0; f < GX_N_FIELDS; f++) o[(long long)f * ntot + i] = vals[f];
+  }
+}
+
+// ComputeFromInitial = ComputeFromPrimitiveCommon on the Current
+// (F/Fluid_P_I__Form.f90:509-601): the density floor N<0 -> 0 is written back
+// to the primitive field, then D, S, G.
+template <typename R>
+__global__ __launch_bounds__(256) void from_primitive_kernel(long long ntot,
+                                                             double mass_d, R *P,
+                                                             R *U) {
+  const R mass = R(mass_d);
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    R N = P[i];
+    if (N < R(0)) {
+      N = R(0);
+      P[i] = N;
+    }
+    const R V1 = P[ntot + i], V2 = P[2 * ntot + i], V3 = P[3 * ntot + i];
+    const R S1 = mass * N * V1, S2 = mass * N * V2, S3 = mass * N * V3;
+    U[i] = N;
+    U[ntot + i] = S1;
+    U[2 * ntot + i] = S2;
+    U[3 * ntot + i] = S3;
+    U[4 * ntot + i] = P[4 * ntot + i] + R(0.5) * ((S1 * V1 + S2 * V2) + S3 * V3);
+  }
+}
+
+// Per-block partial sums of U * volume over proper cells, in FP64.
+template <typename R>
+__global__ __launch_bounds__(256) void totals_kernel(StageArgs<R> a,
+                                                     double *partial) {
+  __shared__ double s[5][256];
+  const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
+       i += (long long)gridDim.x * 256) {
+    const int x = (int)(i % a.n[0]);
+    const int y = (int)((i / a.n[0]) % a.n[1]);
+    const int z = (int)(i / ((long long)a.n[0] * a.n[1]));
+    const double vol = (double)geo(a, 0, G_WIDTH, x + 2) *
+                       (double)geo(a, 1, G_WIDTH, y + 2) *
+                       (double)geo(a, 2, G_WIDTH, z + 2);
+#pragma unroll
+    for (int f = 0; f < 5; f++) acc[f] += (double)a.aux[f * a.fs + i] * vol;
+  }
+#pragma unroll
+  for (int f = 0; f < 5; f++) s[f][threadIdx.x] = acc[f];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int f = 0; f < 5; f++) s[f][threadIdx.x] += s[f][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) partial[blockIdx.x * 5 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+}  // namespace gx

@@ -104,3 +104,24 @@ def riemann_ensemble_member(chart: Chart, rng: np.random.Generator,
     x0 = 0.5 * (chart.lo[axis] + chart.hi[axis])
     return sod(chart, gamma, axis=axis, x0=x0,
                left=(rho[0], p[0], v[0]), right=(rho[1], p[1], v[1])), axis
+
+
+def riemann_ensemble_sizes(n_problems: int, seed: int, n_min: int = 32,
+                           n_max: int = 128) -> list:
+    """Edge sizes of a Riemann-problem ensemble: round(exp(U(ln n_min,
+    ln n_max))) per problem from a seeded generator (cubic charts)."""
+    rng = np.random.default_rng([seed, 0])
+    u = rng.uniform(np.log(n_min), np.log(n_max), n_problems)
+    return [int(round(float(np.exp(x)))) for x in u]
+
+
+def riemann_ensemble_problem(k: int, n: int, seed: int, padded: bool = False):
+    """Problem k of the ensemble: an n^3 OUTFLOW unit cube with a random
+    Riemann problem (riemann_ensemble_member) from the generator seeded by
+    (seed, k + 1).  Returns (chart, primitives, axis)."""
+    chart = Chart((n, n, n), bc=(("OUTFLOW", "OUTFLOW"),) * 3)
+    rng = np.random.default_rng([seed, k + 1])
+    prim, axis = riemann_ensemble_member(chart, rng)
+    if not padded:
+        prim = tuple(np.ascontiguousarray(chart.proper(f)) for f in prim)
+    return chart, prim, axis

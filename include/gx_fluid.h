@@ -190,6 +190,68 @@ int gx_self_test_division(long long n, unsigned long long seed,
 
 const char *gx_last_error(void);
 
+/* ---------------------------------------------------------------------------
+ * Batches: many independent charts (of any sizes) advanced together, each with
+ * its own integrator state.  One batch call does for every problem what a
+ * loop over separate FluidBox programs does for each (FluidBox_Form %
+ * Initialize, Modules/Physics/Universes/FluidBox_Form.f90:37-92; Integrator_
+ * C_PS % ComputeCycle_ASC, Integrator_C_PS__Form.f90:328-371), and gives
+ * each problem the bits a gx_ctx of the same chart gives it.  Every RK stage
+ * of all problems is one persistent launch; gx_batch_step captures its
+ * cycles in one HIP graph.  Parameters are shared by the problems.
+ * ------------------------------------------------------------------------- */
+typedef struct gx_batch gx_batch;
+
+int gx_batch_create(int nProblems, const gx_grid *grids, const gx_params *params,
+                    gx_batch **batch);
+int gx_batch_destroy(gx_batch *batch);
+
+/* Primitives (N, V_1..3, E: five contiguous proper-cell fields) of problem
+ * iProblem; the conserved fields follow on the device as ComputeFromInitial
+ * does.  Resets the problem's Time and cycle count to 0. */
+int gx_batch_set_primitive(gx_batch *batch, int iProblem, const double *primitive);
+
+/* Integrator parameters of every problem (IntegratorHeader_Form.f90:95-131):
+ * FinishTime (default 1), FinishCycle (default huge), nRampCycles (100). */
+int gx_batch_set_integrator(gx_batch *batch, double FinishTime, int FinishCycle,
+                            int nRampCycles);
+
+/* nSteps cycles of Integrator_C_PS % ComputeCycle_ASC for every problem:
+ * dt from the problem's own CFL candidate (ComputeTimeStep_C_ASC), times the
+ * ramp min((iCycle+1)/nRampCycles, 1), clipped onto FinishTime; a problem at
+ * FinishTime or FinishCycle is not stepped.  Stream-ordered, asynchronous. */
+int gx_batch_step(gx_batch *batch, int nSteps);
+int gx_batch_sync(gx_batch *batch);
+
+/* ComputeTimeStep_C_ASC of every problem: candidates[nProblems].
+ * Synchronises. */
+int gx_batch_cfl(gx_batch *batch, double *candidates);
+
+/* Time, last time step and cycle count of every problem (lastTimeStep and
+ * cycle may be NULL).  Synchronises. */
+int gx_batch_times(gx_batch *batch, double *time, double *lastTimeStep, int *cycle);
+
+/* Proper-cell views of problem iProblem, as gx_get_conserved / gx_get_flags. */
+int gx_batch_get_conserved(gx_batch *batch, int iProblem, double *conserved);
+int gx_batch_get_flags(gx_batch *batch, int iProblem, unsigned char *flags);
+
+/* Launch shape: the z-chunk of each size class (a size class is
+ * floor(2*log2(largest edge)), half-octaves), and whether gx_batch_step
+ * captures a HIP graph (useGraph) or launches its kernels one by one. */
+int gx_batch_set_tuning(gx_batch *batch, int nClasses, const int *sizeClass,
+                        const int *zChunk, int useGraph);
+/* Up to capacity (class, z-chunk) pairs; *nClasses = number of classes. */
+int gx_batch_get_tuning(gx_batch *batch, int capacity, int *sizeClass, int *zChunk,
+                        int *nClasses);
+/* Chooses the z-chunk of every size class by timing the stage-1 launch over
+ * the problems of that class for z-chunks 4 .. 64 (best of nRepeat).  Needs
+ * every problem's state; leaves the states unchanged. */
+int gx_batch_autotune(gx_batch *batch, int nRepeat);
+
+/* Total proper cells of the batch, and its HIP stream (a hipStream_t). */
+long long gx_batch_cells(gx_batch *batch);
+void *gx_batch_stream(gx_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

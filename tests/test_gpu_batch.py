@@ -1,0 +1,99 @@
+"""Batches: every problem of a batch must get, bit for bit, what a single
+chart context driven by the host integrator (compute_cycle) gives it -- the
+same time steps, conserved fields and shock flags -- whether the batch runs
+its cycles as a captured HIP graph (replayed several times, with host work
+and synchronisation between the replays) or launches them one by one, before
+and after the autotuner changes its z-chunks."""
+import numpy as np
+import pytest
+
+from genasis_amd import problems
+from genasis_amd.batch import Batch_RK2_C_ASC
+from genasis_amd.chart import Chart
+from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
+from genasis_amd.integrator import Integrator_C_PS, IntegratorParameters
+
+pytestmark = pytest.mark.gpu
+
+OUT = (("OUTFLOW", "OUTFLOW"),) * 3
+
+
+def _problems():
+    """Uneven sizes, partial tiles, every boundary condition."""
+    rng = np.random.default_rng(2024)
+    out = []
+    shapes = [(12, 10, 9), (40, 8, 8), (8, 33, 17), (20, 20, 20), (7, 5, 40), (33, 9, 6),
+              (16, 16, 16), (9, 17, 3)]
+    bcs = [OUT, (("REFLECTING", "OUTFLOW"), ("PERIODIC", "PERIODIC"), ("OUTFLOW", "REFLECTING")),
+           (("PERIODIC", "PERIODIC"),) * 3]
+    for i, n in enumerate(shapes):
+        chart = Chart(n, bc=bcs[i % len(bcs)])
+        prim, _ = problems.riemann_ensemble_member(chart, rng)
+        out.append((chart, prim))
+    kh = Chart((24, 16, 20), bc=(("PERIODIC", "PERIODIC"),) * 3)
+    out.append((kh, problems.kelvin_helmholtz(kh, seed=5)))
+    return out
+
+
+class _Single:
+    def __init__(self, chart, prim, params, finish):
+        self.st = Step_RK2_C_ASC(chart, params)
+        self.st.set_primitive(*prim)
+        self.it = Integrator_C_PS(self.st, IntegratorParameters(FinishTime=finish, nWrite=1))
+        self.it._administer_checkpoint()
+        self.last_dt = 0.0
+
+    def cycles(self, n):
+        for _ in range(n):
+            s = self.it.s
+            if not s.Time < self.it.p.FinishTime:
+                return
+            self.last_dt = self.it.compute_cycle()
+
+
+def _compare(batch, singles, tag):
+    t, dt, cyc = batch.times()
+    for k, ref in enumerate(singles):
+        s = ref.it.s
+        assert t[k] == s.Time and cyc[k] == s.iCycle, (tag, k, t[k], s.Time, cyc[k], s.iCycle)
+        if cyc[k] > 0:
+            assert dt[k] == ref.last_dt, (tag, k, dt[k], ref.last_dt)
+        ub, us = batch.conserved(k), ref.st.conserved()
+        assert np.array_equal(ub, us), (tag, k, np.argwhere(ub != us)[:3].tolist())
+        assert np.array_equal(batch.flags(k), ref.st.flags()), (tag, k)
+    return cyc
+
+
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_batch_matches_single_charts(precision):
+    probs = _problems()
+    params = FluidParameters(CourantFactor=0.8, Precision=precision)
+    batch = Batch_RK2_C_ASC([c for c, _ in probs], params)
+    for k, (chart, prim) in enumerate(probs):
+        batch.set_primitive(k, *prim)
+    cand = batch.time_step_candidates()
+    # about half of the problems reach FinishTime within the cycles below
+    finish = 0.5 * float(np.median(cand))
+    batch.set_integrator(FinishTime=finish)
+    singles = [_Single(c, p, params, finish) for c, p in probs]
+    for k, ref in enumerate(singles):
+        assert cand[k] == ref.st.time_step_candidate(), k
+    # the first two calls replay one captured graph
+    plan = [(3, True), (3, True), (1, True), (2, False), (4, True)]
+    mode = True
+    for i, (n, graph) in enumerate(plan):
+        if i == 4:
+            tuned = batch.autotune(n_repeat=1)
+            assert tuned and all(z >= 1 for z in tuned.values())
+        if graph != mode:
+            batch.set_tuning(batch.tuning(), use_graph=graph)
+            mode = graph
+        batch.step(n)
+        batch.sync()
+        for ref in singles:
+            ref.cycles(n)
+        cyc = _compare(batch, singles, f"call {i}")
+    # some problems stopped at FinishTime, others are still running
+    t, _, _ = batch.times()
+    assert (t == finish).any() and (t < finish).any(), t
+    assert cyc.max() == sum(n for n, _ in plan)
