@@ -16,6 +16,11 @@ restatement of the reference step) on the host cores on a bounded sample.
 ``fp32`` runs the same seeded problem at 512^3 for 200 steps with the state
 and sweeps in FP32 and in FP64, with the same time steps, and reports the
 speed ratio and the per-field L1 / Linf error of FP32 against FP64.
+``batch`` advances an ensemble of 256 seeded 3-D Riemann problems (cubic
+edges log-uniform in [32, 128], outflow, FP64) for 50 steps with the batch
+entry points (autotuned z-chunks, HIP graph of the cycles) and, for
+comparison, the same problems one after another through single-chart
+contexts driven by the host integrator.
 """
 from __future__ import annotations
 
@@ -129,6 +134,78 @@ def precision_check(n: int, steps: int, z_chunk: int):
     }
 
 
+def batch_check(n_problems: int, steps: int, seed: int, z_chunk: int, sequential: bool):
+    """Ensemble of Riemann problems: batch throughput and the sequential one."""
+    from genasis_amd import problems
+    from genasis_amd.batch import Batch_RK2_C_ASC
+    from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
+    from genasis_amd.integrator import Integrator_C_PS, IntegratorParameters
+
+    sizes = problems.riemann_ensemble_sizes(n_problems, seed)
+    made = [problems.riemann_ensemble_problem(k, n, seed) for k, n in enumerate(sizes)]
+    charts = [m[0] for m in made]
+    params = FluidParameters(CourantFactor=0.8)
+    batch = Batch_RK2_C_ASC(charts, params)
+
+    def upload():
+        for k, (_, prim, _) in enumerate(made):
+            batch.set_primitive(k, *prim)
+
+    upload()
+    t0 = time.perf_counter()
+    tuning = batch.autotune(n_repeat=2)
+    t_tune = time.perf_counter() - t0
+    per_call = 10
+    calls = max(1, steps // per_call)
+    batch.step(per_call)  # untimed: graph capture and first launches
+    batch.sync()
+    upload()              # the timed run starts from the initial data
+    batch.step(0)         # first CFL maxima, outside the timed region
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        batch.step(per_call)
+    batch.sync()
+    t1 = time.perf_counter()
+    n_steps = calls * per_call
+    cells = batch.cells
+    _, _, cyc = batch.times()
+    assert int(cyc.min()) == n_steps, cyc
+    batch.close()
+    out = {
+        "workload": f"{n_problems} 3-D Riemann problems (seed {seed}), cubic edges "
+                    f"{min(sizes)}..{max(sizes)} log-uniform in [32, 128], outflow, "
+                    f"PLM+HLLC RK2 FP64, CFL 0.8, {n_steps} steps",
+        "cells": cells,
+        "value": cells * n_steps / (t1 - t0),
+        "unit": "cell-updates/s",
+        "ms_per_step": (t1 - t0) / n_steps * 1e3,
+        "z_chunk_by_size_class": tuning,
+        "autotune_s": t_tune,
+        "sequential_value": None,
+        "speedup_vs_sequential": None,
+    }
+    if sequential:
+        elapsed = 0.0
+        for chart, prim, _ in made:
+            st = Step_RK2_C_ASC(chart, params)
+            st.set_tuning(z_chunk)
+            st.set_primitive(*prim)
+            it = Integrator_C_PS(st, IntegratorParameters(FinishTime=1.0, nWrite=1))
+            it._administer_checkpoint()
+            st.sync()
+            t0 = time.perf_counter()
+            for _ in range(n_steps):
+                it.compute_cycle()
+            st.sync()
+            elapsed += time.perf_counter() - t0
+            st.close()
+        out["sequential_value"] = cells * n_steps / elapsed
+        out["speedup_vs_sequential"] = out["value"] / out["sequential_value"]
+    print(f"# batch: {out['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration():
     """A plain 1 GiB device copy (torch), for scaling the PMC byte counters."""
     import torch
@@ -158,6 +235,10 @@ def main():
     ap.add_argument("--fp32-check", choices=("auto", "off"), default="auto")
     ap.add_argument("--fp32-n", type=int, default=512)
     ap.add_argument("--fp32-steps", type=int, default=200)
+    ap.add_argument("--batch-check", choices=("auto", "off"), default="auto")
+    ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--batch-sequential", choices=("on", "off"), default="on")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -214,6 +295,10 @@ def main():
     fp32 = None
     if args.fp32_check == "auto":
         fp32 = precision_check(args.fp32_n, args.fp32_steps, args.z_chunk)
+    batch = None
+    if args.batch_check == "auto":
+        batch = batch_check(args.batch_problems, args.batch_steps, 2024, args.z_chunk,
+                            args.batch_sequential == "on")
 
     cpu = None
     if args.cpu_baseline == "auto":
@@ -253,6 +338,7 @@ def main():
         "conserved_totals_change": [float(a - b) for a, b in zip(total1, total0)],
         "cpu_baseline": cpu,
         "fp32": fp32,
+        "batch": batch,
         "adjoint": None,
     }
     print(json.dumps(out))

@@ -37,21 +37,37 @@ def uniform_pm1(seed: int, index: np.ndarray) -> np.ndarray:
 
 
 def sod(chart: Chart, gamma: float = 1.4, axis: int = 0, x0: float = 0.5,
-        left=(1.0, 1.0, 0.0), right=(0.125, 0.1, 0.0)):
+        left=(1.0, 1.0, 0.0), right=(0.125, 0.1, 0.0), padded: bool = True):
     """Sod shock tube along ``axis``: (density, pressure, speed) left/right of
     the plane ``x_axis = x0``; the left state holds where
     ``x_axis - x0 <= 1e-10`` (RiemannProblem_Form.f90 SetFluidKernel)."""
-    xs = chart.padded_centers()
-    s = xs[axis] - x0
-    is_left = s <= 1.0e-10
+    g = chart.g
+    centers = np.asarray(chart.center[axis])
+    if padded:
+        shape = chart.m[::-1]
+    else:
+        shape = chart.n[::-1]
+        centers = centers[g:g + chart.n[axis]]
+    is_left = (centers - x0) <= 1.0e-10
+    # centres increase along the axis, so the left cells are a leading run
+    n_left = int(is_left.sum())
+    assert is_left[:n_left].all()
     rho_l, p_l, v_l = left
     rho_r, p_r, v_r = right
     e_l = p_l / (gamma - 1.0)
     e_r = p_r / (gamma - 1.0)
-    N = np.where(is_left, rho_l, rho_r)
-    E = np.where(is_left, e_l, e_r)
-    V = [np.zeros(chart.m[::-1]) for _ in range(3)]
-    V[axis] = np.where(is_left, v_l, v_r)
+
+    def fill(a, b):
+        out = np.full(shape, float(b))
+        sl = [slice(None)] * 3
+        sl[2 - axis] = slice(0, n_left)
+        out[tuple(sl)] = a
+        return out
+
+    N = fill(rho_l, rho_r)
+    E = fill(e_l, e_r)
+    V = [np.zeros(shape) for _ in range(3)]
+    V[axis] = fill(v_l, v_r)
     return N, V[0], V[1], V[2], E
 
 
@@ -93,7 +109,7 @@ def kelvin_helmholtz(chart: Chart, seed: int = 1234, gamma: float = 1.4,
 
 
 def riemann_ensemble_member(chart: Chart, rng: np.random.Generator,
-                            gamma: float = 1.4):
+                            gamma: float = 1.4, padded: bool = True):
     """A random 3-D Riemann problem: rho and p log-uniform in [0.1, 10],
     normal velocity uniform in [-1, 1], discontinuity plane through the
     centre with its normal along x, y or z."""
@@ -102,8 +118,8 @@ def riemann_ensemble_member(chart: Chart, rng: np.random.Generator,
     p = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
     v = rng.uniform(-1.0, 1.0, 2)
     x0 = 0.5 * (chart.lo[axis] + chart.hi[axis])
-    return sod(chart, gamma, axis=axis, x0=x0,
-               left=(rho[0], p[0], v[0]), right=(rho[1], p[1], v[1])), axis
+    return sod(chart, gamma, axis=axis, x0=x0, left=(rho[0], p[0], v[0]),
+               right=(rho[1], p[1], v[1]), padded=padded), axis
 
 
 def riemann_ensemble_sizes(n_problems: int, seed: int, n_min: int = 32,
@@ -121,7 +137,5 @@ def riemann_ensemble_problem(k: int, n: int, seed: int, padded: bool = False):
     (seed, k + 1).  Returns (chart, primitives, axis)."""
     chart = Chart((n, n, n), bc=(("OUTFLOW", "OUTFLOW"),) * 3)
     rng = np.random.default_rng([seed, k + 1])
-    prim, axis = riemann_ensemble_member(chart, rng)
-    if not padded:
-        prim = tuple(np.ascontiguousarray(chart.proper(f)) for f in prim)
+    prim, axis = riemann_ensemble_member(chart, rng, padded=padded)
     return chart, prim, axis
