@@ -40,6 +40,8 @@ EXPORTS = (
     "gx_batch_times", "gx_batch_get_conserved", "gx_batch_get_flags",
     "gx_batch_set_tuning", "gx_batch_get_tuning", "gx_batch_autotune",
     "gx_batch_cells", "gx_batch_stream",
+    "gx_adjoint_forward", "gx_adjoint_backward", "gx_adjoint_kinetic_energy",
+    "gx_adjoint_face_vjp", "gx_adjoint_slope_vjp",
 )
 
 
@@ -124,6 +126,11 @@ def load(path: str = LIB_PATH):
         "gx_batch_autotune": (i, [vp, i]),
         "gx_batch_cells": (ll, [vp]),
         "gx_batch_stream": (vp, [vp]),
+        "gx_adjoint_forward": (i, [vp, i, i, i]),
+        "gx_adjoint_backward": (i, [vp, pd, pd, pd]),
+        "gx_adjoint_kinetic_energy": (i, [vp, i, d, d, pd, pd]),
+        "gx_adjoint_face_vjp": (None, [pd, pd, i, i, d, d, i, pd, pd, pd, pd, pd]),
+        "gx_adjoint_slope_vjp": (None, [d, d, d, d, d, d, i, d, pd]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

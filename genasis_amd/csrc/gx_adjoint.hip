@@ -1,0 +1,833 @@
+// gx_adjoint.hip -- discrete adjoint of the FP64 step of one chart
+// (gx_adjoint_* of include/gx_fluid.h).
+//
+// The forward run records the time steps and the state every `every` steps.
+// The reverse pass walks the steps backwards; for each segment it recomputes
+// the states from the checkpoint, and for each step it recomputes the two
+// stage inputs (Y and W = U + K1/2) and Z = W + K2/2 with the forward stage
+// kernel, then applies the transposed step:
+//
+//   U' = fix(Z), Z = W + K2/2, K2 = dt R(Y), Y = U + K1, W = U + K1/2,
+//   K1 = dt R(U or the uploaded primitives), dt = dt(CFL maximum of U),
+//
+// where R(S) = -div F / Vol of the stage input S.  The transpose of R is one
+// face pass per dimension (the face flux's reverse sweep, writing the bars
+// of the two reconstructed face states), a gather pass (reconstruction and
+// limiter reverse), a fold of the ghost bars onto the cells they copy, and
+// the conserved -> primitive reverse.  Work arrays that take ghosts use the
+// padded layout (n + 4 per dimension).  Shock flags are recomputed from each
+// step's input and replayed as constants; dt and gamma bars are reduced per
+// block in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/gx_fluid.h"
+#include "gx_adjoint.hpp"
+#include "gx_context.hpp"
+#include "gx_host.hpp"
+#include "gx_kernels.hpp"
+#include "gx_tiles.hpp"
+
+namespace gx {
+
+constexpr int AB = 256;  // threads per block of the adjoint kernels
+
+struct PadGeom {
+  int n[3];
+  int m[3];
+  long long np;
+  long long s[3];  // padded strides
+};
+
+__host__ __device__ inline long long plin(const PadGeom &g, int x, int y, int z) {
+  return (long long)(x + 2) + g.m[0] * ((long long)(y + 2) + (long long)g.m[1] * (z + 2));
+}
+
+struct AdjointState {
+  int nsteps = 0, every = 0;
+  std::vector<double> dt, ts, invdt, ramp;
+  std::vector<double *> ckpt;
+  double *seg = nullptr;       // states U^n of one segment, every x 5 ncell
+  double *Z = nullptr, *trash = nullptr;
+  double *Vp = nullptr, *Vb = nullptr, *plus = nullptr, *minus = nullptr;  // 5 np
+  double *Ub = nullptr, *Yb = nullptr, *Wb = nullptr, *K1b = nullptr, *K2b = nullptr,
+         *P0b = nullptr;       // 5 ncell
+  double *partials = nullptr;  // 2 per face block, both stages, three dimensions
+  int part_stride = 0;         // blocks of the largest face pass
+  double *gb = nullptr;        // gamma bar
+  int *argmax = nullptr;
+  uint8_t *flags = nullptr;
+  double *band = nullptr;      // per y-row weight of the observable, n1
+  double *lossp = nullptr;     // per-block loss partials
+  PadGeom pg;
+  bool forward_done = false;
+};
+
+// ---- kernels ---------------------------------------------------------------
+
+__global__ __launch_bounds__(AB) void adj_fill_kernel(double *p, long long n, double v) {
+  for (long long i = blockIdx.x * (long long)AB + threadIdx.x; i < n;
+       i += (long long)gridDim.x * AB)
+    p[i] = v;
+}
+
+__global__ void adj_fill_int_kernel(int *p, int v) { *p = v; }
+
+// Primitives of the stage input on the padded layout, ghosts by the ghost
+// rule (cells with two or more coordinates outside are unused and 0).
+__global__ __launch_bounds__(AB) void adj_prims_kernel(StageArgs<double> a, PadGeom g,
+                                                       double *Vp) {
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < g.np;
+       t += (long long)gridDim.x * AB) {
+    const int x = (int)(t % g.m[0]) - 2;
+    const int y = (int)((t / g.m[0]) % g.m[1]) - 2;
+    const int z = (int)(t / ((long long)g.m[0] * g.m[1])) - 2;
+    const int out = (x < 0 || x >= g.n[0]) + (y < 0 || y >= g.n[1]) + (z < 0 || z >= g.n[2]);
+    double V[5] = {0, 0, 0, 0, 0};
+    if (out <= 1) load_prim(a, x, y, z, V);
+#pragma unroll
+    for (int f = 0; f < 5; f++) Vp[f * g.np + t] = V[f];
+  }
+}
+
+__device__ inline double block_sum(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = AB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double cell_width(const StageArgs<double> &a, int e, int c) {
+  return geo(a, e, G_WIDTH, c + 2);
+}
+
+// Face pass of dimension D: the reverse of the face flux for every face of
+// the sweep (faces 0 .. n_D between cells i-1 and i, transverse interior).
+// Fb = dt * A * (Kb_i / Vol_i - Kb_{i-1} / Vol_{i-1}); writes the bar of the
+// left state to plus[i-1] and of the right state to minus[i], and per block
+// the sums of F . Fb / dt (the dt bar) and of the gamma bar.
+template <int D>
+__global__ __launch_bounds__(AB) void adj_face_kernel(StageArgs<double> a, PadGeom g,
+                                                      const double *Vp, const double *Kb,
+                                                      double dt, double *plus, double *minus,
+                                                      double *partial) {
+  __shared__ double sh[AB];
+  const int e0 = g.n[0] + (D == 0), e1 = g.n[1] + (D == 1), e2 = g.n[2] + (D == 2);
+  const long long nf = (long long)e0 * e1 * e2;
+  const long long t = blockIdx.x * (long long)AB + threadIdx.x;
+  double dtb = 0.0, gb = 0.0;
+  if (t < nf) {
+    int c3[3];
+    c3[0] = (int)(t % e0);
+    c3[1] = (int)((t / e0) % e1);
+    c3[2] = (int)(t / ((long long)e0 * e1));
+    const int i = c3[D], n = g.n[D];
+    const long long sd = g.s[D];
+    const long long p0 = plin(g, c3[0], c3[1], c3[2]);
+    double Vm2[5], Vm1[5], V0[5], Vp1[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      const double *v = Vp + f * g.np + p0;
+      Vm2[f] = v[-2 * sd];
+      Vm1[f] = v[-sd];
+      V0[f] = v[0];
+      Vp1[f] = v[sd];
+    }
+    const double th = a.ph.theta;
+    double WL[5], WR[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      const double sl = slope(Vm2[f], Vm1[f], V0[f], geo(a, D, G_DXL, i + 1),
+                              geo(a, D, G_DXR, i + 1), geo(a, D, G_DXL2, i + 1),
+                              geo(a, D, G_DXR2, i + 1), geo(a, D, G_DEN, i + 1),
+                              geo(a, D, G_RDXL, i + 1), geo(a, D, G_RDXR, i + 1),
+                              geo(a, D, G_RDEN, i + 1), th, a.ph.use_limiter);
+      const double sr = slope(Vm1[f], V0[f], Vp1[f], geo(a, D, G_DXL, i + 2),
+                              geo(a, D, G_DXR, i + 2), geo(a, D, G_DXL2, i + 2),
+                              geo(a, D, G_DXR2, i + 2), geo(a, D, G_DEN, i + 2),
+                              geo(a, D, G_RDXL, i + 2), geo(a, D, G_RDXR, i + 2),
+                              geo(a, D, G_RDEN, i + 2), th, a.ph.use_limiter);
+      WL[f] = Vm1[f] + geo(a, D, G_WR, i + 1) * sl;
+      WR[f] = V0[f] - geo(a, D, G_WL, i + 2) * sr;
+    }
+    bool df = false;
+    if (i < n) df = (a.flags[lin(a.n, c3[0], c3[1], c3[2])] >> D) & 1;
+    double Gf[5] = {0, 0, 0, 0, 0};
+    const double wx = cell_width(a, 0, c3[0]), wy = cell_width(a, 1, c3[1]),
+                 wz = cell_width(a, 2, c3[2]);
+    if (i < n) {
+      const double vol = wx * wy * wz;
+      const double A = D == 0 ? wy * wz : (D == 1 ? wz * wx : wx * wy);
+      const unsigned ic = lin(a.n, c3[0], c3[1], c3[2]);
+#pragma unroll
+      for (int f = 0; f < 5; f++) Gf[f] += A * Kb[f * a.fs + ic] / vol;
+    }
+    if (i >= 1) {
+      int q[3] = {c3[0], c3[1], c3[2]};
+      q[D] = i - 1;
+      const double vx = cell_width(a, 0, q[0]), vy = cell_width(a, 1, q[1]),
+                   vz = cell_width(a, 2, q[2]);
+      const double vol = vx * vy * vz;
+      const double A = D == 0 ? vy * vz : (D == 1 ? vz * vx : vx * vy);
+      const unsigned ic = lin(a.n, q[0], q[1], q[2]);
+#pragma unroll
+      for (int f = 0; f < 5; f++) Gf[f] -= A * Kb[f * a.fs + ic] / vol;
+    }
+    double Fb[5], F[5], WLb[5] = {0, 0, 0, 0, 0}, WRb[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int f = 0; f < 5; f++) Fb[f] = dt * Gf[f];
+    adj::face_flux_vjp(WL, WR, D, df, a.ph.gamma, a.ph.mass, a.ph.hllc != 0, Fb, WLb, WRb,
+                       gb, F);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      dtb += F[f] * Gf[f];
+      plus[f * g.np + p0 - sd] = WLb[f];
+      minus[f * g.np + p0] = WRb[f];
+    }
+  }
+  const double s1 = block_sum(dtb, sh);
+  const double s2 = block_sum(gb, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s1;
+    partial[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// Gather pass of dimension D over cells -2 .. n+1 along D (transverse
+// interior): Vb += bars of the cell's own face states, and the limiter
+// reverse of the slopes of cells c-1, c, c+1 (slope bar s_j = wr_j plus_j -
+// wl_j minus_j) with respect to cell c.
+template <int D>
+__global__ __launch_bounds__(AB) void adj_gather_kernel(StageArgs<double> a, PadGeom g,
+                                                        const double *Vp, const double *plus,
+                                                        const double *minus, double *Vb) {
+  const int e0 = D == 0 ? g.n[0] + 4 : g.n[0];
+  const int e1 = D == 1 ? g.n[1] + 4 : g.n[1];
+  const int e2 = D == 2 ? g.n[2] + 4 : g.n[2];
+  const long long nc = (long long)e0 * e1 * e2;
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < nc;
+       t += (long long)gridDim.x * AB) {
+    int c3[3];
+    c3[0] = (int)(t % e0);
+    c3[1] = (int)((t / e0) % e1);
+    c3[2] = (int)(t / ((long long)e0 * e1));
+    c3[D] -= 2;
+    const int c = c3[D], n = g.n[D];
+    const long long sd = g.s[D];
+    const long long p = plin(g, c3[0], c3[1], c3[2]);
+    double acc[5] = {0, 0, 0, 0, 0};
+    const bool hasp = c >= -1 && c <= n - 1, hasm = c >= 0 && c <= n;
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      if (hasp) acc[f] += plus[f * g.np + p];
+      if (hasm) acc[f] += minus[f * g.np + p];
+    }
+    for (int o = -1; o <= 1; o++) {
+      const int j = c + o;
+      if (j < -1 || j > n) continue;
+      const long long pj = p + o * sd;
+      const int q = j + 2;  // padded geometry index
+      const double wr = geo(a, D, G_WR, q), wl = geo(a, D, G_WL, q);
+      const double dxl = geo(a, D, G_DXL, q), dxr = geo(a, D, G_DXR, q);
+      const double dxl2 = geo(a, D, G_DXL2, q), dxr2 = geo(a, D, G_DXR2, q);
+      const double den = geo(a, D, G_DEN, q);
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        double sb = 0.0;
+        if (j <= n - 1) sb += wr * plus[f * g.np + pj];
+        if (j >= 0) sb -= wl * minus[f * g.np + pj];
+        if (sb == 0.0) continue;
+        const double *v = Vp + f * g.np + pj;
+        double vmb = 0.0, v0b = 0.0, vpb = 0.0;
+        adj::slope_vjp(v[-sd], v[0], v[sd], dxl, dxr, dxl2, dxr2, den, a.ph.theta,
+                       a.ph.use_limiter, sb, vmb, v0b, vpb);
+        acc[f] += o == 1 ? vmb : (o == 0 ? v0b : vpb);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 5; f++) Vb[f * g.np + p] += acc[f];
+  }
+}
+
+// Interior cells: the primitive bar of the cell plus the bars of the ghost
+// values copied from it (normal velocity negated across a REFLECTING face),
+// then the conserved -> primitive reverse (or the primitive bar itself for an
+// uploaded-primitive input), added to out.
+__global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGeom g,
+                                                      const double *Vb, double *out) {
+  const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
+       t += (long long)gridDim.x * AB) {
+    const int c3[3] = {(int)(t % g.n[0]), (int)((t / g.n[0]) % g.n[1]),
+                       (int)(t / ((long long)g.n[0] * g.n[1]))};
+    const long long p = plin(g, c3[0], c3[1], c3[2]);
+    double V[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) V[f] = Vb[f * g.np + p];
+    for (int d = 0; d < 3; d++) {
+      const int c = c3[d], n = g.n[d];
+      const int bi = a.bc[2 * d], bo = a.bc[2 * d + 1];
+      int gh[2], fl[2], ng = 0;
+      if (bi == BC_PERIODIC) {
+        if (c >= n - 2) { gh[ng] = c - n; fl[ng++] = 0; }
+      } else if (c <= 1) {
+        gh[ng] = -1 - c;
+        fl[ng++] = bi == BC_REFLECTING;
+      }
+      if (bo == BC_PERIODIC) {
+        if (c <= 1) { gh[ng] = c + n; fl[ng++] = 0; }
+      } else if (c >= n - 2) {
+        gh[ng] = 2 * n - 1 - c;
+        fl[ng++] = bo == BC_REFLECTING;
+      }
+      for (int k = 0; k < ng; k++) {
+        const long long pg = p + (long long)(gh[k] - c) * g.s[d];
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          const double v = Vb[f * g.np + pg];
+          V[f] += (fl[k] && f == 1 + d) ? -v : v;
+        }
+      }
+    }
+    if (a.src_is_prim) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) out[f * ncell + t] += V[f];
+    } else {
+      double U[5], Ub[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];
+      adj::prim_vjp(U, a.ph.mass, a.ph.nmin, V, Ub);
+#pragma unroll
+      for (int f = 0; f < 5; f++) out[f * ncell + t] += Ub[f];
+    }
+  }
+}
+
+// Zb = fix^T Ub' ; Wb = Zb, K2b = Zb/2.
+__global__ __launch_bounds__(AB) void adj_fix_kernel(long long ncell, double mass, double nmin,
+                                                     const double *Z, const double *Ubn,
+                                                     double *Wb, double *K2b) {
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
+       t += (long long)gridDim.x * AB) {
+    double z[5], ub[5], zb[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      z[f] = Z[f * ncell + t];
+      ub[f] = Ubn[f * ncell + t];
+    }
+    adj::fix_vjp(z, mass, nmin, ub, zb);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      Wb[f * ncell + t] = zb[f];
+      K2b[f * ncell + t] = 0.5 * zb[f];
+    }
+  }
+}
+
+// Y = U + K1, W = U + K1/2: Ub = Yb + Wb, K1b = Yb + Wb/2.
+__global__ __launch_bounds__(AB) void adj_rk_kernel(long long n5, const double *Yb,
+                                                    const double *Wb, double *Ub, double *K1b) {
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < n5;
+       t += (long long)gridDim.x * AB) {
+    Ub[t] = Yb[t] + Wb[t];
+    K1b[t] = Yb[t] + 0.5 * Wb[t];
+  }
+}
+
+// First cell (x fastest) whose CFL term has the bits of the maximum.
+__global__ __launch_bounds__(AB) void adj_argmax_kernel(StageArgs<double> a,
+                                                        unsigned long long bits, int *idx) {
+  const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
+       t += (long long)gridDim.x * AB) {
+    const int x = (int)(t % a.n[0]), y = (int)((t / a.n[0]) % a.n[1]),
+              z = (int)(t / ((long long)a.n[0] * a.n[1]));
+    double V[5];
+    load_prim(a, x, y, z, V);
+    const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, x + 2),
+                               geo(a, 1, G_WIDTH, y + 2), geo(a, 2, G_WIDTH, z + 2));
+    if ((unsigned long long)__double_as_longlong(tc) == bits) atomicMin(idx, (int)t);
+  }
+}
+
+// The dt bar of one step: sum of the per-block partials (fixed order), the
+// CFL maximum's reverse at its cell (dt = ts = ramp * C / invdt), added to
+// out through the conserved -> primitive reverse; gamma bars into *gbar.
+__global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const double *partial,
+                                                    int nparts, double ts, double invdt,
+                                                    const int *idx, double *out,
+                                                    double *gbar) {
+  __shared__ double sh[AB];
+  double sd = 0.0, sg = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += AB) {
+    sd += partial[2 * i];
+    sg += partial[2 * i + 1];
+  }
+  const double dtb = block_sum(sd, sh);
+  const double gsum = block_sum(sg, sh);
+  if (threadIdx.x != 0) return;
+  double gb = gsum;
+  const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
+  const int t = *idx;
+  if (invdt > DBL_MIN && ts > 0.0 && t >= 0 && t < ncell) {
+    const double invb = dtb * (-ts / invdt);
+    const int x = t % a.n[0], y = (t / a.n[0]) % a.n[1], z = t / (a.n[0] * a.n[1]);
+    double V[5];
+    load_prim(a, x, y, z, V);
+    const double w[3] = {geo(a, 0, G_WIDTH, x + 2), geo(a, 1, G_WIDTH, y + 2),
+                         geo(a, 2, G_WIDTH, z + 2)};
+    double Vb[5] = {0, 0, 0, 0, 0};
+    adj::cfl_term_vjp(V, a.ph.gamma, a.ph.mass, w, invb, Vb, gb);
+    if (a.src_is_prim) {
+      for (int f = 0; f < 5; f++) out[f * ncell + t] += Vb[f];
+    } else {
+      double U[5], Ub[5] = {0, 0, 0, 0, 0};
+      for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];
+      adj::prim_vjp(U, a.ph.mass, a.ph.nmin, Vb, Ub);
+      for (int f = 0; f < 5; f++) out[f * ncell + t] += Ub[f];
+    }
+  }
+  *gbar += gb;
+}
+
+// ComputeFromInitial's reverse: D = N, S_k = (M N) V_k, G = E + S.V/2 with
+// N >= 0 (P holds the floored primitives); Pb += (dU/dP)^T Ub.
+__global__ __launch_bounds__(AB) void adj_from_prim_kernel(long long ncell, double mass,
+                                                           const double *P, const double *Ub,
+                                                           double *Pb) {
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
+       t += (long long)gridDim.x * AB) {
+    const double N = P[t], V[3] = {P[ncell + t], P[2 * ncell + t], P[3 * ncell + t]};
+    double ub[5];
+    for (int f = 0; f < 5; f++) ub[f] = Ub[f * ncell + t];
+    double S[3], Sb[3], Vb[3];
+    for (int k = 0; k < 3; k++) S[k] = mass * N * V[k];
+    // G = E + 1/2 ((S1 V1 + S2 V2) + S3 V3)
+    const double Eb = ub[4];
+    for (int k = 0; k < 3; k++) {
+      Sb[k] = ub[1 + k] + 0.5 * ub[4] * V[k];
+      Vb[k] = 0.5 * ub[4] * S[k];
+    }
+    double Nb = ub[0];
+    for (int k = 0; k < 3; k++) {
+      Nb += Sb[k] * mass * V[k];
+      Vb[k] += Sb[k] * mass * N;
+    }
+    Pb[t] += Nb;
+    for (int k = 0; k < 3; k++) Pb[(1 + k) * ncell + t] += Vb[k];
+    Pb[4 * ncell + t] += Eb;
+  }
+}
+
+// The observable sum_c band(y_c) * 1/2 N V_k^2 dV on the state U (per-block
+// partials) and its bar with respect to U.
+__global__ __launch_bounds__(AB) void adj_loss_kernel(StageArgs<double> a, int comp,
+                                                      const double *band, double *loss_part,
+                                                      double *Ub) {
+  __shared__ double sh[AB];
+  const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
+  double acc = 0.0;
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
+       t += (long long)gridDim.x * AB) {
+    const int x = (int)(t % a.n[0]), y = (int)((t / a.n[0]) % a.n[1]),
+              z = (int)(t / ((long long)a.n[0] * a.n[1]));
+    double U[5], V[5];
+    for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];
+    prim_from_cons(U, a.ph.mass, a.ph.nmin, V);
+    const double vol = geo(a, 0, G_WIDTH, x + 2) * geo(a, 1, G_WIDTH, y + 2) *
+                       geo(a, 2, G_WIDTH, z + 2);
+    const double w = band[y];
+    const double v = V[1 + comp];
+    acc += w * (0.5 * V[0] * v * v) * vol;
+    double Vb[5] = {0, 0, 0, 0, 0}, ub[5] = {0, 0, 0, 0, 0};
+    Vb[0] = w * (0.5 * v * v) * vol;
+    Vb[1 + comp] = w * (V[0] * v) * vol;
+    adj::prim_vjp(U, a.ph.mass, a.ph.nmin, Vb, ub);
+    for (int f = 0; f < 5; f++) Ub[f * ncell + t] = ub[f];
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = s;
+}
+
+}  // namespace gx
+
+using namespace gx;
+
+namespace gx {
+
+void release_adjoint(gx_ctx *c) {
+  AdjointState *A = c->adj;
+  if (!A) return;
+  for (double *p : A->ckpt) (void)hipFree(p);
+  for (void *p : {(void *)A->seg, (void *)A->Z, (void *)A->trash, (void *)A->Vp,
+                  (void *)A->Vb, (void *)A->plus, (void *)A->minus, (void *)A->Ub,
+                  (void *)A->Yb, (void *)A->Wb, (void *)A->K1b, (void *)A->K2b,
+                  (void *)A->P0b, (void *)A->partials, (void *)A->gb, (void *)A->argmax,
+                  (void *)A->flags, (void *)A->band, (void *)A->lossp})
+    (void)hipFree(p);
+  delete A;
+  c->adj = nullptr;
+}
+
+}  // namespace gx
+
+static int blocks_for(long long n) {
+  long long b = (n + AB - 1) / AB;
+  if (b > 65536) b = 65536;
+  return (int)std::max(1ll, b);
+}
+
+static int fill(gx_ctx *c, double *p, long long n, double v) {
+  adj_fill_kernel<<<blocks_for(n), AB, 0, c->stream>>>(p, n, v);
+  HIP_TRY(hipGetLastError());
+  return GX_OK;
+}
+
+// Forward pieces, as gx_step_rk2 launches them.
+static void stage_geometry(const gx_ctx *c, StageArgs<double> &a) {
+  a.nbx = (c->n[0] + TX - 1) / TX;
+  a.nby = (c->n[1] + TY - 1) / TY;
+  a.nblk = a.nbx * a.nby * ((c->n[2] + c->tz - 1) / c->tz);
+}
+
+static int launch_flags(gx_ctx *c, const double *U, uint8_t *flags) {
+  StageArgs<double> a = base_args<double>(c);
+  a.src = U;
+  a.tz = FLAG_TZ;
+  a.nbx = (c->n[0] + TX - 1) / TX;
+  a.nby = (c->n[1] + TY - 1) / TY;
+  a.nblk = a.nbx * a.nby * ((c->n[2] + FLAG_TZ - 1) / FLAG_TZ);
+  shock_flags_kernel<double><<<a.nblk, NT, 0, c->stream>>>(a, flags);
+  HIP_TRY(hipGetLastError());
+  return GX_OK;
+}
+
+// One stage-1 launch: out1 = aux + K(src), out2 = aux + K(src)/2.
+static int launch_stage1(gx_ctx *c, const double *src, bool prim, const double *aux,
+                         double *out1, double *out2, double dt, const uint8_t *flags) {
+  StageArgs<double> a = base_args<double>(c);
+  a.dt = dt;
+  a.src_is_prim = prim ? 1 : 0;
+  a.src = src;
+  a.aux = aux;
+  a.out1 = out1;
+  a.out2 = out2;
+  a.flags = flags;
+  stage_geometry(c, a);
+  stage_kernel<double, 1><<<a.nblk, NT, 0, c->stream>>>(a);
+  HIP_TRY(hipGetLastError());
+  return GX_OK;
+}
+
+static int launch_stage2(gx_ctx *c, const double *Y, const double *W, double *out,
+                         double dt, const uint8_t *flags) {
+  StageArgs<double> a = base_args<double>(c);
+  a.dt = dt;
+  a.src = Y;
+  a.aux = W;
+  a.out1 = out;
+  a.flags = flags;
+  stage_geometry(c, a);
+  HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
+  stage_kernel<double, 2><<<a.nblk, NT, 0, c->stream>>>(a);
+  HIP_TRY(hipGetLastError());
+  return GX_OK;
+}
+
+// Transposed residual: out += (dt R(src))^T Kb; dt and gamma bar partials at
+// part (3 face passes).
+static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, double dt,
+                     const uint8_t *flags, double *out, double *part) {
+  AdjointState *A = c->adj;
+  const PadGeom &g = A->pg;
+  StageArgs<double> a = base_args<double>(c);
+  a.src = src;
+  a.src_is_prim = prim ? 1 : 0;
+  a.flags = flags;
+  adj_prims_kernel<<<blocks_for(g.np), AB, 0, c->stream>>>(a, g, A->Vp);
+  HIP_TRY(hipGetLastError());
+  int rc = fill(c, A->Vb, 5 * g.np, 0.0);
+  if (rc) return rc;
+  for (int d = 0; d < 3; d++) {
+    long long nf = 1, ng = 1;
+    for (int e = 0; e < 3; e++) {
+      nf *= g.n[e] + (e == d);
+      ng *= e == d ? g.n[e] + 4 : g.n[e];
+    }
+    const int fb = (int)((nf + AB - 1) / AB);
+    double *pp = part + 2 * (size_t)d * A->part_stride;
+    if (d == 0) {
+      adj_face_kernel<0><<<fb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->plus, A->minus, pp);
+      HIP_TRY(hipGetLastError());
+      adj_gather_kernel<0><<<blocks_for(ng), AB, 0, c->stream>>>(a, g, A->Vp, A->plus,
+                                                                  A->minus, A->Vb);
+    } else if (d == 1) {
+      adj_face_kernel<1><<<fb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->plus, A->minus, pp);
+      HIP_TRY(hipGetLastError());
+      adj_gather_kernel<1><<<blocks_for(ng), AB, 0, c->stream>>>(a, g, A->Vp, A->plus,
+                                                                  A->minus, A->Vb);
+    } else {
+      adj_face_kernel<2><<<fb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->plus, A->minus, pp);
+      HIP_TRY(hipGetLastError());
+      adj_gather_kernel<2><<<blocks_for(ng), AB, 0, c->stream>>>(a, g, A->Vp, A->plus,
+                                                                  A->minus, A->Vb);
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, g, A->Vb, out);
+  HIP_TRY(hipGetLastError());
+  return GX_OK;
+}
+
+static int ensure_state(gx_ctx *c, int nsteps, int every) {
+  if (c->fp32) return fail(GX_E_ARG, "gx_adjoint: only Precision = GX_FP64");
+  release_adjoint(c);
+  AdjointState *A = new (std::nothrow) AdjointState();
+  if (!A) return fail(GX_E_NOMEM, "gx_adjoint: host allocation failed");
+  c->adj = A;
+  A->nsteps = nsteps;
+  A->every = every;
+  PadGeom &g = A->pg;
+  g.np = 1;
+  for (int d = 0; d < 3; d++) {
+    g.n[d] = c->n[d];
+    g.m[d] = c->n[d] + 4;
+    g.np *= g.m[d];
+  }
+  g.s[0] = 1;
+  g.s[1] = g.m[0];
+  g.s[2] = (long long)g.m[0] * g.m[1];
+  const size_t s5 = 5 * sizeof(double) * (size_t)c->ncell;
+  const size_t p5 = 5 * sizeof(double) * (size_t)g.np;
+  const int nck = (nsteps + every - 1) / every + 1;
+  A->ckpt.assign(nck, nullptr);
+  for (int i = 0; i < nck; i++) HIP_TRY(hipMalloc(&A->ckpt[i], s5));
+  HIP_TRY(hipMalloc(&A->seg, s5 * every));
+  for (double **p : {&A->Z, &A->trash, &A->Ub, &A->Yb, &A->Wb, &A->K1b, &A->K2b, &A->P0b})
+    HIP_TRY(hipMalloc(p, s5));
+  for (double **p : {&A->Vp, &A->Vb, &A->plus, &A->minus}) HIP_TRY(hipMalloc(p, p5));
+  long long maxf = 0;
+  for (int d = 0; d < 3; d++) {
+    long long nf = 1;
+    for (int e = 0; e < 3; e++) nf *= c->n[e] + (e == d);
+    maxf = std::max(maxf, nf);
+  }
+  A->part_stride = (int)((maxf + AB - 1) / AB);
+  HIP_TRY(hipMalloc(&A->partials, sizeof(double) * 2 * 6 * (size_t)A->part_stride));
+  HIP_TRY(hipMemset(A->partials, 0, sizeof(double) * 2 * 6 * (size_t)A->part_stride));
+  HIP_TRY(hipMalloc(&A->gb, sizeof(double)));
+  HIP_TRY(hipMalloc(&A->argmax, sizeof(int)));
+  HIP_TRY(hipMalloc(&A->flags, (size_t)c->ncell));
+  HIP_TRY(hipMalloc(&A->band, sizeof(double) * c->n[1]));
+  HIP_TRY(hipMalloc(&A->lossp, sizeof(double) * blocks_for(c->ncell)));
+  return GX_OK;
+}
+
+extern "C" {
+
+int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycles) {
+  if (!c || nSteps < 1 || checkpointEvery < 1 || nRampCycles < 1)
+    return fail(GX_E_ARG, "gx_adjoint_forward: bad argument");
+  if (!c->uploaded || !c->prims_valid)
+    return fail(GX_E_STATE, "gx_adjoint_forward: needs freshly uploaded primitives");
+  int rc = ensure_state(c, nSteps, checkpointEvery);
+  if (rc) return rc;
+  AdjointState *A = c->adj;
+  const size_t s5 = 5 * sizeof(double) * (size_t)c->ncell;
+  double time = 0.0;
+  for (int n = 0; n < nSteps; n++) {
+    if (n % checkpointEvery == 0)
+      HIP_TRY(hipMemcpyAsync(A->ckpt[n / checkpointEvery], c->U, s5, hipMemcpyDeviceToDevice,
+                             c->stream));
+    double cand;
+    rc = gx_compute_time_step(c, &cand);
+    if (rc) return rc;
+    unsigned long long bits;
+    HIP_TRY(hipMemcpy(&bits, c->invdt, sizeof(bits), hipMemcpyDeviceToHost));
+    double invdt;
+    memcpy(&invdt, &bits, sizeof(invdt));
+    // Integrator_Template % ComputeTimeStep and ComputeNewTime
+    double ts = fmin(DBL_MAX, cand);
+    const double ramp = fmin((double)(n + 1) / (double)nRampCycles, 1.0);
+    if (ramp < 1.0) ts = ramp * ts;
+    const double dt = (time + ts) - time;
+    A->dt.push_back(dt);
+    A->ts.push_back(ts);
+    A->invdt.push_back(invdt);
+    A->ramp.push_back(ramp);
+    rc = gx_step_rk2(c, time, dt);
+    if (rc) return rc;
+    time = time + dt;
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  A->forward_done = true;
+  return GX_OK;
+}
+
+int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *gammaBar) {
+  if (!c || !Ubar || !P0bar || !gammaBar)
+    return fail(GX_E_ARG, "gx_adjoint_backward: null argument");
+  AdjointState *A = c->adj;
+  if (!A || !A->forward_done)
+    return fail(GX_E_STATE, "gx_adjoint_backward: no forward run recorded");
+  const long long n5 = 5 * c->ncell;
+  const size_t s5 = sizeof(double) * (size_t)n5;
+  HIP_TRY(hipMemcpyAsync(A->Ub, Ubar, s5, hipMemcpyHostToDevice, c->stream));
+  int rc = fill(c, A->P0b, n5, 0.0);
+  if (rc) return rc;
+  rc = fill(c, A->gb, 1, 0.0);
+  if (rc) return rc;
+  const double *P0 = (const double *)c->P0;
+  const int every = A->every, N = A->nsteps;
+  for (int seg0 = ((N - 1) / every) * every; seg0 >= 0; seg0 -= every) {
+    const int len = std::min(every, N - seg0);
+    // states U^n, n = seg0 .. seg0+len-1
+    HIP_TRY(hipMemcpyAsync(A->seg, A->ckpt[seg0 / every], s5, hipMemcpyDeviceToDevice,
+                           c->stream));
+    for (int j = 0; j + 1 < len; j++) {
+      const int n = seg0 + j;
+      double *Un = A->seg + (size_t)j * n5;
+      double *Un1 = A->seg + (size_t)(j + 1) * n5;
+      if (n == 0) {
+        HIP_TRY(hipMemsetAsync(A->flags, 0, (size_t)c->ncell, c->stream));
+      } else {
+        rc = launch_flags(c, Un, A->flags);
+        if (rc) return rc;
+      }
+      rc = launch_stage1(c, n == 0 ? P0 : Un, n == 0, Un, (double *)c->Y, (double *)c->W,
+                         A->dt[n], A->flags);
+      if (rc) return rc;
+      rc = launch_stage2(c, (const double *)c->Y, (const double *)c->W, Un1, A->dt[n],
+                         A->flags);
+      if (rc) return rc;
+    }
+    for (int j = len - 1; j >= 0; j--) {
+      const int n = seg0 + j;
+      const double *Un = A->seg + (size_t)j * n5;
+      const bool first = n == 0;
+      if (first) {
+        HIP_TRY(hipMemsetAsync(A->flags, 0, (size_t)c->ncell, c->stream));
+      } else {
+        rc = launch_flags(c, Un, A->flags);
+        if (rc) return rc;
+      }
+      const double dt = A->dt[n];
+      rc = fill(c, A->partials, 12 * (long long)A->part_stride, 0.0);
+      if (rc) return rc;
+      // Y, W, and Z = W + K2/2
+      rc = launch_stage1(c, first ? P0 : Un, first, Un, (double *)c->Y, (double *)c->W, dt,
+                         A->flags);
+      if (rc) return rc;
+      rc = launch_stage1(c, (const double *)c->Y, false, (const double *)c->W, A->trash, A->Z,
+                         dt, A->flags);
+      if (rc) return rc;
+      adj_fix_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
+          c->ncell, c->params.BaryonMassReference, c->params.BaryonDensityMin, A->Z, A->Ub,
+          A->Wb, A->K2b);
+      HIP_TRY(hipGetLastError());
+      rc = fill(c, A->Yb, n5, 0.0);
+      if (rc) return rc;
+      rc = stage_vjp(c, (const double *)c->Y, false, A->K2b, dt, A->flags, A->Yb,
+                     A->partials + 6 * (size_t)A->part_stride);
+      if (rc) return rc;
+      adj_rk_kernel<<<blocks_for(n5), AB, 0, c->stream>>>(n5, A->Yb, A->Wb, A->Ub, A->K1b);
+      HIP_TRY(hipGetLastError());
+      double *out = first ? A->P0b : A->Ub;
+      rc = stage_vjp(c, first ? P0 : Un, first, A->K1b, dt, A->flags, out, A->partials);
+      if (rc) return rc;
+      // dt = ts(invdt(U^n))
+      StageArgs<double> a = base_args<double>(c);
+      a.src = first ? P0 : Un;
+      a.src_is_prim = first ? 1 : 0;
+      adj_fill_int_kernel<<<1, 1, 0, c->stream>>>(A->argmax, 0x7fffffff);
+      HIP_TRY(hipGetLastError());
+      unsigned long long bits;
+      memcpy(&bits, &A->invdt[n], sizeof(bits));
+      adj_argmax_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, bits, A->argmax);
+      HIP_TRY(hipGetLastError());
+      adj_dt_kernel<<<1, AB, 0, c->stream>>>(a, A->partials, 6 * A->part_stride, A->ts[n],
+                                             A->invdt[n], A->argmax, out, A->gb);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  // U^0 = ComputeFromInitial(P0)
+  adj_from_prim_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
+      c->ncell, c->params.BaryonMassReference, P0, A->Ub, A->P0b);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(P0bar, A->P0b, s5, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(gammaBar, A->gb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->invdt_valid = false;  // the recomputation reused the CFL accumulator
+  return GX_OK;
+}
+
+int gx_adjoint_kinetic_energy(gx_ctx *c, int component, double lo, double hi,
+                              double *loss, double *Ubar) {
+  if (!c || !loss || !Ubar || component < 0 || component > 2)
+    return fail(GX_E_ARG, "gx_adjoint_kinetic_energy: bad argument");
+  if (!c->adj) return fail(GX_E_STATE, "gx_adjoint_kinetic_energy: no forward run recorded");
+  AdjointState *A = c->adj;
+  std::vector<double> band(c->n[1]);
+  {
+    // cell centres of dimension 1 (Chart_Template.f90:827-841)
+    const int n = c->n[1];
+    const double y0 = c->grid.MinCoordinate[1], w = (c->grid.MaxCoordinate[1] - y0) / n;
+    double e = y0;
+    for (int j = 0; j < n; j++) {
+      const double e1 = e + w;
+      const double yc = 0.5 * (e + e1);
+      band[j] = (yc > lo && yc < hi) ? 1.0 : 0.0;
+      e = e1;
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(A->band, band.data(), sizeof(double) * band.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  StageArgs<double> a = base_args<double>(c);
+  a.src = (const double *)c->U;
+  const int nb = blocks_for(c->ncell);
+  adj_loss_kernel<<<nb, AB, 0, c->stream>>>(a, component, A->band, A->lossp, A->Ub);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> part(nb);
+  HIP_TRY(hipMemcpyAsync(part.data(), A->lossp, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipMemcpyAsync(Ubar, A->Ub, 5 * sizeof(double) * (size_t)c->ncell,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double s = 0.0;
+  for (double v : part) s += v;
+  *loss = s;
+  return GX_OK;
+}
+
+// Host evaluations of the device code's reverse sweeps, for checking.
+void gx_adjoint_face_vjp(const double WL[5], const double WR[5], int d, int df,
+                         double gamma, double mass, int hllc, const double Fbar[5],
+                         double WLbar[5], double WRbar[5], double *gammaBar, double F[5]) {
+  for (int f = 0; f < 5; f++) WLbar[f] = WRbar[f] = 0.0;
+  double gb = 0.0;
+  adj::face_flux_vjp(WL, WR, d, df != 0, gamma, mass, hllc != 0, Fbar, WLbar, WRbar, gb, F);
+  *gammaBar = gb;
+}
+
+void gx_adjoint_slope_vjp(double vm, double v0, double vp, double dxl, double dxr,
+                          double theta, int useLimiter, double sbar, double out[3]) {
+  double a = 0.0, b = 0.0, cc = 0.0;
+  adj::slope_vjp(vm, v0, vp, dxl, dxr, dxl * dxl, dxr * dxr, dxl * dxr * (dxl + dxr), theta,
+                 useLimiter, sbar, a, b, cc);
+  out[0] = a;
+  out[1] = b;
+  out[2] = cc;
+}
+
+}  // extern "C"

@@ -18,6 +18,7 @@
 #include "gx_host.hpp"
 #include "gx_kernels.hpp"
 #include "gx_tiles.hpp"
+#include "gx_context.hpp"
 
 namespace gx {
 
@@ -74,57 +75,6 @@ using namespace gx;
 
 thread_local std::string gx::g_last_error;
 
-struct gx_ctx {
-  gx_grid grid;
-  gx_params params;
-  int n[3];
-  long long ncell;
-  hipStream_t stream = nullptr;
-  int tz = 16;
-  bool fp32 = false;          // state and sweeps in float (Precision = GX_FP32)
-  size_t esz = sizeof(double);  // bytes per state value
-  // device buffers, of the state's real type
-  void *geo_buf = nullptr;    // 3 dimensions x N_GEO arrays x gs
-  int gs = 0;                 // geometry stride: max(n) + 4
-  // five-field states, field f at base + f * ncell
-  void *U = nullptr, *Y = nullptr, *W = nullptr, *P0 = nullptr;
-  uint8_t *flags = nullptr;
-  unsigned long long *invdt = nullptr;
-  double *partial = nullptr;
-  int n_partial = 0;
-  double *scratch = nullptr;  // download staging: GX_N_FIELDS x ncell
-  bool uploaded = false;
-  bool prims_valid = false;   // next stage 1 reads P0
-  bool invdt_valid = false;   // invdt holds the max for the current state
-  // per-kernel timing (gx_set_profiling): stage 1, stage 2, shock flags, (unused)
-  bool profiling = false;
-  hipEvent_t ev[5] = {};
-  double kernel_ms[4] = {0, 0, 0, 0};
-  long long kernel_n[4] = {0, 0, 0, 0};
-  ChartGeometry geom;
-};
-
-template <typename R>
-static StageArgs<R> base_args(const gx_ctx *c) {
-  StageArgs<R> a;
-  memset(&a, 0, sizeof(a));
-  for (int d = 0; d < 3; d++) a.n[d] = c->n[d];
-  for (int i = 0; i < 6; i++) a.bc[i] = c->grid.BoundaryCondition[i];
-  a.tz = c->tz;
-  a.ph.gamma = c->params.AdiabaticIndex;
-  a.ph.theta = c->params.LimiterParameter;
-  a.ph.mass = c->params.BaryonMassReference;
-  a.ph.nmin = c->params.BaryonDensityMin;
-  a.ph.st = c->params.ShockThreshold;
-  a.ph.use_limiter = c->params.UseLimiter;
-  a.ph.hllc = c->params.RiemannSolverType == GX_HLLC;
-  a.gs = c->gs;
-  a.fs = c->ncell;
-  a.geo = (const R *)c->geo_buf;
-  a.flags = c->flags;
-  a.invdt_bits = c->invdt;
-  return a;
-}
 
 extern "C" {
 
@@ -163,6 +113,7 @@ int gx_self_test_division(long long n, unsigned long long seed,
 int gx_destroy(gx_ctx *c) {
   if (!c) return GX_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  gx::release_adjoint(c);
   for (int e = 0; e < 5; e++)
     if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
   (void)hipFree(c->geo_buf);

@@ -252,6 +252,42 @@ int gx_batch_autotune(gx_batch *batch, int nRepeat);
 long long gx_batch_cells(gx_batch *batch);
 void *gx_batch_stream(gx_batch *batch);
 
+/* ---------------------------------------------------------------------------
+ * Discrete adjoint (FP64 contexts).  The reference has no adjoint; these
+ * differentiate the step that gx_step_rk2 takes (Step_RK_C_ASC_Template %
+ * Compute, Step_RK_C_ASC__Template.f90:504-542) driven by the integrator's
+ * time steps (Integrator_Template.f90:896-939, nRampCycles ramp), time steps
+ * included (they follow the CFL maximum of the state).  Shock flags and the
+ * HLLC/HLL choice are replayed as constants; at ties of the limiter the
+ * centred slope is taken, and where exactly one of its two differences is
+ * zero the derivative is the mean of the two sides.
+ * ------------------------------------------------------------------------- */
+
+/* nSteps cycles from freshly uploaded primitives (gx_set_primitive /
+ * gx_upload), recording the time steps and the state every checkpointEvery
+ * steps.  Leaves the final state in the context. */
+int gx_adjoint_forward(gx_ctx *ctx, int nSteps, int checkpointEvery, int nRampCycles);
+
+/* Reverse pass of the recorded run: given Ubar = dJ/dU_final (five proper-cell
+ * fields D, S_1..3, G), returns P0bar = dJ/d(N, V_1..3, E) of the uploaded
+ * primitives and *gammaBar = dJ/dAdiabaticIndex.  Recomputes each segment
+ * from its checkpoint.  Synchronises. */
+int gx_adjoint_backward(gx_ctx *ctx, const double *Ubar, double *P0bar, double *gammaBar);
+
+/* The observable J = sum over cells with lo < y_c < hi of 1/2 N V_comp^2 dV
+ * (comp 0..2) on the current state, and Ubar = dJ/dU.  Synchronises. */
+int gx_adjoint_kinetic_energy(gx_ctx *ctx, int component, double lo, double hi,
+                              double *loss, double *Ubar);
+
+/* Host evaluations of the device code's reverse sweeps, for checking: one
+ * face flux (returns F and adds nothing: the bars are overwritten) and one
+ * limited slope (out = bars of vm, v0, vp for sbar). */
+void gx_adjoint_face_vjp(const double WL[5], const double WR[5], int d, int df,
+                         double gamma, double mass, int hllc, const double Fbar[5],
+                         double WLbar[5], double WRbar[5], double *gammaBar, double F[5]);
+void gx_adjoint_slope_vjp(double vm, double v0, double vp, double dxl, double dxr,
+                          double theta, int useLimiter, double sbar, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

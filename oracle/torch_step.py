@@ -1,0 +1,426 @@
+"""Differentiable FP64 transcription of the Fluid_P_I RK2 step in torch (CPU).
+
+TEST INFRASTRUCTURE ONLY: the checker of the discrete adjoint.  Imported by
+tests/ (never by the product package); autograd through it gives the
+reference gradient that the HIP adjoint must reproduce.
+
+It restates the same passes as gx_oracle.c on interior cells, padding each
+sweep with the ghost rule of that sweep (periodic wrap; OUTFLOW mirror;
+REFLECTING mirror with the normal velocity negated), in the reference's
+expression order:
+
+* limited slope      Gradient_Kernel.f90:166-190
+* reconstruction     IncrementDivergence_FV__Kernel.f90:11-94
+* EOS, prim -> cons  Fluid_P_I__Kernel.f90:197-271, Fluid_D__Kernel.f90:60-100
+* HLL / HLLC         Current_Kernel.f90:50-137, Fluid_P__Kernel.f90:319-668
+* increment          IncrementDivergence_FV__Kernel.f90:672-752
+* RK2 (Heun)         Step_RK__Template.f90:207-288
+* cons -> prim fix   Fluid_P__Kernel.f90:94-192
+* CFL                Integrator_C_PS__Kernel.f90:12-144
+* shock flags        FluidFeatures_P__Kernel.f90:12-313 (no gradient)
+
+Every branch is a torch.where, so autograd differentiates the branch the
+forward value takes.  At ties: fmax/fmin take the first operand, the limiter
+takes the centred slope when it equals the smaller one-sided slope, |x| has
+derivative copysign(1, x), and the CFL maximum is the first cell (x fastest)
+holding it.  Where exactly one of the two differences of a limited slope is
+zero (a uniform region meeting a jump), the slope is zero and one-sided
+linear in that difference: its derivative there is the mean of the two
+sides, theta/(2 dx), the value central differences measure.  Non-selected branches are evaluated on safe operands so that no
+NaN reaches the gradient.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+TINY = 2.2250738585072014e-308
+SQRT_TINY = 1.4916681462400413e-154
+HUGE = 1.7976931348623157e308
+F64 = torch.float64
+
+
+def fmax(a, b):
+    return torch.where(a >= b, a, b)
+
+
+def fmin(a, b):
+    return torch.where(a <= b, a, b)
+
+
+def fabs(x):
+    return torch.where(torch.signbit(x), -x, x)
+
+
+class _Sqrt(torch.autograd.Function):
+    """Correctly rounded FP64 square root (numpy's; torch's vectorised CPU
+    sqrt is not always correctly rounded), derivative 1/(2 sqrt(x))."""
+
+    @staticmethod
+    def forward(ctx, x):
+        out = torch.from_numpy(np.sqrt(x.detach().cpu().numpy()))
+        ctx.save_for_backward(out)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (out,) = ctx.saved_tensors
+        return g * 0.5 / out
+
+
+def sqrt(x):
+    return _Sqrt.apply(x)
+
+
+def half_sign(x):
+    """Fortran sign(0.5, x): +0.5 for +0 and positive x."""
+    return torch.where(torch.signbit(x), -0.5, 0.5).to(F64)
+
+
+class TorchFluid:
+    """One chart (genasis_amd.chart.Chart) with the step's parameters."""
+
+    def __init__(self, chart, theta=1.4, courant=0.8, n_min=0.0, mass=1.0,
+                 use_limiter=True, hllc=True, shock_threshold=0.1):
+        self.chart = chart
+        self.n = chart.n
+        self.theta = theta
+        self.courant = courant
+        self.n_min = n_min
+        self.mass = mass
+        self.use_limiter = use_limiter
+        self.hllc = hllc
+        self.st = shock_threshold
+        self.bc = chart.bc
+        g = chart.g
+        self.geo = []
+        for d in range(3):
+            c = np.asarray(chart.center[d])
+            m = len(c)
+            dxl = np.ones(m)
+            dxr = np.ones(m)
+            dxl[1:m - 1] = c[1:m - 1] - c[0:m - 2]
+            dxr[1:m - 1] = c[2:m] - c[1:m - 1]
+            self.geo.append({
+                "wl": torch.tensor(chart.width_left[d], dtype=F64),
+                "wr": torch.tensor(chart.width_right[d], dtype=F64),
+                "dxl": torch.tensor(dxl, dtype=F64),
+                "dxr": torch.tensor(dxr, dtype=F64),
+            })
+        w = [torch.tensor(chart.width_left[d][g:g + self.n[d]] +
+                          chart.width_right[d][g:g + self.n[d]], dtype=F64)
+             for d in range(3)]
+        self.width = w  # per dimension, proper cells
+        # broadcast shapes (n2, n1, n0)
+        self.dX = w[0].reshape(1, 1, -1)
+        self.dY = w[1].reshape(1, -1, 1)
+        self.dZ = w[2].reshape(-1, 1, 1)
+
+    # ---- helpers -----------------------------------------------------------
+    @staticmethod
+    def _axis(d):
+        return 2 - d  # arrays are (n2, n1, n0)
+
+    def _pad(self, V, d):
+        """(5, ...) interior primitives -> padded by two ghosts along d."""
+        n = self.n[d]
+        ax = 1 + self._axis(d)
+        idx, flip = [], []
+        bi, bo = self.bc[d]
+        for c in range(-2, n + 2):
+            f = False
+            if c < 0:
+                if bi == "PERIODIC":
+                    q = c + n
+                else:
+                    q = -1 - c
+                    f = bi == "REFLECTING"
+            elif c >= n:
+                if bo == "PERIODIC":
+                    q = c - n
+                else:
+                    q = 2 * n - 1 - c
+                    f = bo == "REFLECTING"
+            else:
+                q = c
+            idx.append(q)
+            flip.append(f)
+        Vp = torch.index_select(V, ax, torch.tensor(idx))
+        if any(flip):
+            shape = [1] * Vp.dim()
+            shape[ax] = n + 4
+            sgn = torch.tensor([-1.0 if f else 1.0 for f in flip], dtype=F64).reshape(shape)
+            Vp = torch.cat([Vp[:1 + d], Vp[1 + d:2 + d] * sgn, Vp[2 + d:]], dim=0)
+        return Vp
+
+    @staticmethod
+    def _sl(x, ax, a, b):
+        """x sliced [a:b] along axis ax."""
+        return x.narrow(ax, a, b - a)
+
+    def _geo(self, d, key, a, b):
+        shape = [1, 1, 1]
+        shape[self._axis(d)] = b - a
+        return self.geo[d][key][a:b].reshape(shape)
+
+    # ---- physics -------------------------------------------------------------
+    def slope(self, vm, v0, vp, dxl, dxr):
+        dVL = v0 - vm
+        dVR = vp - v0
+        central = (dxr * dxr * dVL + dxl * dxl * dVR) / (dxl * dxr * (dxl + dxr))
+        if not self.use_limiter:
+            return central
+        tl = fabs(self.theta * dVL / dxl)
+        tr = fabs(self.theta * dVR / dxr)
+        m1 = fmin(tl, tr)
+        ac = fabs(central)
+        m = torch.where(ac <= m1, ac, m1)
+        s = (half_sign(dVL) + half_sign(dVR)) * m
+        # Exactly one zero difference: the slope is 0 and one-sided linear in
+        # that difference; the derivative is the mean of the two sides,
+        # theta/(2 dx) (the value stays +0).
+        zL, zR = dVL == 0.0, dVR == 0.0
+        s = torch.where(zL & ~zR, self.theta * dVL / (2.0 * dxl), s)
+        return torch.where(zR & ~zL, self.theta * dVR / (2.0 * dxr), s)
+
+    def face_state(self, W, gamma):
+        N, V1, V2, V3, E = W
+        M = self.mass
+        P = (gamma - 1.0) * E
+        ok = (N > 0.0) & (P > 0.0)
+        Ns = torch.where(ok, N, torch.ones_like(N))
+        Ps = torch.where(ok, P, torch.ones_like(P))
+        CS = torch.where(ok, sqrt(gamma * Ps / (M * Ns)), torch.zeros_like(P))
+        N = torch.where(N < 0.0, torch.zeros_like(N), N)
+        D = N
+        S1, S2, S3 = M * N * V1, M * N * V2, M * N * V3
+        G = E + 0.5 * ((S1 * V1 + S2 * V2) + S3 * V3)
+        return dict(N=N, V=(V1, V2, V3), E=E, P=P, CS=CS, D=D, S=(S1, S2, S3), G=G)
+
+    @staticmethod
+    def raw_flux(D, S, G, P, Vd, d):
+        F = [D * Vd, S[0] * Vd, S[1] * Vd, S[2] * Vd]
+        F[1 + d] = F[1 + d] + P
+        F.append((G + P) * Vd)
+        return F
+
+    def face_flux(self, WL, WR, d, df, gamma):
+        M = self.mass
+        L = self.face_state(WL, gamma)
+        R = self.face_state(WR, gamma)
+        vdL, vdR = L["V"][d], R["V"][d]
+        zero = torch.zeros_like(vdL)
+        AP = fmax(fmax(zero, vdL + L["CS"]), vdR + R["CS"])
+        AM = fmax(fmax(zero, -(vdL - L["CS"])), -(vdR - R["CS"]))
+        FL = self.raw_flux(L["D"], L["S"], L["G"], L["P"], vdL, d)
+        FR = self.raw_flux(R["D"], R["S"], R["G"], R["P"], vdR, d)
+        UL = [L["D"], *L["S"], L["G"]]
+        UR = [R["D"], *R["S"], R["G"]]
+        den = fmax(AP + AM, torch.full_like(AP, TINY))
+        F = [(AP * FL[f] + AM * FR[f] - 1.0 * AP * AM * (UR[f] - UL[f])) / den
+             for f in range(5)]
+        if not self.hllc:
+            return F
+        SdL, SdR = L["S"][d], R["S"][d]
+        MD = AP * M * R["D"] + AM * M * L["D"] - FR[0] + FL[0]
+        SN = AP * SdR + AM * SdL - FR[1 + d] + FL[1 + d]
+        MDinv = fmax(MD, zero) / fmax(MD * MD, torch.full_like(MD, TINY))
+        AC = 1.0 * SN * MDinv
+        # left star state
+        AM_VL = AM + vdL
+        AM_AC = AM + AC
+        invL = 1.0 / fmax(fabs(AM_AC), torch.full_like(AM_AC, SQRT_TINY))
+        DsL = L["D"] * AM_VL * invL
+        VsL = [AC if k == d else L["V"][k] for k in range(3)]
+        SsL = [M * DsL * VsL[k] for k in range(3)]
+        PsL = L["P"] + SdL * AM_VL - SsL[d] * AM_AC
+        GsL = (L["G"] * AM_VL + vdL * L["P"] - AC * PsL) * invL
+        FsL = self.raw_flux(DsL, SsL, GsL, PsL, AC, d)
+        # right star state
+        AP_VR = AP - vdR
+        AP_AC = AP - AC
+        invR = 1.0 / fmax(fabs(AP_AC), torch.full_like(AP_AC, SQRT_TINY))
+        DsR = R["D"] * AP_VR * invR
+        VsR = [AC if k == d else R["V"][k] for k in range(3)]
+        SsR = [M * DsR * VsR[k] for k in range(3)]
+        PsR = R["P"] - SdR * AP_VR + SsR[d] * AP_AC
+        GsR = (R["G"] * AP_VR - vdR * R["P"] + AC * PsR) * invR
+        FsR = self.raw_flux(DsR, SsR, GsR, PsR, AC, d)
+        use = (~df) & (AP != 0.0) & (AM != 0.0)
+        left = AC >= 0.0
+        return [torch.where(use, torch.where(left, FsL[f], FsR[f]), F[f]) for f in range(5)]
+
+    def residual_increment(self, V, dt, flags, gamma):
+        """K = dt * (-div F) over the three dimensions (K = ((0+x)+y)+z)."""
+        K = [torch.zeros(self.n[::-1], dtype=F64) for _ in range(5)]
+        g = self.chart.g
+        for d in range(3):
+            n = self.n[d]
+            ax = self._axis(d)
+            Vp = self._pad(V, d)                         # cells -2 .. n+1
+            vm = self._sl(Vp, 1 + ax, 0, n + 2)          # cells -2 .. n-1 (j-1)
+            v0 = self._sl(Vp, 1 + ax, 1, n + 3)          # cells -1 .. n
+            vp = self._sl(Vp, 1 + ax, 2, n + 4)          # cells  0 .. n+1
+            dxl = self._geo(d, "dxl", g - 1, g + n + 1)  # padded p of cells -1..n
+            dxr = self._geo(d, "dxr", g - 1, g + n + 1)
+            s = torch.stack([self.slope(vm[f], v0[f], vp[f], dxl, dxr) for f in range(5)])
+            # faces i = 0..n between cells i-1 and i
+            Vc = self._sl(v0, 1 + ax, 0, n + 1)          # cells -1 .. n-1
+            Vn = self._sl(v0, 1 + ax, 1, n + 2)          # cells  0 .. n
+            sc = self._sl(s, 1 + ax, 0, n + 1)
+            sn = self._sl(s, 1 + ax, 1, n + 2)
+            wr = self._geo(d, "wr", g - 1, g + n)
+            wl = self._geo(d, "wl", g, g + n + 1)
+            WL = [Vc[f] + wr * sc[f] for f in range(5)]
+            WR = [Vn[f] - wl * sn[f] for f in range(5)]
+            fl = ((flags >> d) & 1).bool()
+            pad = torch.zeros_like(self._sl(fl, ax, 0, 1))
+            df = torch.cat([fl, pad], dim=ax)             # face n: ghost, cleared
+            F = self.face_flux(WL, WR, d, df, gamma)
+            A = (self.dY * self.dZ, self.dZ * self.dX, self.dX * self.dY)[d]
+            Vol = self.dX * self.dY * self.dZ
+            for f in range(5):
+                Flo = self._sl(F[f], ax, 0, n)
+                Fhi = self._sl(F[f], ax, 1, n + 1)
+                K[f] = K[f] + dt * (A * Flo - A * Fhi) / Vol
+        return torch.stack(K)
+
+    def primitive(self, U):
+        """Fluid_P__Kernel.f90:94-192 primitive part, E < 0 -> 0."""
+        M = self.mass
+        D, S1, S2, S3, G = U
+        ok = D > self.n_min
+        Ds = torch.where(ok, D, torch.ones_like(D))
+        V1 = S1 / (M * Ds)
+        V2 = S2 / (M * Ds)
+        V3 = S3 / (M * Ds)
+        E = G - 0.5 * ((S1 * V1 + S2 * V2) + S3 * V3)
+        zero = torch.zeros_like(D)
+        N = torch.where(ok, D, torch.full_like(D, self.n_min))
+        V1, V2, V3, E = (torch.where(ok, x, zero) for x in (V1, V2, V3, E))
+        E = torch.where(E < 0.0, zero, E)
+        return torch.stack([N, V1, V2, V3, E])
+
+    def fix(self, U):
+        """StoreSolution's conserved fix (Fluid_P__Kernel.f90:94-192)."""
+        M = self.mass
+        D, S1, S2, S3, G = U
+        ok = D > self.n_min
+        Ds = torch.where(ok, D, torch.ones_like(D))
+        V = [S / (M * Ds) for S in (S1, S2, S3)]
+        E = G - 0.5 * ((S1 * V[0] + S2 * V[1]) + S3 * V[2])
+        zero = torch.zeros_like(D)
+        D2 = torch.where(ok, D, torch.full_like(D, self.n_min))
+        S = [torch.where(ok, x, zero) for x in (S1, S2, S3)]
+        V = [torch.where(ok, x, zero) for x in V]
+        E = torch.where(ok, E, zero)
+        neg = E < 0.0
+        Grep = zero + 0.5 * ((S[0] * V[0] + S[1] * V[1]) + S[2] * V[2])
+        G2 = torch.where(ok, torch.where(neg, Grep, G), zero)
+        return torch.stack([D2, S[0], S[1], S[2], G2])
+
+    def from_primitive(self, P):
+        M = self.mass
+        N, V1, V2, V3, E = P
+        N = torch.where(N < 0.0, torch.zeros_like(N), N)
+        S1, S2, S3 = M * N * V1, M * N * V2, M * N * V3
+        G = E + 0.5 * ((S1 * V1 + S2 * V2) + S3 * V3)
+        return torch.stack([N, V1, V2, V3, E]), torch.stack([N, S1, S2, S3, G])
+
+    def invdt(self, V, gamma):
+        """Max over cells of the CFL term, at the first cell holding it."""
+        M = self.mass
+        N, V1, V2, V3, E = V
+        P = (gamma - 1.0) * E
+        ok = (N > 0.0) & (P > 0.0)
+        Ns = torch.where(ok, N, torch.ones_like(N))
+        Ps = torch.where(ok, P, torch.ones_like(P))
+        CS = torch.where(ok, sqrt(gamma * Ps / (M * Ns)), torch.zeros_like(P))
+        t = [fmax(Vd + CS, -(Vd - CS)) for Vd in (V1, V2, V3)]
+        term = t[0] / self.dX + t[1] / self.dY + t[2] / self.dZ
+        flat = term.reshape(-1)
+        return flat[int(torch.argmax(flat.detach()))]
+
+    def candidate(self, invdt):
+        tsi = fmax(torch.as_tensor(TINY, dtype=F64), invdt)
+        return self.courant * fmin(torch.as_tensor(HUGE, dtype=F64), 1.0 / tsi)
+
+    @torch.no_grad()
+    def shock_flags(self, V, gamma):
+        """DiffusiveFlux bits (bit d) of cells 1 .. n-2 from the state."""
+        P = (gamma - 1.0) * V[4]
+        n = self.n
+        bits = torch.zeros(n[::-1], dtype=torch.int64)
+        for d in range(3):
+            ax = self._axis(d)
+            Pm = torch.roll(P, 1, ax)
+            Vm = torch.roll(V[1 + d], 1, ax)
+            dP = torch.abs(P - Pm)
+            Pmin = torch.clamp(torch.minimum(P, Pm), min=SQRT_TINY)
+            b = ((dP / Pmin) > self.st) & ((V[1 + d] - Vm) <= 0.0)
+            idx = torch.arange(n[d]).reshape([-1 if a == ax else 1 for a in range(3)])
+            b = b & (idx >= 1)
+            bits |= b.to(torch.int64) << d
+        out = torch.zeros_like(bits)
+        for d in range(3):
+            f = torch.zeros(n[::-1], dtype=torch.bool)
+            for i in range(3):
+                if i == d:
+                    continue
+                m = 1 << i
+                bi = (bits & m) != 0
+                axi, axd = self._axis(i), self._axis(d)
+                f = f | bi | torch.roll(bi, -1, axi) | torch.roll(bi, 1, axd) | \
+                    torch.roll(torch.roll(bi, -1, axi), 1, axd)
+            out |= f.to(torch.int64) << d
+        inside = torch.ones(n[::-1], dtype=torch.bool)
+        for d in range(3):
+            ax = self._axis(d)
+            idx = torch.arange(n[d]).reshape([-1 if a == ax else 1 for a in range(3)])
+            inside = inside & (idx >= 1) & (idx <= n[d] - 2)
+        return torch.where(inside, out, torch.zeros_like(out))
+
+    # ---- the step --------------------------------------------------------------
+    def step(self, U, dt, flags, gamma, prims=None):
+        V = prims if prims is not None else self.primitive(U)
+        K1 = self.residual_increment(V, dt, flags, gamma)
+        Y = U + 1.0 * K1
+        K2 = self.residual_increment(self.primitive(Y), dt, flags, gamma)
+        W = U + 0.5 * K1
+        return self.fix(W + 0.5 * K2)
+
+    def run(self, P0, gamma, n_steps, n_ramp=100):
+        """n_steps cycles from primitives P0 (5, n2, n1, n0); returns the final
+        conserved state and the time steps.  Time steps follow the CFL of the
+        state (differentiable through the maximising cell)."""
+        V, U = self.from_primitive(P0)
+        flags = torch.zeros(self.n[::-1], dtype=torch.int64)
+        prims = V
+        dts = []
+        time = 0.0
+        for s in range(n_steps):
+            cand = self.candidate(self.invdt(prims if prims is not None else
+                                             self.primitive(U), gamma))
+            ts = fmin(torch.as_tensor(HUGE, dtype=F64), cand)
+            ramp = min(float(s + 1) / n_ramp, 1.0)
+            if ramp < 1.0:
+                ts = ramp * ts
+            # dt = (Time + ts) - Time: the rounding of the sum is a constant
+            ts_val = float(ts.detach())
+            dt_val = (time + ts_val) - time
+            dt = ts + (dt_val - ts_val)
+            time = time + dt_val
+            dts.append(dt)
+            U = self.step(U, dt, flags, gamma, prims=prims)
+            prims = None
+            flags = self.shock_flags(self.primitive(U).detach(), float(torch.as_tensor(gamma).detach()))
+        return U, dts
+
+    def kinetic_energy_vy(self, U, y_lo=0.2, y_hi=0.8):
+        """Sum over cells with y_lo < y_c < y_hi of 0.5 N vy^2 dV."""
+        g = self.chart.g
+        yc = torch.tensor(self.chart.center[1][g:g + self.n[1]], dtype=F64)
+        mask = ((yc > y_lo) & (yc < y_hi)).to(F64).reshape(1, -1, 1)
+        V = self.primitive(U)
+        vol = self.dX * self.dY * self.dZ
+        return torch.sum(mask * (0.5 * V[0] * V[2] * V[2]) * vol)
