@@ -263,7 +263,7 @@ class TorchFluid:
             vp = self._sl(Vp, 1 + ax, 2, n + 4)          # cells  0 .. n+1
             dxl = self._geo(d, "dxl", g - 1, g + n + 1)  # padded p of cells -1..n
             dxr = self._geo(d, "dxr", g - 1, g + n + 1)
-            s = torch.stack([self.slope(vm[f], v0[f], vp[f], dxl, dxr) for f in range(5)])
+            s = self.slope(vm, v0, vp, dxl, dxr)
             # faces i = 0..n between cells i-1 and i
             Vc = self._sl(v0, 1 + ax, 0, n + 1)          # cells -1 .. n-1
             Vn = self._sl(v0, 1 + ax, 1, n + 2)          # cells  0 .. n
