@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -53,9 +54,14 @@ struct AdjointState {
   int nsteps = 0, every = 0;
   std::vector<double> dt, ts, invdt, ramp;
   std::vector<double *> ckpt;
-  double *seg = nullptr;       // states U^n of one segment, every x 5 ncell
-  double *Z = nullptr, *trash = nullptr;
+  // Per step of one segment (every steps): the input U^n, the stage
+  // values Y and W, the state before the fix Z, and the shock flags.
+  double *seg = nullptr, *segY = nullptr, *segW = nullptr, *segZ = nullptr;
+  uint8_t *segF = nullptr;
+  double *trash = nullptr;
   double *Vp = nullptr, *Vb = nullptr, *plus = nullptr, *minus = nullptr;  // 5 np
+  double *S = nullptr;         // slopes, 5 np
+  double *Dp = nullptr;        // slope partials d s / d(V_{j-1}, V_j, V_{j+1}), 15 np
   double *Ub = nullptr, *Yb = nullptr, *Wb = nullptr, *K1b = nullptr, *K2b = nullptr,
          *P0b = nullptr;       // 5 ncell
   double *partials = nullptr;  // 2 per face block, both stages, three dimensions
@@ -112,15 +118,58 @@ __device__ inline double cell_width(const StageArgs<double> &a, int e, int c) {
   return geo(a, e, G_WIDTH, c + 2);
 }
 
+// Slope pass of dimension D over cells -1 .. n along D (transverse
+// interior): the forward's limited slope of every field and its three
+// partial derivatives (the limiter's branch, replayed).
+template <int D>
+__global__ __launch_bounds__(AB) void adj_slope_kernel(StageArgs<double> a, PadGeom g,
+                                                       const double *Vp, double *S,
+                                                       double *Dp) {
+  const int e0 = D == 0 ? g.n[0] + 2 : g.n[0];
+  const int e1 = D == 1 ? g.n[1] + 2 : g.n[1];
+  const int e2 = D == 2 ? g.n[2] + 2 : g.n[2];
+  const long long nc = (long long)e0 * e1 * e2;
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < nc;
+       t += (long long)gridDim.x * AB) {
+    int c3[3];
+    c3[0] = (int)(t % e0);
+    c3[1] = (int)((t / e0) % e1);
+    c3[2] = (int)(t / ((long long)e0 * e1));
+    c3[D] -= 1;
+    const int q = c3[D] + 2;  // padded geometry index
+    const long long sd = g.s[D];
+    const long long p = plin(g, c3[0], c3[1], c3[2]);
+    const double dxl = geo(a, D, G_DXL, q), dxr = geo(a, D, G_DXR, q);
+    const double dxl2 = geo(a, D, G_DXL2, q), dxr2 = geo(a, D, G_DXR2, q);
+    const double den = geo(a, D, G_DEN, q);
+    const double rdxl = geo(a, D, G_RDXL, q), rdxr = geo(a, D, G_RDXR, q);
+    const double rden = geo(a, D, G_RDEN, q);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      const double *v = Vp + f * g.np + p;
+      const double vm = v[-sd], v0 = v[0], vp = v[sd];
+      S[f * g.np + p] = slope(vm, v0, vp, dxl, dxr, dxl2, dxr2, den, rdxl, rdxr, rden,
+                              a.ph.theta, a.ph.use_limiter);
+      double dm = 0.0, d0 = 0.0, dp = 0.0;
+      adj::slope_vjp(vm, v0, vp, dxl, dxr, dxl2, dxr2, den, a.ph.theta, a.ph.use_limiter, 1.0,
+                     dm, d0, dp);
+      Dp[(3 * f) * g.np + p] = dm;
+      Dp[(3 * f + 1) * g.np + p] = d0;
+      Dp[(3 * f + 2) * g.np + p] = dp;
+    }
+  }
+}
+
 // Face pass of dimension D: the reverse of the face flux for every face of
 // the sweep (faces 0 .. n_D between cells i-1 and i, transverse interior).
 // Fb = dt * A * (Kb_i / Vol_i - Kb_{i-1} / Vol_{i-1}); writes the bar of the
 // left state to plus[i-1] and of the right state to minus[i], and per block
 // the sums of F . Fb / dt (the dt bar) and of the gamma bar.
 template <int D>
-__global__ __launch_bounds__(AB) void adj_face_kernel(StageArgs<double> a, PadGeom g,
-                                                      const double *Vp, const double *Kb,
-                                                      double dt, double *plus, double *minus,
+__global__ __launch_bounds__(AB, 3) void adj_face_kernel(StageArgs<double> a, PadGeom g,
+                                                      const double *Vp, const double *S,
+                                                      const double *Kb, double dt,
+                                                      double *plus, double *minus,
                                                       double *partial) {
   __shared__ double sh[AB];
   const int e0 = g.n[0] + (D == 0), e1 = g.n[1] + (D == 1), e2 = g.n[2] + (D == 2);
@@ -135,31 +184,13 @@ __global__ __launch_bounds__(AB) void adj_face_kernel(StageArgs<double> a, PadGe
     const int i = c3[D], n = g.n[D];
     const long long sd = g.s[D];
     const long long p0 = plin(g, c3[0], c3[1], c3[2]);
-    double Vm2[5], Vm1[5], V0[5], Vp1[5];
-#pragma unroll
-    for (int f = 0; f < 5; f++) {
-      const double *v = Vp + f * g.np + p0;
-      Vm2[f] = v[-2 * sd];
-      Vm1[f] = v[-sd];
-      V0[f] = v[0];
-      Vp1[f] = v[sd];
-    }
-    const double th = a.ph.theta;
+    const double wr = geo(a, D, G_WR, i + 1), wl = geo(a, D, G_WL, i + 2);
     double WL[5], WR[5];
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      const double sl = slope(Vm2[f], Vm1[f], V0[f], geo(a, D, G_DXL, i + 1),
-                              geo(a, D, G_DXR, i + 1), geo(a, D, G_DXL2, i + 1),
-                              geo(a, D, G_DXR2, i + 1), geo(a, D, G_DEN, i + 1),
-                              geo(a, D, G_RDXL, i + 1), geo(a, D, G_RDXR, i + 1),
-                              geo(a, D, G_RDEN, i + 1), th, a.ph.use_limiter);
-      const double sr = slope(Vm1[f], V0[f], Vp1[f], geo(a, D, G_DXL, i + 2),
-                              geo(a, D, G_DXR, i + 2), geo(a, D, G_DXL2, i + 2),
-                              geo(a, D, G_DXR2, i + 2), geo(a, D, G_DEN, i + 2),
-                              geo(a, D, G_RDXL, i + 2), geo(a, D, G_RDXR, i + 2),
-                              geo(a, D, G_RDEN, i + 2), th, a.ph.use_limiter);
-      WL[f] = Vm1[f] + geo(a, D, G_WR, i + 1) * sl;
-      WR[f] = V0[f] - geo(a, D, G_WL, i + 2) * sr;
+      const long long o = f * g.np + p0;
+      WL[f] = Vp[o - sd] + wr * S[o - sd];
+      WR[f] = Vp[o] - wl * S[o];
     }
     bool df = false;
     if (i < n) df = (a.flags[lin(a.n, c3[0], c3[1], c3[2])] >> D) & 1;
@@ -205,12 +236,12 @@ __global__ __launch_bounds__(AB) void adj_face_kernel(StageArgs<double> a, PadGe
 }
 
 // Gather pass of dimension D over cells -2 .. n+1 along D (transverse
-// interior): Vb += bars of the cell's own face states, and the limiter
-// reverse of the slopes of cells c-1, c, c+1 (slope bar s_j = wr_j plus_j -
-// wl_j minus_j) with respect to cell c.
+// interior): Vb += bars of the cell's own face states, and the slope partials
+// of cells c-1, c, c+1 with respect to cell c times their slope bars
+// (s_j bar = wr_j plus_j - wl_j minus_j).
 template <int D>
 __global__ __launch_bounds__(AB) void adj_gather_kernel(StageArgs<double> a, PadGeom g,
-                                                        const double *Vp, const double *plus,
+                                                        const double *Dp, const double *plus,
                                                         const double *minus, double *Vb) {
   const int e0 = D == 0 ? g.n[0] + 4 : g.n[0];
   const int e1 = D == 1 ? g.n[1] + 4 : g.n[1];
@@ -233,26 +264,20 @@ __global__ __launch_bounds__(AB) void adj_gather_kernel(StageArgs<double> a, Pad
       if (hasp) acc[f] += plus[f * g.np + p];
       if (hasm) acc[f] += minus[f * g.np + p];
     }
+#pragma unroll
     for (int o = -1; o <= 1; o++) {
       const int j = c + o;
       if (j < -1 || j > n) continue;
       const long long pj = p + o * sd;
-      const int q = j + 2;  // padded geometry index
+      const int q = j + 2;
       const double wr = geo(a, D, G_WR, q), wl = geo(a, D, G_WL, q);
-      const double dxl = geo(a, D, G_DXL, q), dxr = geo(a, D, G_DXR, q);
-      const double dxl2 = geo(a, D, G_DXL2, q), dxr2 = geo(a, D, G_DXR2, q);
-      const double den = geo(a, D, G_DEN, q);
+      const int role = 1 - o;  // c is j-1 (0), j (1) or j+1 (2)
 #pragma unroll
       for (int f = 0; f < 5; f++) {
         double sb = 0.0;
         if (j <= n - 1) sb += wr * plus[f * g.np + pj];
         if (j >= 0) sb -= wl * minus[f * g.np + pj];
-        if (sb == 0.0) continue;
-        const double *v = Vp + f * g.np + pj;
-        double vmb = 0.0, v0b = 0.0, vpb = 0.0;
-        adj::slope_vjp(v[-sd], v[0], v[sd], dxl, dxr, dxl2, dxr2, den, a.ph.theta,
-                       a.ph.use_limiter, sb, vmb, v0b, vpb);
-        acc[f] += o == 1 ? vmb : (o == 0 ? v0b : vpb);
+        acc[f] += Dp[(3 * f + role) * g.np + pj] * sb;
       }
     }
 #pragma unroll
@@ -470,8 +495,9 @@ void release_adjoint(gx_ctx *c) {
   AdjointState *A = c->adj;
   if (!A) return;
   for (double *p : A->ckpt) (void)hipFree(p);
-  for (void *p : {(void *)A->seg, (void *)A->Z, (void *)A->trash, (void *)A->Vp,
-                  (void *)A->Vb, (void *)A->plus, (void *)A->minus, (void *)A->Ub,
+  for (void *p : {(void *)A->seg, (void *)A->segY, (void *)A->segW, (void *)A->segZ,
+                  (void *)A->segF, (void *)A->trash, (void *)A->Vp, (void *)A->S,
+                  (void *)A->Dp, (void *)A->Vb, (void *)A->plus, (void *)A->minus, (void *)A->Ub,
                   (void *)A->Yb, (void *)A->Wb, (void *)A->K1b, (void *)A->K2b,
                   (void *)A->P0b, (void *)A->partials, (void *)A->gb, (void *)A->argmax,
                   (void *)A->flags, (void *)A->band, (void *)A->lossp})
@@ -531,12 +557,13 @@ static int launch_stage1(gx_ctx *c, const double *src, bool prim, const double *
 }
 
 static int launch_stage2(gx_ctx *c, const double *Y, const double *W, double *out,
-                         double dt, const uint8_t *flags) {
+                         double *Z, double dt, const uint8_t *flags) {
   StageArgs<double> a = base_args<double>(c);
   a.dt = dt;
   a.src = Y;
   a.aux = W;
   a.out1 = out;
+  a.out2 = Z;
   a.flags = flags;
   stage_geometry(c, a);
   HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
@@ -560,29 +587,28 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, 
   int rc = fill(c, A->Vb, 5 * g.np, 0.0);
   if (rc) return rc;
   for (int d = 0; d < 3; d++) {
-    long long nf = 1, ng = 1;
+    long long nf = 1, ng = 1, ns = 1;
     for (int e = 0; e < 3; e++) {
       nf *= g.n[e] + (e == d);
       ng *= e == d ? g.n[e] + 4 : g.n[e];
+      ns *= e == d ? g.n[e] + 2 : g.n[e];
     }
     const int fb = (int)((nf + AB - 1) / AB);
     double *pp = part + 2 * (size_t)d * A->part_stride;
-    if (d == 0) {
-      adj_face_kernel<0><<<fb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->plus, A->minus, pp);
-      HIP_TRY(hipGetLastError());
-      adj_gather_kernel<0><<<blocks_for(ng), AB, 0, c->stream>>>(a, g, A->Vp, A->plus,
-                                                                  A->minus, A->Vb);
-    } else if (d == 1) {
-      adj_face_kernel<1><<<fb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->plus, A->minus, pp);
-      HIP_TRY(hipGetLastError());
-      adj_gather_kernel<1><<<blocks_for(ng), AB, 0, c->stream>>>(a, g, A->Vp, A->plus,
-                                                                  A->minus, A->Vb);
-    } else {
-      adj_face_kernel<2><<<fb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->plus, A->minus, pp);
-      HIP_TRY(hipGetLastError());
-      adj_gather_kernel<2><<<blocks_for(ng), AB, 0, c->stream>>>(a, g, A->Vp, A->plus,
-                                                                  A->minus, A->Vb);
-    }
+    auto pass = [&](auto dim) {
+      constexpr int D = decltype(dim)::value;
+      adj_slope_kernel<D><<<blocks_for(ns), AB, 0, c->stream>>>(a, g, A->Vp, A->S, A->Dp);
+      adj_face_kernel<D><<<fb, AB, 0, c->stream>>>(a, g, A->Vp, A->S, Kb, dt, A->plus,
+                                                   A->minus, pp);
+      adj_gather_kernel<D><<<blocks_for(ng), AB, 0, c->stream>>>(a, g, A->Dp, A->plus,
+                                                                 A->minus, A->Vb);
+    };
+    if (d == 0)
+      pass(std::integral_constant<int, 0>());
+    else if (d == 1)
+      pass(std::integral_constant<int, 1>());
+    else
+      pass(std::integral_constant<int, 2>());
     HIP_TRY(hipGetLastError());
   }
   adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, g, A->Vb, out);
@@ -613,10 +639,12 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   const int nck = (nsteps + every - 1) / every + 1;
   A->ckpt.assign(nck, nullptr);
   for (int i = 0; i < nck; i++) HIP_TRY(hipMalloc(&A->ckpt[i], s5));
-  HIP_TRY(hipMalloc(&A->seg, s5 * every));
-  for (double **p : {&A->Z, &A->trash, &A->Ub, &A->Yb, &A->Wb, &A->K1b, &A->K2b, &A->P0b})
+  for (double **p : {&A->seg, &A->segY, &A->segW, &A->segZ}) HIP_TRY(hipMalloc(p, s5 * every));
+  HIP_TRY(hipMalloc(&A->segF, (size_t)c->ncell * every));
+  for (double **p : {&A->trash, &A->Ub, &A->Yb, &A->Wb, &A->K1b, &A->K2b, &A->P0b})
     HIP_TRY(hipMalloc(p, s5));
-  for (double **p : {&A->Vp, &A->Vb, &A->plus, &A->minus}) HIP_TRY(hipMalloc(p, p5));
+  for (double **p : {&A->Vp, &A->Vb, &A->plus, &A->minus, &A->S}) HIP_TRY(hipMalloc(p, p5));
+  HIP_TRY(hipMalloc(&A->Dp, 3 * p5));
   long long maxf = 0;
   for (int d = 0; d < 3; d++) {
     long long nf = 1;
@@ -692,59 +720,49 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   const int every = A->every, N = A->nsteps;
   for (int seg0 = ((N - 1) / every) * every; seg0 >= 0; seg0 -= every) {
     const int len = std::min(every, N - seg0);
-    // states U^n, n = seg0 .. seg0+len-1
+    // Recompute the segment from its checkpoint, keeping per step the input
+    // U^n, the flags, Y, W and the state before the fix Z.
     HIP_TRY(hipMemcpyAsync(A->seg, A->ckpt[seg0 / every], s5, hipMemcpyDeviceToDevice,
                            c->stream));
-    for (int j = 0; j + 1 < len; j++) {
+    for (int j = 0; j < len; j++) {
       const int n = seg0 + j;
-      double *Un = A->seg + (size_t)j * n5;
-      double *Un1 = A->seg + (size_t)(j + 1) * n5;
+      const double *Un = A->seg + (size_t)j * n5;
+      double *Un1 = j + 1 < len ? A->seg + (size_t)(j + 1) * n5 : A->trash;
+      uint8_t *Fn = A->segF + (size_t)j * c->ncell;
       if (n == 0) {
-        HIP_TRY(hipMemsetAsync(A->flags, 0, (size_t)c->ncell, c->stream));
+        HIP_TRY(hipMemsetAsync(Fn, 0, (size_t)c->ncell, c->stream));
       } else {
-        rc = launch_flags(c, Un, A->flags);
+        rc = launch_flags(c, Un, Fn);
         if (rc) return rc;
       }
-      rc = launch_stage1(c, n == 0 ? P0 : Un, n == 0, Un, (double *)c->Y, (double *)c->W,
-                         A->dt[n], A->flags);
+      double *Yn = A->segY + (size_t)j * n5, *Wn = A->segW + (size_t)j * n5;
+      rc = launch_stage1(c, n == 0 ? P0 : Un, n == 0, Un, Yn, Wn, A->dt[n], Fn);
       if (rc) return rc;
-      rc = launch_stage2(c, (const double *)c->Y, (const double *)c->W, Un1, A->dt[n],
-                         A->flags);
+      rc = launch_stage2(c, Yn, Wn, Un1, A->segZ + (size_t)j * n5, A->dt[n], Fn);
       if (rc) return rc;
     }
     for (int j = len - 1; j >= 0; j--) {
       const int n = seg0 + j;
       const double *Un = A->seg + (size_t)j * n5;
+      const double *Yn = A->segY + (size_t)j * n5;
+      const uint8_t *Fn = A->segF + (size_t)j * c->ncell;
       const bool first = n == 0;
-      if (first) {
-        HIP_TRY(hipMemsetAsync(A->flags, 0, (size_t)c->ncell, c->stream));
-      } else {
-        rc = launch_flags(c, Un, A->flags);
-        if (rc) return rc;
-      }
       const double dt = A->dt[n];
       rc = fill(c, A->partials, 12 * (long long)A->part_stride, 0.0);
       if (rc) return rc;
-      // Y, W, and Z = W + K2/2
-      rc = launch_stage1(c, first ? P0 : Un, first, Un, (double *)c->Y, (double *)c->W, dt,
-                         A->flags);
-      if (rc) return rc;
-      rc = launch_stage1(c, (const double *)c->Y, false, (const double *)c->W, A->trash, A->Z,
-                         dt, A->flags);
-      if (rc) return rc;
       adj_fix_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
-          c->ncell, c->params.BaryonMassReference, c->params.BaryonDensityMin, A->Z, A->Ub,
-          A->Wb, A->K2b);
+          c->ncell, c->params.BaryonMassReference, c->params.BaryonDensityMin,
+          A->segZ + (size_t)j * n5, A->Ub, A->Wb, A->K2b);
       HIP_TRY(hipGetLastError());
       rc = fill(c, A->Yb, n5, 0.0);
       if (rc) return rc;
-      rc = stage_vjp(c, (const double *)c->Y, false, A->K2b, dt, A->flags, A->Yb,
+      rc = stage_vjp(c, Yn, false, A->K2b, dt, Fn, A->Yb,
                      A->partials + 6 * (size_t)A->part_stride);
       if (rc) return rc;
       adj_rk_kernel<<<blocks_for(n5), AB, 0, c->stream>>>(n5, A->Yb, A->Wb, A->Ub, A->K1b);
       HIP_TRY(hipGetLastError());
       double *out = first ? A->P0b : A->Ub;
-      rc = stage_vjp(c, first ? P0 : Un, first, A->K1b, dt, A->flags, out, A->partials);
+      rc = stage_vjp(c, first ? P0 : Un, first, A->K1b, dt, Fn, out, A->partials);
       if (rc) return rc;
       // dt = ts(invdt(U^n))
       StageArgs<double> a = base_args<double>(c);

@@ -88,9 +88,9 @@ __host__ __device__ inline StageArgs<R> problem_args(const BatchArgs<R> &B,
   a.ph = B.ph;
   const long long o = 5 * P.off;
   a.src = (use_prim ? B.prim : B.src) + o;
-  a.aux = B.aux + o;
-  a.out1 = B.out1 + o;
-  a.out2 = B.out2 + o;
+  a.aux = B.aux ? B.aux + o : nullptr;
+  a.out1 = B.out1 ? B.out1 + o : nullptr;
+  a.out2 = B.out2 ? B.out2 + o : nullptr;  // stage 2 writes the pre-fix state only if set
   a.geo = B.geo + P.geo_off;
   a.flags = B.flags + P.off;
   a.invdt_bits = B.invdt + p;

@@ -51,7 +51,7 @@ struct StageArgs {
   const R *src;         // stage input: conserved (or primitive) state
   const R *aux;         // stage 1: U (Solution); stage 2: W = U + K1/2
   R *out1;              // stage 1: Y = U + K1; stage 2: U_new
-  R *out2;              // stage 1: W
+  R *out2;              // stage 1: W; stage 2: the state before the fix, if set
   const R *geo;
   const uint8_t *flags; // DiffusiveFlux bits of the step
   unsigned long long *invdt_bits;  // stage 2: max CFL term (as FP64 bits)
@@ -467,6 +467,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         double U[5], V[5];
 #pragma unroll
         for (int f = 0; f < 5; f++) U[f] = (double)(ar[f] + R(0.5) * Kc[f]);
+        if (a.out2 != nullptr) {
+#pragma unroll
+          for (int f = 0; f < 5; f++) st(a.out2 + f * a.fs, ic, (R)U[f]);
+        }
         fix_conserved(U, ph.mass, ph.nmin, V);
 #pragma unroll
         for (int f = 0; f < 5; f++) st(a.out1 + f * a.fs, ic, (R)U[f]);
