@@ -27,6 +27,14 @@ FIELD_NAMES = (
     "EntropyPerBaryon", "ConservedEntropy", "SoundSpeed", "MachNumber",
 )
 
+# Conservation tallies (Tally_F_P % Value order, 0-based)
+TALLY_NAMES = (
+    "BaryonNumber", "Momentum_1", "Momentum_2", "Momentum_3", "KineticEnergy",
+    "AngularMomentum_1", "AngularMomentum_2", "AngularMomentum_3",
+    "GravitationalEnergy", "TotalEnergy", "FluidEnergy", "InternalEnergy",
+)
+GX_TALLY_N = len(TALLY_NAMES)
+
 # Every symbol include/gx_fluid.h declares.
 EXPORTS = (
     "gx_default_params", "gx_create", "gx_destroy", "gx_upload", "gx_download",
@@ -35,6 +43,7 @@ EXPORTS = (
     "gx_get_conserved", "gx_get_primitive", "gx_get_flags", "gx_set_flags",
     "gx_set_tuning", "gx_stream", "gx_set_profiling", "gx_kernel_times",
     "gx_self_test_division", "gx_last_error",
+    "gx_get_boundary_fluence", "gx_compute_tally",
     "gx_batch_create", "gx_batch_destroy", "gx_batch_set_primitive",
     "gx_batch_set_integrator", "gx_batch_step", "gx_batch_sync", "gx_batch_cfl",
     "gx_batch_times", "gx_batch_get_conserved", "gx_batch_get_flags",
@@ -98,6 +107,8 @@ def load(path: str = LIB_PATH):
         "gx_step_rk2": (i, [vp, d, d]),
         "gx_sync": (i, [vp]),
         "gx_totals": (i, [vp, pd]),
+        "gx_get_boundary_fluence": (i, [vp, pd]),
+        "gx_compute_tally": (i, [vp, i, pd, pd, pd, pd]),
         "gx_set_state": (i, [vp, pd, pd]),
         "gx_set_primitive": (i, [vp, pd]),
         "gx_get_conserved": (i, [vp, pd]),

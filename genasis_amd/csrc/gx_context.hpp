@@ -8,6 +8,7 @@
 
 #include "../../include/gx_fluid.h"
 #include "gx_host.hpp"
+#include "gx_tally.hpp"
 #include "gx_tiles.hpp"
 
 namespace gx {
@@ -41,7 +42,18 @@ struct gx_ctx {
   bool uploaded = false;
   bool prims_valid = false;   // next stage 1 reads P0
   bool invdt_valid = false;   // invdt holds the max for the current state
-  // per-kernel timing (gx_set_profiling): stage 1, stage 2, shock flags, (unused)
+  // boundary fluence of the last step and the conservation tallies
+  double *fluence = nullptr;       // fluence_count(n) values
+  double *tgeo = nullptr;          // FP64 tally geometry: N_TG x 3 dimensions x gs
+  double *tally_local = nullptr;   // device TallyBoundaryLocal (TALLY_N)
+  double *bpartial = nullptr;      // boundary partials (n_bpartial x NB_SUM)
+  int n_bpartial = 0;              // blocks of the boundary kernels: one line each
+  double *tpartial = nullptr;      // interior partials (n_partial x NI_SUM)
+  double tally_global[gx::TALLY_N] = {};  // TallyBoundaryGlobal
+  double tally_total[gx::TALLY_N] = {};   // TallyTotal
+  double tally_change[gx::TALLY_N] = {};  // TallyChange
+  // per-kernel timing (gx_set_profiling): stage 1, stage 2 (each with its
+  // boundary-fluence blocks), shock flags, boundary tally
   bool profiling = false;
   hipEvent_t ev[5] = {};
   double kernel_ms[4] = {0, 0, 0, 0};

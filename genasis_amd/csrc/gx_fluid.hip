@@ -3,10 +3,13 @@
 // chart per context.
 //
 // One RK2 step = two launches of the fused stage kernel (reconstruction,
-// fluxes and divergence of all three dimensions, plus the RK combination)
-// and one launch of the shock-flag kernel for the next step.
+// fluxes and divergence of all three dimensions, plus the RK combination;
+// blocks appended to it record the boundary fluence), one launch of the
+// shock-flag kernel for the next step, and one small launch that adds the
+// step's surface integral to the boundary tally.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +21,7 @@
 #include "gx_host.hpp"
 #include "gx_kernels.hpp"
 #include "gx_tiles.hpp"
+#include "gx_tally.hpp"
 #include "gx_context.hpp"
 
 namespace gx {
@@ -116,6 +120,11 @@ int gx_destroy(gx_ctx *c) {
   gx::release_adjoint(c);
   for (int e = 0; e < 5; e++)
     if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
+  (void)hipFree(c->fluence);
+  (void)hipFree(c->tgeo);
+  (void)hipFree(c->tally_local);
+  (void)hipFree(c->bpartial);
+  (void)hipFree(c->tpartial);
   (void)hipFree(c->geo_buf);
   (void)hipFree(c->U);
   (void)hipFree(c->Y);
@@ -185,6 +194,26 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
   c->n_partial = grid_blocks(c->ncell);
   if (c->n_partial > 1024) c->n_partial = 1024;
   TRY_CLEAN(hipMalloc(&c->partial, sizeof(double) * 5 * c->n_partial));
+  {
+    const size_t nf = (size_t)fluence_count(c->n);
+    TRY_CLEAN(hipMalloc(&c->fluence, sizeof(double) * nf));
+    TRY_CLEAN(hipMemset(c->fluence, 0, sizeof(double) * nf));
+    std::vector<double> tg((size_t)3 * N_TG * c->gs, 0.0);
+    for (int d = 0; d < 3; d++)
+      for (int p = 0; p < c->n[d] + 4; p++) {
+        const double ctr = c->geom.center[d][p], wl = c->geom.a[d][G_WL][p];
+        tg[(size_t)(3 * d + TG_CENTER) * c->gs + p] = ctr;
+        tg[(size_t)(3 * d + TG_WIDTH) * c->gs + p] = wl + c->geom.a[d][G_WR][p];
+        tg[(size_t)(3 * d + TG_FACE) * c->gs + p] = ctr - wl;
+      }
+    TRY_CLEAN(hipMalloc(&c->tgeo, sizeof(double) * tg.size()));
+    TRY_CLEAN(hipMemcpy(c->tgeo, tg.data(), sizeof(double) * tg.size(), hipMemcpyHostToDevice));
+    TRY_CLEAN(hipMalloc(&c->tally_local, sizeof(double) * TALLY_N));
+    TRY_CLEAN(hipMemset(c->tally_local, 0, sizeof(double) * TALLY_N));
+    c->n_bpartial = (int)((boundary_lines(c->n) + TALLY_BLOCK - 1) / TALLY_BLOCK);
+    TRY_CLEAN(hipMalloc(&c->bpartial, sizeof(double) * NB_SUM * c->n_bpartial));
+    TRY_CLEAN(hipMalloc(&c->tpartial, sizeof(double) * NI_SUM * c->n_partial));
+  }
 #undef TRY_CLEAN
   *out = c;
   return GX_OK;
@@ -219,12 +248,25 @@ int gx_kernel_times(gx_ctx *c, double ms[4], long long count[4]) {
   return GX_OK;
 }
 
+// A new state starts new tallies: fluence, boundary tallies, total and
+// change cleared (the reference's tallies start at zero with the Current).
+static int reset_tallies(gx_ctx *c) {
+  HIP_TRY(hipMemsetAsync(c->fluence, 0, sizeof(double) * (size_t)fluence_count(c->n),
+                         c->stream));
+  HIP_TRY(hipMemsetAsync(c->tally_local, 0, sizeof(double) * TALLY_N, c->stream));
+  for (int t = 0; t < TALLY_N; t++)
+    c->tally_global[t] = c->tally_total[t] = c->tally_change[t] = 0.0;
+  return GX_OK;
+}
+
 int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
   if (!c || !primitive || !conserved) return fail(GX_E_ARG, "gx_set_state: null argument");
   const size_t n = 5 * (size_t)c->ncell;
   int rc = put_values(c->stream, c->fp32, c->P0, primitive, n);
   if (rc) return rc;
   rc = put_values(c->stream, c->fp32, c->U, conserved, n);
+  if (rc) return rc;
+  rc = reset_tallies(c);
   if (rc) return rc;
   HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -245,6 +287,8 @@ int gx_set_primitive(gx_ctx *c, const double *primitive) {
     return 0;
   });
   HIP_TRY(hipGetLastError());
+  const int rt = reset_tallies(c);
+  if (rt) return rt;
   HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->uploaded = true;
@@ -452,8 +496,13 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   const int nbx = (c->n[0] + TX - 1) / TX, nby = (c->n[1] + TY - 1) / TY,
             nbz = (c->n[2] + c->tz - 1) / c->tz;
   const int nblk = nbx * nby * nbz;
-  const dim3 grid(nblk);
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  BoundaryArgs B;
+  B.bf = c->fluence;
+  B.fw = 0.5 * dt;  // Weight_RK * TimeStep, B = (1/2, 1/2)
+  B.tg = c->tgeo;
+  B.partial = c->bpartial;
+  hipStream_t st = c->stream;
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[0], st));
   const int rc = by_precision(c->fp32, [&](auto tag) {
     using R = decltype(tag);
     // Stage 1: Current = uploaded primitives or the Solution.
@@ -467,11 +516,12 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     a1.nbx = nbx;
     a1.nby = nby;
     a1.nblk = nblk;
-    stage_kernel<R, 1><<<grid, block, 0, c->stream>>>(a1);
+    stage_kernel<R, 1><<<nblk, block, 0, st>>>(a1);
+    boundary_flux_kernel<R, 1><<<c->n_bpartial, TALLY_BLOCK, 0, st>>>(a1, B);
     HIP_TRY(hipGetLastError());
     // Stage 2: Current = Y.
-    HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
-    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), st));
+    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], st));
     StageArgs<R> a2 = base_args<R>(c);
     a2.dt = dt;
     a2.src = (const R *)c->Y;
@@ -480,27 +530,35 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     a2.nbx = nbx;
     a2.nby = nby;
     a2.nblk = nblk;
-    stage_kernel<R, 2><<<grid, block, 0, c->stream>>>(a2);
+    stage_kernel<R, 2><<<nblk, block, 0, st>>>(a2);
+    boundary_flux_kernel<R, 2><<<c->n_bpartial, TALLY_BLOCK, 0, st>>>(a2, B);
     HIP_TRY(hipGetLastError());
-    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    // Shock flags of the next step from the new state.
+    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], st));
+    // Shock flags of the next step from the new state (after stage 2's
+    // boundary blocks have read the step's flags).
     StageArgs<R> a3 = base_args<R>(c);
     a3.src = (const R *)c->U;
     a3.tz = FLAG_TZ;
     a3.nbx = nbx;
     a3.nby = nby;
     a3.nblk = nbx * nby * ((c->n[2] + FLAG_TZ - 1) / FLAG_TZ);
-    shock_flags_kernel<R><<<a3.nblk, NT, 0, c->stream>>>(a3, c->flags);
+    shock_flags_kernel<R><<<a3.nblk, NT, 0, st>>>(a3, c->flags);
     HIP_TRY(hipGetLastError());
     return GX_OK;
   });
   if (rc) return rc;
+  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], st));
+  // The step's surface integral into the boundary tally.
+  boundary_tally_finish<<<1, TALLY_BLOCK, 0, st>>>(c->bpartial, c->n_bpartial,
+                                                   c->tally_local);
+  HIP_TRY(hipGetLastError());
   if (c->profiling) {
     // The stage-1 interval ends after the invdt reset; the reset is a
-    // few-byte fill and is counted with stage 1.
-    HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-    HIP_TRY(hipEventSynchronize(c->ev[3]));
-    for (int k = 0; k < 3; k++) {
+    // few-byte fill and is counted with stage 1.  The stage intervals hold
+    // their boundary blocks.
+    HIP_TRY(hipEventRecord(c->ev[4], st));
+    HIP_TRY(hipEventSynchronize(c->ev[4]));
+    for (int k = 0; k < 4; k++) {
       float ms = 0.0f;
       HIP_TRY(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
       c->kernel_ms[k] += ms;
@@ -515,6 +573,57 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
 int gx_sync(gx_ctx *c) {
   if (!c) return fail(GX_E_ARG, "gx_sync: null context");
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_get_boundary_fluence(gx_ctx *c, double *fluence) {
+  if (!c || !fluence) return fail(GX_E_ARG, "gx_get_boundary_fluence: null argument");
+  const size_t n = (size_t)fluence_count(c->n);
+  HIP_TRY(hipMemcpyAsync(fluence, c->fluence, sizeof(double) * n, hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
+int gx_compute_tally(gx_ctx *c, int computeChange, double *interior, double *boundary,
+                     double *total, double *change) {
+  if (!c) return fail(GX_E_ARG, "gx_compute_tally: null context");
+  if (!c->uploaded) return fail(GX_E_STATE, "gx_compute_tally: no state uploaded");
+  by_precision(c->fp32, [&](auto tag) {
+    using R = decltype(tag);
+    StageArgs<R> a = base_args<R>(c);
+    a.aux = (const R *)c->U;
+    interior_tally_kernel<R><<<c->n_partial, TALLY_BLOCK, 0, c->stream>>>(
+        a, c->prims_valid ? (const R *)c->P0 : nullptr, c->tgeo, c->tpartial);
+    return 0;
+  });
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h((size_t)NI_SUM * c->n_partial);
+  double local[TALLY_N];
+  HIP_TRY(hipMemcpyAsync(h.data(), c->tpartial, sizeof(double) * h.size(),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(local, c->tally_local, sizeof(local), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipMemsetAsync(c->tally_local, 0, sizeof(local), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // Current_ASC % ComputeTallyTemplate (FLD/Current_ASC__Template.f90:239-349),
+  // one chart and one boundary, no reduction across ranks.
+  double in[TALLY_N] = {};
+  for (int k = 0; k < NI_SUM; k++) {
+    double s = 0.0;
+    for (int b = 0; b < c->n_partial; b++) s += h[(size_t)b * NI_SUM + k];
+    in[interior_slot(k)] = s;
+  }
+  for (int t = 0; t < TALLY_N; t++) {
+    const double old_total = c->tally_total[t];
+    c->tally_global[t] = c->tally_global[t] + local[t];
+    c->tally_total[t] = in[t] + c->tally_global[t];
+    if (computeChange) c->tally_change[t] = c->tally_change[t] + (c->tally_total[t] - old_total);
+  }
+  if (interior) memcpy(interior, in, sizeof(in));
+  if (boundary) memcpy(boundary, c->tally_global, sizeof(in));
+  if (total) memcpy(total, c->tally_total, sizeof(in));
+  if (change) memcpy(change, c->tally_change, sizeof(in));
   return GX_OK;
 }
 

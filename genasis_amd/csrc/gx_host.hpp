@@ -61,7 +61,8 @@ inline int check_grid(const gx_grid &g, const char *who) {
 // p = c + 2 for c in [-2, n+2): the same construction as the reference chart
 // (Chart_Template.f90:463-593), including single-precision ghost widths
 // (real(KDL) at :477-478).
-inline void geometry_1d(int n, double lo, double hi, std::vector<double> out[N_GEO]) {
+inline void geometry_1d(int n, double lo, double hi, std::vector<double> out[N_GEO],
+                        std::vector<double> *center = nullptr) {
   const int g = 2;
   std::vector<double> e(n + 2 * g + 1);
   auto E = [&](int i) -> double & { return e[i + g - 1]; };
@@ -83,6 +84,7 @@ inline void geometry_1d(int n, double lo, double hi, std::vector<double> out[N_G
     out[G_WL][p] = c[p] - E(ic);
     out[G_WR][p] = E(ic + 1) - c[p];
   }
+  if (center) *center = c;
   for (int p = 0; p < m; p++) {
     if (p >= 1 && p + 1 < m) {
       const double dxl = c[p] - c[p - 1];
@@ -108,6 +110,7 @@ struct ChartGeometry {
   int n[3] = {0, 0, 0};
   int gs = 0;
   std::vector<double> a[3][N_GEO];
+  std::vector<double> center[3];  // Center (Chart_Template.f90:827-841), padded
 };
 
 // Builds the geometry and checks the divisors of the exactly rounded
@@ -117,7 +120,8 @@ inline int make_geometry(const gx_grid &grid, ChartGeometry &cg, const char *who
   cg.gs = 0;
   for (int d = 0; d < 3; d++) {
     cg.n[d] = grid.nCells[d];
-    geometry_1d(cg.n[d], grid.MinCoordinate[d], grid.MaxCoordinate[d], cg.a[d]);
+    geometry_1d(cg.n[d], grid.MinCoordinate[d], grid.MaxCoordinate[d], cg.a[d],
+                &cg.center[d]);
     cg.gs = cg.n[d] + 4 > cg.gs ? cg.n[d] + 4 : cg.gs;
   }
   const double lo = 0x1p-500, hi = 0x1p122;

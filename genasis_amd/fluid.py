@@ -134,6 +134,31 @@ class Step_RK2_C_ASC:
         _capi.check(self.lib.gx_totals(self._h, _pd(out)), "gx_totals")
         return out
 
+    def boundary_fluence(self):
+        """Fluence of the last step through the six faces: a list over
+        b = 2*d (inner) and 2*d + 1 (outer) of (5,) + slab arrays (D, S_1,
+        S_2, S_3, G; the slab is the chart's numpy shape without dimension d)."""
+        n = self.chart.n
+        slabs = [n[1] * n[2], n[0] * n[2], n[0] * n[1]]
+        out = np.empty(10 * sum(slabs))
+        _capi.check(self.lib.gx_get_boundary_fluence(self._h, _pd(out)),
+                    "gx_get_boundary_fluence")
+        faces, o = [], 0
+        for b in range(6):
+            d = b // 2
+            shape = tuple(n[e] for e in (2, 1, 0) if e != d)
+            faces.append(out[o:o + 5 * slabs[d]].reshape((5,) + shape))
+            o += 5 * slabs[d]
+        return faces
+
+    def compute_tally(self, compute_change: bool = True):
+        """Current_ASC % ComputeTally: interior, boundary, total and change
+        tallies, GX_TALLY_N values each (Tally_F_P % Value order)."""
+        outs = [np.zeros(_capi.GX_TALLY_N) for _ in range(4)]
+        _capi.check(self.lib.gx_compute_tally(self._h, int(bool(compute_change)),
+                                              *[_pd(o) for o in outs]), "gx_compute_tally")
+        return dict(zip(("interior", "boundary", "total", "change"), outs))
+
     def upload(self, value: np.ndarray, field_map=None):
         """Reference layout Value(nValues, nFields) as a (nFields, nValues)
         C-contiguous array (column f is value[f])."""
@@ -172,11 +197,12 @@ class Step_RK2_C_ASC:
         _capi.check(self.lib.gx_set_profiling(self._h, int(bool(on))), "gx_set_profiling")
 
     def kernel_times(self):
-        """{name: (total_ms, launches)} for stage 1, stage 2 and the shock flags."""
+        """{name: (total_ms, steps)} for stage 1, stage 2, the shock flags and
+        the boundary fluence with the boundary tally."""
         ms = (ctypes.c_double * 4)()
         n = (ctypes.c_longlong * 4)()
         _capi.check(self.lib.gx_kernel_times(self._h, ms, n), "gx_kernel_times")
-        names = ("stage1", "stage2", "shock_flags")
+        names = ("stage1", "stage2", "shock_flags", "boundary_tally")
         return {k: (ms[i], n[i]) for i, k in enumerate(names)}
 
     @property

@@ -38,7 +38,12 @@ module GX_Fluid_Binding
   public :: &
     gx_default_params, gx_create, gx_destroy, gx_upload, gx_download, &
     gx_compute_time_step, gx_step_rk2, gx_sync, gx_totals, gx_last_error, &
-    LastErrorString
+    gx_compute_tally, gx_get_boundary_fluence, LastErrorString
+
+  !-- Tally_F_P % Value (12): BaryonNumber, Momentum_1..3, KineticEnergy,
+  !   AngularMomentum_1..3, GravitationalEnergy, TotalEnergy, FluidEnergy,
+  !   InternalEnergy
+  integer ( c_int ), parameter, public :: GX_TALLY_N = 12
 
   interface
 
@@ -105,6 +110,28 @@ module GX_Fluid_Binding
       type ( c_ptr ), value :: ctx
       real ( c_double ), intent ( out ) :: Total ( 5 )
     end function gx_totals
+
+    !-- Current_ASC % ComputeTally for the chart: Interior, Boundary, Total
+    !   and Change, GX_TALLY_N values each
+    integer ( c_int ) function gx_compute_tally &
+                        ( ctx, ComputeChange, Interior, Boundary, Total, Change ) &
+                        bind ( C, name = 'gx_compute_tally' )
+      import :: c_int, c_ptr, c_double, GX_TALLY_N
+      type ( c_ptr ), value :: ctx
+      integer ( c_int ), value :: ComputeChange
+      real ( c_double ), intent ( out ) :: &
+        Interior ( GX_TALLY_N ), Boundary ( GX_TALLY_N ), &
+        Total ( GX_TALLY_N ), Change ( GX_TALLY_N )
+    end function gx_compute_tally
+
+    !-- BoundaryFluence_CSL of the last step: faces 1..6 (inner and outer of
+    !   each dimension) x D, S_1, S_2, S_3, G x the face's slab of proper cells
+    integer ( c_int ) function gx_get_boundary_fluence ( ctx, Fluence ) &
+                        bind ( C, name = 'gx_get_boundary_fluence' )
+      import :: c_int, c_ptr, c_double
+      type ( c_ptr ), value :: ctx
+      real ( c_double ), intent ( out ) :: Fluence ( * )
+    end function gx_get_boundary_fluence
 
     type ( c_ptr ) function gx_last_error ( ) bind ( C, name = 'gx_last_error' )
       import :: c_ptr

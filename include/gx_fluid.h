@@ -28,6 +28,16 @@
  *   gx_totals          VolumeIntegral of the conserved fields (Modules/
  *                      Mathematics/Operations/Calculus/Integrals/
  *                      VolumeIntegral_Form.f90:63), one rank
+ *   gx_get_boundary_fluence  Step_RK_C_ASC % BoundaryFluence_CSL after a step
+ *                      (…/Evolutions/Steps/Step_RK_C_ASC__Template.f90:
+ *                      526-537, 1370-1422; …/Increments/IncrementDivergence_
+ *                      FV__Form.f90:1002-1069)
+ *   gx_compute_tally   Current_ASC % ComputeTally with the Tally_F_P tallies
+ *                      (Modules/Mathematics/Solvers/DifferentialEquations/
+ *                      Fields/Current_ASC__Template.f90:222-349, Tally_C__
+ *                      Form.f90:170-279; Modules/Physics/StressEnergies/
+ *                      Fluids/Tally_F_P__Form.f90, Tally_F_D__Form.f90), one
+ *                      chart, one rank
  */
 #ifndef GX_FLUID_H
 #define GX_FLUID_H
@@ -154,6 +164,49 @@ int gx_sync(gx_ctx *ctx);
  * FP64.  Synchronises. */
 int gx_totals(gx_ctx *ctx, double out[5]);
 
+/* Conservation tallies (Tally_F_P % Value, Tally_F_D__Form.f90:83-88,
+ * Tally_F_P__Form.f90:63-64), 0-based.  Flat geometry selects all but the
+ * gravitational and total energies, which stay 0. */
+enum {
+  GX_TALLY_BARYON_NUMBER = 0,
+  GX_TALLY_MOMENTUM_1,
+  GX_TALLY_MOMENTUM_2,
+  GX_TALLY_MOMENTUM_3,
+  GX_TALLY_KINETIC_ENERGY,
+  GX_TALLY_ANGULAR_MOMENTUM_1,
+  GX_TALLY_ANGULAR_MOMENTUM_2,
+  GX_TALLY_ANGULAR_MOMENTUM_3,
+  GX_TALLY_GRAVITATIONAL_ENERGY,
+  GX_TALLY_TOTAL_ENERGY,
+  GX_TALLY_FLUID_ENERGY,
+  GX_TALLY_INTERNAL_ENERGY,
+  GX_TALLY_N
+};
+
+/* Boundary fluence of the last gx_step_rk2: sum over its two stages of
+ * (1/2) dt times the flux through each boundary face of the chart, for D,
+ * S_1, S_2, S_3, G (the reference's conserved entropy DS is recomputed from
+ * the primitives and has no tally; its fluence is not kept).  Layout: face
+ * b = 2*d (inner) or 2*d+1 (outer) of dimension d, then field, then the slab
+ * of transverse proper cells with x fastest (d = 0: y + n1*z; d = 1:
+ * x + n0*z; d = 2: x + n0*y); faces in the order b = 0..5, so face b starts
+ * at 10 * sum_{e < d} slab_e + 5 * (b & 1) * slab_d.  fluence holds
+ * 10 * (n1*n2 + n0*n2 + n0*n1) doubles.  Zero before the first step.
+ * Synchronises. */
+int gx_get_boundary_fluence(gx_ctx *ctx, double *fluence);
+
+/* ComputeTallyTemplate: interior = VolumeIntegral of the tally integrands of
+ * the Current (D, S, 0.5 S.V, X x S, G, E); the boundary tally accumulated
+ * by every step since the upload (the SurfaceIntegral of the boundary
+ * integrands over each step's fluence, outward positive) is added to the
+ * global boundary tally; total = interior + boundary; change += total -
+ * previous total when computeChange.  An upload starts all tallies at zero,
+ * so the first call (computeChange = 0) sets total = interior.  Sums are in
+ * FP64 with a fixed order.  Any output may be NULL (GX_TALLY_N values each).
+ * Steps taken by gx_adjoint_forward are not tallied.  Synchronises. */
+int gx_compute_tally(gx_ctx *ctx, int computeChange, double *interior,
+                     double *boundary, double *total, double *change);
+
 /* Proper-cell views without the padded layout: five contiguous fields of
  * nCells[0]*nCells[1]*nCells[2] doubles each, x fastest. */
 int gx_set_state(gx_ctx *ctx, const double *primitive, const double *conserved);
@@ -175,8 +228,8 @@ void *gx_stream(gx_ctx *ctx);
 
 /* Per-kernel timing with HIP events on the context's stream: when on, each
  * gx_step_rk2 waits for its own kernels and adds their durations to
- * ms[0..2] (stage 1, stage 2, shock flags; ms[3] is unused and stays 0);
- * count[k] launches.
+ * ms[0..3] (stage 1 and stage 2, each with its boundary-fluence blocks,
+ * shock flags, boundary tally); count[k] steps.
  * Switching it on (or off) clears the sums. */
 int gx_set_profiling(gx_ctx *ctx, int on);
 int gx_kernel_times(gx_ctx *ctx, double ms[4], long long count[4]);

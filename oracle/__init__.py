@@ -23,6 +23,9 @@ FIELD = {
     "SB": 19, "DS": 20, "CS": 21, "MN": 22,
 }
 CONSERVED = ("D", "S_1", "S_2", "S_3", "G")
+TALLY_N = 12  # Tally_F_P % Value: BaryonNumber, Momentum_1..3, KineticEnergy,
+# AngularMomentum_1..3, GravitationalEnergy, TotalEnergy, FluidEnergy,
+# InternalEnergy
 PRIMITIVE = ("N", "V_1", "V_2", "V_3", "E")
 
 _D = ctypes.c_double
@@ -58,6 +61,12 @@ def _load():
         lib.gxo_limited_slope.argtypes = [_D, _D, _D, _D, _D]
         lib.gxo_face_flux.argtypes = [_PD, _PD, ctypes.c_int, _D, ctypes.c_int,
                                       _D, _PD]
+        lib.gxo_boundary_fluence.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_int, _PD]
+        lib.gxo_compute_tally.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_PD] * 4
+        lib.gxo_set_boundary_fluence.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_int, _PD]
+        lib.gxo_surface_integral.argtypes = [ctypes.c_void_p, _PD]
         _lib = lib
     return _lib
 
@@ -153,3 +162,30 @@ class OracleFluid:
 
     def primitive(self):
         return [self.field(f) for f in PRIMITIVE]
+
+    def boundary_fluence(self, b) -> np.ndarray:
+        """Fluence of the last step through face b = 2*d (inner) or 2*d + 1
+        (outer): D, S_1, S_2, S_3, G over the transverse proper cells, shape
+        (5,) + the chart's extents without dimension d (numpy order)."""
+        d = b // 2
+        shape = tuple(self.chart.n[e] for e in (2, 1, 0) if e != d)
+        out = np.empty((5,) + shape)
+        for f in range(5):
+            self._lib.gxo_boundary_fluence(self._h, b, f, _ptr(out[f]))
+        return out
+
+    def compute_tally(self, compute_change=True):
+        """Interior, boundary, total and change tallies (12 values each)."""
+        outs = [np.zeros(TALLY_N) for _ in range(4)]
+        self._lib.gxo_compute_tally(self._h, int(compute_change), *[_ptr(o) for o in outs])
+        return dict(zip(("interior", "boundary", "total", "change"), outs))
+
+    def set_boundary_fluence(self, b, fluence):
+        for f in range(5):
+            a = np.ascontiguousarray(fluence[f], dtype=np.float64)
+            self._lib.gxo_set_boundary_fluence(self._h, b, f, _ptr(a))
+
+    def surface_integral(self) -> np.ndarray:
+        out = np.zeros(TALLY_N)
+        self._lib.gxo_surface_integral(self._h, _ptr(out))
+        return out

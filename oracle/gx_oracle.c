@@ -57,6 +57,16 @@ struct gxo_ctx {
   double *fil[6], *fir[6], *fi[6];
   double *ap, *am, *ac;
   double *dxi, *dvi[5], *grad[5], *xc;
+  /* BoundaryFluence_CSL of the step: face 2*d (inner) and 2*d+1 (outer) of
+   * dimension d, conserved fields D, S_1, S_2, S_3, G; slab of the transverse
+   * proper cells, x fastest (size bf_n[d]). */
+  double *bf[6][5];
+  long bf_n[3];
+  double weight_rk; /* Weight_RK of the stage being computed */
+  /* Tally_F_P values (12, Tally_F_D__Form.f90:83-88, Tally_F_P__Form.f90:
+   * 63-64): TallyBoundaryLocal, TallyBoundaryGlobal, TallyTotal, TallyChange */
+  double tally_local[GXO_TALLY_N], tally_global[GXO_TALLY_N];
+  double tally_total[GXO_TALLY_N], tally_change[GXO_TALLY_N];
 };
 
 static inline long IDX(const gxo_ctx *c, int p0, int p1, int p2) {
@@ -161,6 +171,13 @@ gxo_ctx *gxo_create(const int n[3], const double lo[3], const double hi[3],
     c->dvi[f] = alloc0(c->nv);
     c->grad[f] = alloc0(c->nv);
   }
+  for (int d = 0; d < 3; d++) {
+    c->bf_n[d] = 1;
+    for (int e = 0; e < 3; e++)
+      if (e != d) c->bf_n[d] *= c->n[e];
+    for (int s = 0; s < 2; s++)
+      for (int f = 0; f < 5; f++) c->bf[2 * d + s][f] = alloc0(c->bf_n[d]);
+  }
   return c;
 }
 
@@ -180,6 +197,8 @@ void gxo_destroy(gxo_ctx *c) {
   }
   free(c->ap); free(c->am); free(c->ac); free(c->dxi); free(c->xc);
   for (int f = 0; f < 5; f++) { free(c->dvi[f]); free(c->grad[f]); }
+  for (int b = 0; b < 6; b++)
+    for (int f = 0; f < 5; f++) free(c->bf[b][f]);
   free(c);
 }
 
@@ -595,6 +614,30 @@ static void increment(gxo_ctx *c, double **K, double dt, int d) {
   }
 }
 
+/* RecordBoundaryFluence_CSL (EV/Increments/IncrementDivergence_FV__Form.f90:
+ * 1002-1069) with its kernel (IncrementDivergence_FV__Kernel.f90:755-800):
+ * BF = BF + (Weight_RK*dT) * F_I over the transverse proper cells, F_I taken
+ * on the inner face of the first proper layer (inner boundary) and of the
+ * first outer ghost layer (outer boundary) of dimension d. */
+static void record_fluence(gxo_ctx *c, double dt, int d) {
+  const double factor = c->weight_rk * dt;
+  int nb[3];
+  for (int e = 0; e < 3; e++) nb[e] = (e == d) ? 1 : c->n[e];
+  for (int side = 0; side < 2; side++) {
+    int ob[3] = {NG, NG, NG};
+    if (side) ob[d] += c->n[d];
+    for (int f = 0; f < 5; f++) {
+      double *B = c->bf[2 * d + side][f];
+      const double *F = c->fi[f];
+      long i = 0;
+      for (int k = 0; k < nb[2]; k++)
+        for (int j = 0; j < nb[1]; j++)
+          for (int l = 0; l < nb[0]; l++, i++)
+            B[i] = B[i] + factor * F[IDX(c, ob[0] + l, ob[1] + j, ob[2] + k)];
+    }
+  }
+}
+
 /* IncrementDivergence_FV_Form % Compute (EV/Increments/IncrementDivergence_FV__
  * Form.f90:344-412, 582-686, 689-738, 934-999). */
 static void increment_divergence(gxo_ctx *c, double **K, double dt) {
@@ -611,7 +654,98 @@ static void increment_divergence(gxo_ctx *c, double **K, double dt) {
     }
     fluxes(c, d);
     increment(c, K, dt, d);
+    record_fluence(c, dt, d);
   }
+}
+
+/* The Tally_F_P integrands of one value, in Value order (GXO_TALLY_*).
+ * Interior (Tally_F_D__Form.f90:248-384, 737-782; Tally_F_P__Form.f90:
+ * 147-187): D, S, KE = 0.5 (S.V), L = X x S, G, E; boundary (Tally_F_D__Form.
+ * f90:387-480, 1007-1034; Tally_F_P__Form.f90:190-239): the fluences of D, S
+ * and G with L = X x S from the momentum fluences; KE and E have no boundary
+ * integrand.  ComputeDensity_AM_Rectangular (X_J, X_K, S_J, S_K) is
+ * X_J*S_K - X_K*S_J with (J, K) = (2, 3), (3, 1), (1, 2). */
+static void tally_integrands(const double X[3], double D, const double S[3],
+                             double G, const double *V, double E,
+                             double out[GXO_TALLY_N]) {
+  for (int t = 0; t < GXO_TALLY_N; t++) out[t] = 0.0;
+  out[GXO_TALLY_BARYON_NUMBER] = D;
+  if (V)
+    out[GXO_TALLY_KINETIC_ENERGY] =
+        0.5 * ((S[0] * V[0] + S[1] * V[1]) + S[2] * V[2]);
+  for (int e = 0; e < 3; e++) out[GXO_TALLY_MOMENTUM_1 + e] = S[e];
+  out[GXO_TALLY_ANGULAR_MOMENTUM_1] = X[1] * S[2] - X[2] * S[1];
+  out[GXO_TALLY_ANGULAR_MOMENTUM_2] = X[2] * S[0] - X[0] * S[2];
+  out[GXO_TALLY_ANGULAR_MOMENTUM_3] = X[0] * S[1] - X[1] * S[0];
+  out[GXO_TALLY_FLUID_ENERGY] = G;
+  if (V) out[GXO_TALLY_INTERNAL_ENERGY] = E;
+}
+
+/* SurfaceIntegral_Form % Compute_CSL (OP/Calculus/Integrals/SurfaceIntegral_
+ * Form.f90:25-146, 150-177) of the boundary integrands over the step's
+ * fluence: I = I + Direction * dIdA * dA, Direction -1 on inner faces and +1
+ * on outer ones, dA the inner-face area of the slab's cell (GeometryFlat_
+ * Kernel.f90:12-47) and X the face positions of Tally_C_Form %
+ * ComputeFacePositions (transverse centres; Center - WidthLeft along d). */
+static void surface_integral(const gxo_ctx *c, double out[GXO_TALLY_N]) {
+  for (int t = 0; t < GXO_TALLY_N; t++) out[t] = 0.0;
+  for (int d = 0; d < 3; d++) {
+    int nb[3];
+    for (int e = 0; e < 3; e++) nb[e] = (e == d) ? 1 : c->n[e];
+    for (int side = 0; side < 2; side++) {
+      const double dir = side ? 1.0 : -1.0;
+      int ob[3] = {NG, NG, NG};
+      if (side) ob[d] += c->n[d];
+      double *const *B = c->bf[2 * d + side];
+      long i = 0;
+      for (int k = 0; k < nb[2]; k++)
+        for (int j = 0; j < nb[1]; j++)
+          for (int l = 0; l < nb[0]; l++, i++) {
+            const int p[3] = {ob[0] + l, ob[1] + j, ob[2] + k};
+            double X[3], w[3], v[GXO_TALLY_N];
+            for (int e = 0; e < 3; e++) {
+              X[e] = c->center[e][p[e]];
+              w[e] = c->wl[e][p[e]] + c->wr[e][p[e]];
+            }
+            X[d] = X[d] - c->wl[d][p[d]];
+            const double dA = d == 0 ? w[1] * w[2] : d == 1 ? w[2] * w[0] : w[0] * w[1];
+            const double S[3] = {B[1][i], B[2][i], B[3][i]};
+            tally_integrands(X, B[0][i], S, B[4][i], NULL, 0.0, v);
+            for (int t = 0; t < GXO_TALLY_N; t++) out[t] = out[t] + dir * v[t] * dA;
+          }
+    }
+  }
+}
+
+/* VolumeIntegral_Form % Compute_CSL (OP/Calculus/Integrals/VolumeIntegral_
+ * Form.f90:25-117) of the interior integrands over the proper cells of the
+ * Current: I = I + dIdV * dV, dV = (dX*dY)*dZ. */
+static void volume_integral(const gxo_ctx *c, double out[GXO_TALLY_N]) {
+  for (int t = 0; t < GXO_TALLY_N; t++) out[t] = 0.0;
+  double *const *F = c->cur;
+  for (int p2 = NG; p2 < c->m[2] - NG; p2++)
+    for (int p1 = NG; p1 < c->m[1] - NG; p1++)
+      for (int p0 = NG; p0 < c->m[0] - NG; p0++) {
+        const long iv = IDX(c, p0, p1, p2);
+        const int p[3] = {p0, p1, p2};
+        double X[3], w[3], v[GXO_TALLY_N];
+        for (int e = 0; e < 3; e++) {
+          X[e] = c->center[e][p[e]];
+          w[e] = c->wl[e][p[e]] + c->wr[e][p[e]];
+        }
+        const double dV = w[0] * w[1] * w[2];
+        const double S[3] = {F[GXO_S_1][iv], F[GXO_S_2][iv], F[GXO_S_3][iv]};
+        const double V[3] = {F[GXO_V_1][iv], F[GXO_V_2][iv], F[GXO_V_3][iv]};
+        tally_integrands(X, F[GXO_D][iv], S, F[GXO_G][iv], V, F[GXO_E][iv], v);
+        for (int t = 0; t < GXO_TALLY_N; t++) out[t] = out[t] + v[t] * dV;
+      }
+}
+
+static void reset_tallies(gxo_ctx *c) {
+  for (int t = 0; t < GXO_TALLY_N; t++)
+    c->tally_local[t] = c->tally_global[t] = c->tally_total[t] = c->tally_change[t] = 0.0;
+  for (int b = 0; b < 6; b++)
+    for (int f = 0; f < 5; f++) memset(c->bf[b][f], 0, sizeof(double) * c->bf_n[b / 2]);
 }
 
 static void clear6(const gxo_ctx *c, double **A) {
@@ -716,6 +850,7 @@ void gxo_set_primitive(gxo_ctx *c, const double *N, const double *V1,
   exchange_ghosts(c, prim, 5);
   from_primitive_all(c, c->cur);
   for (int d = 0; d < 3; d++) memset(c->df[d], 0, b);
+  reset_tallies(c);
 }
 
 void gxo_set_conserved(gxo_ctx *c, const double *D, const double *S1,
@@ -736,7 +871,7 @@ double gxo_time_step_candidate(gxo_ctx *c, double *invdt_out) {
   double tsi = -DBL_MAX;
   int lo[3], hi[3];
   for (int e = 0; e < 3; e++) { lo[e] = NG; hi[e] = c->m[e] - NG; }
-  double **F = c->cur;
+  double *const *F = c->cur;
 #pragma omp parallel for collapse(3) schedule(static) reduction(max : tsi)
   for (int p2 = lo[2]; p2 < hi[2]; p2++)
     for (int p1 = lo[1]; p1 < hi[1]; p1++)
@@ -761,11 +896,15 @@ double gxo_time_step_candidate(gxo_ctx *c, double *invdt_out) {
  * 692-816, 833-914, 1112-1281) with the Heun tableau of Step_RK2_C_ASC
  * (EV/Steps/Step_RK2_C_ASC__Form.f90:51-57): A21 = 1, B = (1/2, 1/2). */
 void gxo_step(gxo_ctx *c, double dt) {
+  /* ClearBoundaryFluence (EV/Steps/Step_RK_C_ASC__Template.f90:526-529) */
+  for (int b = 0; b < 6; b++)
+    for (int f = 0; f < 5; f++) memset(c->bf[b][f], 0, sizeof(double) * c->bf_n[b / 2]);
   /* LoadSolution */
   copy_conserved_to(c, c->sol);
   /* Stage 1: Y = Solution; K1 = L(Current)*dt; exchange K1 */
   for (int f = 0; f < 6; f++) memcpy(c->y[f], c->sol[f], sizeof(double) * c->nv);
   clear6(c, c->k1);
+  c->weight_rk = 0.5; /* B(1) */
   increment_divergence(c, c->k1, dt);
   exchange_ghosts(c, c->k1, 6);
   /* Stage 2: Y = Solution + 1*K1; Current <- Y; K2 = L(Current)*dt */
@@ -774,6 +913,7 @@ void gxo_step(gxo_ctx *c, double dt) {
   copy_conserved_from(c, c->y);
   from_conserved_all(c, c->cur);
   clear6(c, c->k2);
+  c->weight_rk = 0.5; /* B(2) */
   increment_divergence(c, c->k2, dt);
   exchange_ghosts(c, c->k2, 6);
   /* IncrementSolution, K1 first */
@@ -783,6 +923,43 @@ void gxo_step(gxo_ctx *c, double dt) {
   copy_conserved_from(c, c->sol);
   from_conserved_all(c, c->cur);
   detect_features(c);
+  /* AccumulateBoundaryTally (Step_RK_C_ASC__Template.f90:533-537;
+   * FLD/Current_ASC__Template.f90:203-219; FLD/Tally_C__Form.f90:215-279):
+   * TallyBoundaryLocal = TallyBoundaryLocal + SurfaceIntegral */
+  double si[GXO_TALLY_N];
+  surface_integral(c, si);
+  for (int t = 0; t < GXO_TALLY_N; t++) c->tally_local[t] = c->tally_local[t] + si[t];
+}
+
+void gxo_boundary_fluence(const gxo_ctx *c, int b, int f, double *out) {
+  memcpy(out, c->bf[b][f], sizeof(double) * c->bf_n[b / 2]);
+}
+
+void gxo_set_boundary_fluence(gxo_ctx *c, int b, int f, const double *in) {
+  memcpy(c->bf[b][f], in, sizeof(double) * c->bf_n[b / 2]);
+}
+
+void gxo_surface_integral(const gxo_ctx *c, double *out) { surface_integral(c, out); }
+
+/* Current_ASC % ComputeTallyTemplate (FLD/Current_ASC__Template.f90:239-349),
+ * one chart, one boundary (AtlasHeader_Form.f90:164), no reduction. */
+void gxo_compute_tally(gxo_ctx *c, int compute_change, double *interior,
+                       double *boundary, double *total, double *change) {
+  double old_total[GXO_TALLY_N], in[GXO_TALLY_N];
+  memcpy(old_total, c->tally_total, sizeof(old_total));
+  volume_integral(c, in);
+  for (int t = 0; t < GXO_TALLY_N; t++) {
+    c->tally_global[t] = c->tally_global[t] + c->tally_local[t];
+    c->tally_local[t] = 0.0;
+    c->tally_total[t] = in[t];
+    c->tally_total[t] = c->tally_total[t] + c->tally_global[t];
+    if (compute_change)
+      c->tally_change[t] = c->tally_change[t] + (c->tally_total[t] - old_total[t]);
+  }
+  if (interior) memcpy(interior, in, sizeof(in));
+  if (boundary) memcpy(boundary, c->tally_global, sizeof(in));
+  if (total) memcpy(total, c->tally_total, sizeof(in));
+  if (change) memcpy(change, c->tally_change, sizeof(in));
 }
 
 void gxo_get_field(const gxo_ctx *c, int f, double *out) {

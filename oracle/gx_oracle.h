@@ -38,6 +38,20 @@ enum {
   GXO_N_FIELDS
 };
 
+/* Tally_F_P % Value, 0-based (Tally_F_D__Form.f90:83-88, Tally_F_P__Form.
+ * f90:63-64); gravitational and total energy are not selected in flat
+ * geometry (Tally_F_P__Form.f90:105-109) and stay 0. */
+enum {
+  GXO_TALLY_BARYON_NUMBER = 0,
+  GXO_TALLY_MOMENTUM_1, GXO_TALLY_MOMENTUM_2, GXO_TALLY_MOMENTUM_3,
+  GXO_TALLY_KINETIC_ENERGY,
+  GXO_TALLY_ANGULAR_MOMENTUM_1, GXO_TALLY_ANGULAR_MOMENTUM_2,
+  GXO_TALLY_ANGULAR_MOMENTUM_3,
+  GXO_TALLY_GRAVITATIONAL_ENERGY, GXO_TALLY_TOTAL_ENERGY,
+  GXO_TALLY_FLUID_ENERGY, GXO_TALLY_INTERNAL_ENERGY,
+  GXO_TALLY_N
+};
+
 typedef struct gxo_ctx gxo_ctx;
 
 /* nCells[3] proper cells, MinCoordinate/MaxCoordinate, bc[2*d+0] inner and
@@ -84,7 +98,24 @@ void gxo_get_flags(const gxo_ctx *c, int d, double *out);
 /* Overwrite DiffusiveFlux_I_d (d = 0,1,2), padded layout. */
 void gxo_set_flags(gxo_ctx *c, int d, const double *in);
 
+/* BoundaryFluence_CSL of the last step (EV/Increments/IncrementDivergence_FV__
+ * Form.f90:1002-1069): face b = 2*d (inner) or 2*d+1 (outer), field f of D,
+ * S_1, S_2, S_3, G; the slab of transverse proper cells, x fastest. */
+void gxo_boundary_fluence(const gxo_ctx *c, int b, int f, double *out);
+
+/* Current_ASC % ComputeTallyTemplate (FLD/Current_ASC__Template.f90:239-349)
+ * for one chart: interior = VolumeIntegral of the Tally_F_P integrands, the
+ * accumulated boundary tally, total = interior + boundary, change += total -
+ * old total when compute_change.  Any output may be NULL. */
+void gxo_compute_tally(gxo_ctx *c, int compute_change, double *interior,
+                       double *boundary, double *total, double *change);
+
 /* ---- single-kernel entry points used to pin the restatement ------------ */
+
+/* Overwrite one fluence slab, and the SurfaceIntegral of the boundary
+ * integrands over the fluence held (what a step adds to the boundary tally). */
+void gxo_set_boundary_fluence(gxo_ctx *c, int b, int f, const double *in);
+void gxo_surface_integral(const gxo_ctx *c, double *out);
 
 /* Gradient_Kernel limited slope for one stencil (OP/Calculus/Derivatives/
  * Gradient_Kernel.f90:166-190). */
