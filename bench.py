@@ -304,6 +304,19 @@ def copy_calibration():
                       "copy_calibration_bytes_written": b.numel() * 8}))
 
 
+def _tally_summary(t0, t1):
+    """Tally_F_P totals after the run and the relative change of the
+    conserved ones (baryon number, momentum, fluid energy): interior plus
+    boundary fluence, so round-off only."""
+    from genasis_amd._capi import TALLY_NAMES
+    conserved = ("BaryonNumber", "Momentum_1", "Momentum_2", "Momentum_3", "FluidEnergy")
+    scale = max(abs(t0["total"][TALLY_NAMES.index("FluidEnergy")]), 1e-300)
+    return {"total": dict(zip(TALLY_NAMES, map(float, t1["total"]))),
+            "boundary": dict(zip(TALLY_NAMES, map(float, t1["boundary"]))),
+            "max_conserved_change_over_fluid_energy":
+                max(abs(float(t1["change"][TALLY_NAMES.index(k)])) for k in conserved) / scale}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,6 +363,7 @@ def main():
     stepper.set_tuning(args.z_chunk)
     stepper.set_primitive(*problems.kelvin_helmholtz(chart, seed=1234, padded=False))
     total0 = stepper.totals()
+    tally0 = stepper.compute_tally(False)
     integ = Integrator_C_PS(stepper, IntegratorParameters(
         FinishTime=1.0e30, FinishCycle=args.warmup + args.steps, nWrite=1))
     integ._administer_checkpoint()
@@ -367,6 +381,7 @@ def main():
     kt = stepper.kernel_times()
     stepper.set_profiling(False)
     total1 = stepper.totals()
+    tally1 = stepper.compute_tally(True)
 
     elapsed = t1 - t0
     value = chart.n_proper * args.steps / elapsed
@@ -427,6 +442,7 @@ def main():
                           / 1e9 / HBM_PEAK_GBS},
         "kernel_ms_avg": {k: v[0] / max(v[1], 1) for k, v in kt.items()},
         "conserved_totals_change": [float(a - b) for a, b in zip(total1, total0)],
+        "tally": _tally_summary(tally0, tally1),
         "cpu_baseline": cpu,
         "fp32": fp32,
         "batch": batch,

@@ -52,10 +52,10 @@ struct gx_ctx {
   double tally_global[gx::TALLY_N] = {};  // TallyBoundaryGlobal
   double tally_total[gx::TALLY_N] = {};   // TallyTotal
   double tally_change[gx::TALLY_N] = {};  // TallyChange
-  // per-kernel timing (gx_set_profiling): stage 1, stage 2 (each with its
-  // boundary-fluence blocks), shock flags, boundary tally
+  // per-kernel timing (gx_set_profiling): stage 1, stage 2, shock flags,
+  // boundary fluence and tally
   bool profiling = false;
-  hipEvent_t ev[5] = {};
+  hipEvent_t ev[6] = {};
   double kernel_ms[4] = {0, 0, 0, 0};
   long long kernel_n[4] = {0, 0, 0, 0};
   ChartGeometry geom;

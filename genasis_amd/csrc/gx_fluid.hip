@@ -118,7 +118,7 @@ int gx_destroy(gx_ctx *c) {
   if (!c) return GX_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   gx::release_adjoint(c);
-  for (int e = 0; e < 5; e++)
+  for (int e = 0; e < 6; e++)
     if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
   (void)hipFree(c->fluence);
   (void)hipFree(c->tgeo);
@@ -230,7 +230,7 @@ void *gx_stream(gx_ctx *c) { return c ? (void *)c->stream : nullptr; }
 int gx_set_profiling(gx_ctx *c, int on) {
   if (!c) return fail(GX_E_ARG, "gx_set_profiling: null context");
   if (on && !c->ev[0])
-    for (int e = 0; e < 5; e++) HIP_TRY(hipEventCreate(&c->ev[e]));
+    for (int e = 0; e < 6; e++) HIP_TRY(hipEventCreate(&c->ev[e]));
   c->profiling = on != 0;
   for (int k = 0; k < 4; k++) {
     c->kernel_ms[k] = 0.0;
@@ -502,7 +502,8 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   B.tg = c->tgeo;
   B.partial = c->bpartial;
   hipStream_t st = c->stream;
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[0], st));
+  const bool prof = c->profiling;
+  if (prof) HIP_TRY(hipEventRecord(c->ev[0], st));
   const int rc = by_precision(c->fp32, [&](auto tag) {
     using R = decltype(tag);
     // Stage 1: Current = uploaded primitives or the Solution.
@@ -517,11 +518,15 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     a1.nby = nby;
     a1.nblk = nblk;
     stage_kernel<R, 1><<<nblk, block, 0, st>>>(a1);
+    HIP_TRY(hipGetLastError());
+    if (prof) HIP_TRY(hipEventRecord(c->ev[1], st));
+    // Stage-1 boundary fluxes from the stage-1 input (stage 2 overwrites the
+    // Solution).
     boundary_flux_kernel<R, 1><<<c->n_bpartial, TALLY_BLOCK, 0, st>>>(a1, B);
     HIP_TRY(hipGetLastError());
-    // Stage 2: Current = Y.
     HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), st));
-    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[1], st));
+    if (prof) HIP_TRY(hipEventRecord(c->ev[2], st));
+    // Stage 2: Current = Y.
     StageArgs<R> a2 = base_args<R>(c);
     a2.dt = dt;
     a2.src = (const R *)c->Y;
@@ -531,11 +536,17 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     a2.nby = nby;
     a2.nblk = nblk;
     stage_kernel<R, 2><<<nblk, block, 0, st>>>(a2);
+    HIP_TRY(hipGetLastError());
+    if (prof) HIP_TRY(hipEventRecord(c->ev[3], st));
+    // Stage-2 boundary fluxes (input Y, the step's flags) and the step's
+    // surface integral into the boundary tally.
     boundary_flux_kernel<R, 2><<<c->n_bpartial, TALLY_BLOCK, 0, st>>>(a2, B);
     HIP_TRY(hipGetLastError());
-    if (c->profiling) HIP_TRY(hipEventRecord(c->ev[2], st));
-    // Shock flags of the next step from the new state (after stage 2's
-    // boundary blocks have read the step's flags).
+    boundary_tally_finish<<<1, TALLY_BLOCK, 0, st>>>(c->bpartial, c->n_bpartial,
+                                                     c->tally_local);
+    HIP_TRY(hipGetLastError());
+    if (prof) HIP_TRY(hipEventRecord(c->ev[4], st));
+    // Shock flags of the next step from the new state.
     StageArgs<R> a3 = base_args<R>(c);
     a3.src = (const R *)c->U;
     a3.tz = FLAG_TZ;
@@ -547,23 +558,17 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     return GX_OK;
   });
   if (rc) return rc;
-  if (c->profiling) HIP_TRY(hipEventRecord(c->ev[3], st));
-  // The step's surface integral into the boundary tally.
-  boundary_tally_finish<<<1, TALLY_BLOCK, 0, st>>>(c->bpartial, c->n_bpartial,
-                                                   c->tally_local);
-  HIP_TRY(hipGetLastError());
-  if (c->profiling) {
-    // The stage-1 interval ends after the invdt reset; the reset is a
-    // few-byte fill and is counted with stage 1.  The stage intervals hold
-    // their boundary blocks.
-    HIP_TRY(hipEventRecord(c->ev[4], st));
-    HIP_TRY(hipEventSynchronize(c->ev[4]));
-    for (int k = 0; k < 4; k++) {
-      float ms = 0.0f;
-      HIP_TRY(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
-      c->kernel_ms[k] += ms;
-      c->kernel_n[k] += 1;
-    }
+  if (prof) {
+    // The boundary interval holds the invdt reset, a few-byte fill.
+    HIP_TRY(hipEventRecord(c->ev[5], st));
+    HIP_TRY(hipEventSynchronize(c->ev[5]));
+    float t[5];
+    for (int k = 0; k < 5; k++) HIP_TRY(hipEventElapsedTime(&t[k], c->ev[k], c->ev[k + 1]));
+    c->kernel_ms[0] += t[0];
+    c->kernel_ms[1] += t[2];
+    c->kernel_ms[2] += t[4];
+    c->kernel_ms[3] += t[1] + t[3];
+    for (int k = 0; k < 4; k++) c->kernel_n[k] += 1;
   }
   c->prims_valid = false;
   c->invdt_valid = true;

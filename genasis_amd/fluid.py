@@ -202,7 +202,7 @@ class Step_RK2_C_ASC:
         ms = (ctypes.c_double * 4)()
         n = (ctypes.c_longlong * 4)()
         _capi.check(self.lib.gx_kernel_times(self._h, ms, n), "gx_kernel_times")
-        names = ("stage1", "stage2", "shock_flags", "boundary_tally")
+        names = ("stage1", "stage2", "shock_flags", "boundary_fluence_tally")
         return {k: (ms[i], n[i]) for i, k in enumerate(names)}
 
     @property

@@ -228,8 +228,8 @@ void *gx_stream(gx_ctx *ctx);
 
 /* Per-kernel timing with HIP events on the context's stream: when on, each
  * gx_step_rk2 waits for its own kernels and adds their durations to
- * ms[0..3] (stage 1 and stage 2, each with its boundary-fluence blocks,
- * shock flags, boundary tally); count[k] steps.
+ * ms[0..3] (stage 1, stage 2, shock flags, and the boundary-fluence and
+ * boundary-tally kernels of both stages); count[k] steps.
  * Switching it on (or off) clears the sums. */
 int gx_set_profiling(gx_ctx *ctx, int on);
 int gx_kernel_times(gx_ctx *ctx, double ms[4], long long count[4]);
