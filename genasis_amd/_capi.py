@@ -27,6 +27,9 @@ FIELD_NAMES = (
     "EntropyPerBaryon", "ConservedEntropy", "SoundSpeed", "MachNumber",
 )
 
+GX_RESTART_PRIMITIVE = 0
+GX_RESTART_EXACT = 1
+
 # Conservation tallies (Tally_F_P % Value order, 0-based)
 TALLY_NAMES = (
     "BaryonNumber", "Momentum_1", "Momentum_2", "Momentum_3", "KineticEnergy",
@@ -44,6 +47,7 @@ EXPORTS = (
     "gx_set_tuning", "gx_stream", "gx_set_profiling", "gx_kernel_times",
     "gx_self_test_division", "gx_last_error",
     "gx_get_boundary_fluence", "gx_compute_tally",
+    "gx_write_checkpoint", "gx_read_checkpoint",
     "gx_batch_create", "gx_batch_destroy", "gx_batch_set_primitive",
     "gx_batch_set_integrator", "gx_batch_step", "gx_batch_sync", "gx_batch_cfl",
     "gx_batch_times", "gx_batch_get_conserved", "gx_batch_get_flags",
@@ -109,6 +113,9 @@ def load(path: str = LIB_PATH):
         "gx_totals": (i, [vp, pd]),
         "gx_get_boundary_fluence": (i, [vp, pd]),
         "gx_compute_tally": (i, [vp, i, pd, pd, pd, pd]),
+        "gx_write_checkpoint": (i, [vp, ctypes.c_char_p, d, ll, i]),
+        "gx_read_checkpoint": (i, [vp, ctypes.c_char_p, i, pd, P(ctypes.c_longlong),
+                                   P(ctypes.c_int)]),
         "gx_set_state": (i, [vp, pd, pd]),
         "gx_set_primitive": (i, [vp, pd]),
         "gx_get_conserved": (i, [vp, pd]),

@@ -11,7 +11,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRCS = [os.path.join(HERE, "csrc", f) for f in ("gx_fluid.hip", "gx_batch.hip",
-                                                        "gx_adjoint.hip")]
+                                                        "gx_adjoint.hip", "gx_checkpoint.hip")]
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libgx_fluid.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -151,6 +151,24 @@ class Step_RK2_C_ASC:
             o += 5 * slabs[d]
         return faces
 
+    def write_checkpoint(self, path: str, time: float, cycle: int, i_checkpoint: int = 0):
+        """Checkpoint file of the context with the integrator's Time, cycle and
+        iCheckpoint (gx_write_checkpoint)."""
+        _capi.check(self.lib.gx_write_checkpoint(self._h, str(path).encode(), float(time),
+                                                 int(cycle), int(i_checkpoint)),
+                    "gx_write_checkpoint")
+
+    def read_checkpoint(self, path: str, exact: bool = False):
+        """Restore from a checkpoint file: exact=False repeats the reference's
+        RestartFrom (primitives, conserved recomputed), exact=True restores the
+        whole device state.  Returns (Time, cycle, iCheckpoint)."""
+        t, cyc, ick = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_int()
+        mode = _capi.GX_RESTART_EXACT if exact else _capi.GX_RESTART_PRIMITIVE
+        _capi.check(self.lib.gx_read_checkpoint(self._h, str(path).encode(), mode,
+                                                ctypes.byref(t), ctypes.byref(cyc),
+                                                ctypes.byref(ick)), "gx_read_checkpoint")
+        return t.value, cyc.value, ick.value
+
     def compute_tally(self, compute_change: bool = True):
         """Current_ASC % ComputeTally: interior, boundary, total and change
         tallies, GX_TALLY_N values each (Tally_F_P % Value order)."""

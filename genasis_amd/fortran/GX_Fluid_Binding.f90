@@ -38,7 +38,12 @@ module GX_Fluid_Binding
   public :: &
     gx_default_params, gx_create, gx_destroy, gx_upload, gx_download, &
     gx_compute_time_step, gx_step_rk2, gx_sync, gx_totals, gx_last_error, &
-    gx_compute_tally, gx_get_boundary_fluence, LastErrorString
+    gx_compute_tally, gx_get_boundary_fluence, gx_write_checkpoint, &
+    gx_read_checkpoint, LastErrorString
+
+  !-- gx_read_checkpoint modes: the reference's RestartFrom, or exact
+  integer ( c_int ), parameter, public :: &
+    GX_RESTART_PRIMITIVE = 0, GX_RESTART_EXACT = 1
 
   !-- Tally_F_P % Value (12): BaryonNumber, Momentum_1..3, KineticEnergy,
   !   AngularMomentum_1..3, GravitationalEnergy, TotalEnergy, FluidEnergy,
@@ -132,6 +137,32 @@ module GX_Fluid_Binding
       type ( c_ptr ), value :: ctx
       real ( c_double ), intent ( out ) :: Fluence ( * )
     end function gx_get_boundary_fluence
+
+    !-- Checkpoint file at AdministerCheckpoint; Path is a NUL-terminated
+    !   character array (trim ( Path ) // c_null_char)
+    integer ( c_int ) function gx_write_checkpoint &
+                        ( ctx, Path, Time, iCycle, iCheckpoint ) &
+                        bind ( C, name = 'gx_write_checkpoint' )
+      import :: c_int, c_ptr, c_double, c_char, c_long_long
+      type ( c_ptr ), value :: ctx
+      character ( kind = c_char ), intent ( in ) :: Path ( * )
+      real ( c_double ), value :: Time
+      integer ( c_long_long ), value :: iCycle
+      integer ( c_int ), value :: iCheckpoint
+    end function gx_write_checkpoint
+
+    !-- RestartFrom: Time, cycle and iCheckpoint of the file
+    integer ( c_int ) function gx_read_checkpoint &
+                        ( ctx, Path, Mode, Time, iCycle, iCheckpoint ) &
+                        bind ( C, name = 'gx_read_checkpoint' )
+      import :: c_int, c_ptr, c_double, c_char, c_long_long
+      type ( c_ptr ), value :: ctx
+      character ( kind = c_char ), intent ( in ) :: Path ( * )
+      integer ( c_int ), value :: Mode
+      real ( c_double ), intent ( out ) :: Time
+      integer ( c_long_long ), intent ( out ) :: iCycle
+      integer ( c_int ), intent ( out ) :: iCheckpoint
+    end function gx_read_checkpoint
 
     type ( c_ptr ) function gx_last_error ( ) bind ( C, name = 'gx_last_error' )
       import :: c_ptr

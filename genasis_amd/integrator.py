@@ -69,6 +69,21 @@ class Integrator_C_PS:
                 s.CheckpointTimeExact = True
         s.iCheckpoint += 1
 
+    # Integrator_Template.f90:476-502 (PrepareInitial with RestartFrom) and
+    # 966-996 (ResetInitial): Time, iCycle and iCheckpoint from the file; the
+    # next evolve() administers the checkpoint that follows.
+    def restart_from(self, path: str, exact: bool = False):
+        t, cycle, i_ck = self.stepper.read_checkpoint(path, exact=exact)
+        self.s.Time = t
+        self.s.iCycle = cycle
+        self.s.iCheckpoint = i_ck
+        return self.s
+
+    def write_checkpoint(self, path: str):
+        """The stepper's checkpoint file with this integrator's Time, iCycle and
+        iCheckpoint (the reference writes at AdministerCheckpoint)."""
+        self.stepper.write_checkpoint(path, self.s.Time, self.s.iCycle, self.s.iCheckpoint)
+
     # Integrator_Template.f90:896-939
     def compute_time_step(self) -> float:
         candidate = self.stepper.time_step_candidate()

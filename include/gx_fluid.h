@@ -32,6 +32,10 @@
  *                      (…/Evolutions/Steps/Step_RK_C_ASC__Template.f90:
  *                      526-537, 1370-1422; …/Increments/IncrementDivergence_
  *                      FV__Form.f90:1002-1069)
+ *   gx_write_checkpoint / gx_read_checkpoint  Integrator_Template % Write at a
+ *                      checkpoint and the RestartFrom path of PrepareInitial
+ *                      (…/Evolutions/Integrators/Integrator_Template.f90:
+ *                      476-502, 966-996)
  *   gx_compute_tally   Current_ASC % ComputeTally with the Tally_F_P tallies
  *                      (Modules/Mathematics/Solvers/DifferentialEquations/
  *                      Fields/Current_ASC__Template.f90:222-349, Tally_C__
@@ -206,6 +210,27 @@ int gx_get_boundary_fluence(gx_ctx *ctx, double *fluence);
  * Steps taken by gx_adjoint_forward are not tallied.  Synchronises. */
 int gx_compute_tally(gx_ctx *ctx, int computeChange, double *interior,
                      double *boundary, double *total, double *change);
+
+/* Checkpoint files.  gx_write_checkpoint stores the host integrator's Time,
+ * cycle and iCheckpoint with the Current's primitives (FP64, proper cells)
+ * and the device state (conserved and uploaded fields in the context's
+ * precision, shock flags, CFL maximum, boundary fluence, tallies); written to
+ * "<path>.tmp", then renamed.  gx_read_checkpoint checks that the file's
+ * chart and parameters are the context's and restores:
+ *   GX_RESTART_PRIMITIVE  the reference's RestartFrom (Integrator_Template.
+ *                         f90:476-502, 966-996; PrepareEvolution): the
+ *                         primitives, with the conserved fields recomputed
+ *                         from them (as gx_set_primitive), flags and tallies
+ *                         cleared;
+ *   GX_RESTART_EXACT      the whole device state: the run continues bit for
+ *                         bit as if it had not stopped (same precision).
+ * time, cycle, iCheckpoint receive the stored values (each may be NULL). */
+#define GX_RESTART_PRIMITIVE 0
+#define GX_RESTART_EXACT 1
+int gx_write_checkpoint(gx_ctx *ctx, const char *path, double time, long long cycle,
+                        int iCheckpoint);
+int gx_read_checkpoint(gx_ctx *ctx, const char *path, int mode, double *time,
+                       long long *cycle, int *iCheckpoint);
 
 /* Proper-cell views without the padded layout: five contiguous fields of
  * nCells[0]*nCells[1]*nCells[2] doubles each, x fastest. */

@@ -1,0 +1,120 @@
+"""Checkpoint files and restart (Integrator_Template's RestartFrom path,
+Modules/Mathematics/Solvers/DifferentialEquations/Evolutions/Integrators/
+Integrator_Template.f90:476-502, 966-996): an exact restart continues bit for
+bit (state, flags, time steps, fluence, tallies); a primitive restart is the
+reference's (conserved fields recomputed from the stored primitives), equal
+to the oracle restarted from the same primitives; bad files are refused."""
+import os
+
+import numpy as np
+import pytest
+
+from genasis_amd._capi import GxError
+from genasis_amd.chart import Chart
+from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
+from genasis_amd.integrator import Integrator_C_PS, IntegratorParameters
+
+from ._helpers import make_oracle, oracle_conserved
+
+pytestmark = pytest.mark.gpu
+
+BC = (("OUTFLOW", "REFLECTING"), ("PERIODIC", "PERIODIC"), ("REFLECTING", "OUTFLOW"))
+
+
+def _state(chart, seed):
+    rng = np.random.default_rng(seed)
+    shape = chart.n[::-1]
+    x = np.broadcast_to(chart.center[0][2:-2], shape)
+    N = np.exp(rng.uniform(np.log(0.3), np.log(3.0), shape)) * (1 + 2 * (x > 0.5))
+    V = [rng.uniform(-0.5, 0.5, shape) for _ in range(3)]
+    E = np.exp(rng.uniform(np.log(0.3), np.log(3.0), shape)) / 0.4 * (1 + 5 * (x > 0.5))
+    return (N, V[0], V[1], V[2], E)
+
+
+def _run(precision="FP64"):
+    chart = Chart((24, 20, 16), bc=BC)
+    params = FluidParameters(CourantFactor=0.8, Precision=precision)
+    g = Step_RK2_C_ASC(chart, params)
+    g.set_primitive(*_state(chart, 5))
+    it = Integrator_C_PS(g, IntegratorParameters(FinishTime=1e30, nRampCycles=4))
+    it._administer_checkpoint()  # the start of Evolve: iCheckpoint 0 -> 1
+    return chart, params, g, it
+
+
+def _continue(it, g, n):
+    for _ in range(n):
+        it.compute_cycle()
+    return g.conserved(), g.flags(), it.s.Time, g.compute_tally(), g.boundary_fluence()
+
+
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_exact_restart_continues_bit_for_bit(tmp_path, precision):
+    chart, params, g, it = _run(precision)
+    _continue(it, g, 7)
+    g.compute_tally(False)
+    path = str(tmp_path / "ck_0001.gx")
+    it.write_checkpoint(path)
+    assert not os.path.exists(path + ".tmp")
+    a = _continue(it, g, 6)
+    g2 = Step_RK2_C_ASC(chart, params)
+    it2 = Integrator_C_PS(g2, IntegratorParameters(FinishTime=1e30, nRampCycles=4))
+    s = it2.restart_from(path, exact=True)
+    assert (s.iCycle, s.iCheckpoint) == (7, 1)
+    b = _continue(it2, g2, 6)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    for key in ("interior", "boundary", "total", "change"):
+        assert np.array_equal(a[3][key], b[3][key]), key
+    for fa, fb in zip(a[4], b[4]):
+        assert np.array_equal(fa, fb)
+
+
+def test_primitive_restart_is_the_references(tmp_path):
+    chart, params, g, it = _run()
+    _continue(it, g, 5)
+    path = str(tmp_path / "ck.gx")
+    it.write_checkpoint(path)
+    prim = g.primitive()
+    g2 = Step_RK2_C_ASC(chart, params)
+    it2 = Integrator_C_PS(g2, IntegratorParameters(FinishTime=1e30, nRampCycles=4))
+    s = it2.restart_from(path, exact=False)
+    assert s.iCycle == 5 and s.Time == it.s.Time
+    # the Current after the restart: the stored primitives, conserved recomputed
+    assert np.array_equal(g2.primitive(), prim)
+    # ghost values are set by the oracle's own boundary fill
+    o = make_oracle(chart, tuple(np.pad(p, chart.g, mode="edge") for p in prim), courant=0.8)
+    assert np.array_equal(g2.conserved(), oracle_conserved(o, chart))
+    for _ in range(4):
+        dt = it2.compute_new_time() - it2.s.Time
+        it2.s.Time += dt
+        it2.s.iCycle += 1
+        g2.step(0.0, dt)
+        o.step(dt)
+        assert np.array_equal(g2.conserved(), oracle_conserved(o, chart))
+
+
+def test_bad_files_are_refused(tmp_path):
+    chart, params, g, it = _run()
+    _continue(it, g, 2)
+    path = str(tmp_path / "ck.gx")
+    it.write_checkpoint(path)
+    other = Step_RK2_C_ASC(Chart((24, 20, 15), bc=BC), params)
+    with pytest.raises(GxError, match="chart"):
+        other.read_checkpoint(path)
+    hll = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, RiemannSolverType="HLL"))
+    with pytest.raises(GxError, match="parameters"):
+        hll.read_checkpoint(path)
+    f32 = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, Precision="FP32"))
+    with pytest.raises(GxError, match="precision"):
+        f32.read_checkpoint(path, exact=True)
+    f32.read_checkpoint(path, exact=False)  # primitives restart across precisions
+    data = open(path, "rb").read()
+    cut = str(tmp_path / "cut.gx")
+    open(cut, "wb").write(data[: len(data) // 2])
+    with pytest.raises(GxError, match="truncated"):
+        g.read_checkpoint(cut, exact=True)
+    bad = str(tmp_path / "bad.gx")
+    open(bad, "wb").write(b"NOTACKPT" + data[8:])
+    with pytest.raises(GxError, match="format"):
+        g.read_checkpoint(bad)
+    with pytest.raises(GxError, match="cannot open"):
+        g.read_checkpoint(str(tmp_path / "missing.gx"))
