@@ -91,6 +91,47 @@ def _ramped(candidate: float, cycle: int, n_ramp: int = 100) -> float:
     return ramp * candidate if ramp < 1.0 else candidate
 
 
+def parabolic_check(n: int, steps: int, z_chunk: int):
+    """ReconstructionType PARABOLIC and LINEAR on the seeded Kelvin-Helmholtz
+    problem in FP64 (same time steps): speed of each, per-kernel times."""
+    from genasis_amd import problems
+    from genasis_amd.chart import Chart
+    from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
+
+    chart = Chart((n, n, n))
+    prim = problems.kelvin_helmholtz(chart, seed=1234, padded=False)
+    dts, runs = [], {}
+    for recon in ("LINEAR", "PARABOLIC"):
+        st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, ReconstructionType=recon))
+        st.set_tuning(z_chunk)
+        st.set_primitive(*prim)
+        st.set_profiling(True)
+        st.sync()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            c = st.time_step_candidate()
+            if recon == "LINEAR":
+                dts.append(_ramped(c, s))
+            st.step(0.0, dts[s])
+        st.sync()
+        t1 = time.perf_counter()
+        kt = st.kernel_times()
+        runs[recon] = {"ms_per_step": (t1 - t0) / steps * 1e3,
+                       "kernel_ms_avg": {k: v[0] / max(v[1], 1) for k, v in kt.items()}}
+        st.close()
+        print(f"# reconstruction {recon}: {runs[recon]['ms_per_step']:.3f} ms/step",
+              file=sys.stderr, flush=True)
+    return {
+        "workload": f"Kelvin-Helmholtz {n}^3 periodic, seed 1234, {steps} RK2 steps FP64, "
+                    "ReconstructionType PARABOLIC (LINEAR with the same time steps beside it)",
+        "value": chart.n_proper / (runs["PARABOLIC"]["ms_per_step"] * 1e-3),
+        "unit": "cell-updates/s",
+        "linear_value": chart.n_proper / (runs["LINEAR"]["ms_per_step"] * 1e-3),
+        "parabolic_over_linear_time": runs["PARABOLIC"]["ms_per_step"] / runs["LINEAR"]["ms_per_step"],
+        "kernel_ms_avg": runs["PARABOLIC"]["kernel_ms_avg"],
+    }
+
+
 def precision_check(n: int, steps: int, z_chunk: int):
     """FP64 and FP32 GPU runs of the seeded Kelvin-Helmholtz problem with the
     FP64 run's time steps; speed and per-field error of FP32 against FP64."""
@@ -338,6 +379,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-sequential", choices=("on", "off"), default="on")
     ap.add_argument("--adjoint-check", choices=("auto", "off"), default="auto")
+    ap.add_argument("--parabolic-check", choices=("auto", "off"), default="auto")
+    ap.add_argument("--parabolic-steps", type=int, default=30)
     ap.add_argument("--adjoint-n", type=int, default=128)
     ap.add_argument("--adjoint-steps", type=int, default=50)
     args = ap.parse_args()
@@ -398,6 +441,9 @@ def main():
     fp32 = None
     if args.fp32_check == "auto":
         fp32 = precision_check(args.fp32_n, args.fp32_steps, args.z_chunk)
+    parabolic = None
+    if args.parabolic_check == "auto":
+        parabolic = parabolic_check(args.n, args.parabolic_steps, args.z_chunk)
     adjoint = None
     if args.adjoint_check == "auto":
         adjoint = adjoint_check(args.adjoint_n, args.adjoint_steps, 10, 8, 2024)
@@ -447,6 +493,7 @@ def main():
         "fp32": fp32,
         "batch": batch,
         "adjoint": adjoint,
+        "parabolic": parabolic,
     }
     print(json.dumps(out))
 

@@ -27,6 +27,7 @@ FIELD_NAMES = (
     "EntropyPerBaryon", "ConservedEntropy", "SoundSpeed", "MachNumber",
 )
 
+GX_LINEAR, GX_PARABOLIC = 0, 1
 GX_RESTART_PRIMITIVE = 0
 GX_RESTART_EXACT = 1
 
@@ -78,6 +79,7 @@ class GxParams(ctypes.Structure):
         ("UseLimiter", ctypes.c_int),
         ("RiemannSolverType", ctypes.c_int),
         ("Precision", ctypes.c_int),
+        ("ReconstructionType", ctypes.c_int),
     ]
 
 

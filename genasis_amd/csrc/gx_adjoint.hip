@@ -669,6 +669,8 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
     return fail(GX_E_ARG, "gx_adjoint_forward: bad argument");
   if (!c->uploaded || !c->prims_valid)
     return fail(GX_E_STATE, "gx_adjoint_forward: needs freshly uploaded primitives");
+  if (c->params.ReconstructionType != GX_LINEAR)
+    return fail(GX_E_ARG, "gx_adjoint_forward: the adjoint differentiates ReconstructionType GX_LINEAR");
   int rc = ensure_state(c, nSteps, checkpointEvery);
   if (rc) return rc;
   AdjointState *A = c->adj;

@@ -414,6 +414,8 @@ int gx_batch_create(int nProblems, const gx_grid *grids, const gx_params *params
     gx_default_params(&p);
   int rc = check_params(p, "gx_batch_create");
   if (rc) return rc;
+  if (p.ReconstructionType != GX_LINEAR)
+    return fail(GX_E_ARG, "gx_batch_create: batches take ReconstructionType GX_LINEAR");
   std::vector<ChartGeometry> geoms(nProblems);
   for (int i = 0; i < nProblems; i++) {
     rc = check_grid(grids[i], "gx_batch_create");

@@ -14,7 +14,7 @@
 // tallies), so that a restart can either repeat the reference's
 // (GX_RESTART_PRIMITIVE) or continue bit for bit (GX_RESTART_EXACT).
 //
-// Layout (little-endian, 592-byte header with the fields in the order of
+// Layout (little-endian, 600-byte header with the fields in the order of
 // Header below and no implicit padding), then the primitives N, V_1,
 // V_2, V_3, E of the proper cells (5 x ncell doubles, x fastest), then U
 // and P0 (5 x ncell values each, 8 or 4 bytes by precision), the flags
@@ -51,6 +51,8 @@ struct Header {
   int32_t iCheckpoint;
   int32_t primsValid;
   int32_t invdtValid;
+  int32_t reconstruction;
+  int32_t pad;
   double lo[3], hi[3];
   double gamma, theta, shockThreshold, courant, nMin, mass;
   double time;
@@ -60,7 +62,7 @@ struct Header {
   int64_t nfluence;
   double tallyLocal[TALLY_N], tallyGlobal[TALLY_N], tallyTotal[TALLY_N], tallyChange[TALLY_N];
 };
-static_assert(sizeof(Header) == 592, "header layout: no implicit padding");
+static_assert(sizeof(Header) == 600, "header layout: no implicit padding");
 
 struct File {
   FILE *f = nullptr;
@@ -90,7 +92,8 @@ int check_match(const gx_ctx *c, const Header &h, const std::string &path) {
   if (h.gamma != p.AdiabaticIndex || h.theta != p.LimiterParameter ||
       h.shockThreshold != p.ShockThreshold || h.courant != p.CourantFactor ||
       h.nMin != p.BaryonDensityMin || h.mass != p.BaryonMassReference ||
-      h.useLimiter != p.UseLimiter || h.riemann != p.RiemannSolverType)
+      h.useLimiter != p.UseLimiter || h.riemann != p.RiemannSolverType ||
+      h.reconstruction != p.ReconstructionType)
     return io_fail(who, path, "the file's fluid parameters differ from the context's");
   if (h.ncell != c->ncell || h.nfluence != fluence_count(c->n))
     return io_fail(who, path, "the file's sizes do not match the chart");
@@ -120,6 +123,7 @@ int gx_write_checkpoint(gx_ctx *c, const char *path, double time, long long cycl
   for (int i = 0; i < 6; i++) h.bc[i] = c->grid.BoundaryCondition[i];
   h.useLimiter = c->params.UseLimiter;
   h.riemann = c->params.RiemannSolverType;
+  h.reconstruction = c->params.ReconstructionType;
   h.gamma = c->params.AdiabaticIndex;
   h.theta = c->params.LimiterParameter;
   h.shockThreshold = c->params.ShockThreshold;

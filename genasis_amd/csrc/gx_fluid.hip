@@ -80,6 +80,25 @@ using namespace gx;
 thread_local std::string gx::g_last_error;
 
 
+// The stage kernel and the boundary-line kernel of the context's
+// ReconstructionType.
+template <typename R, int STAGE>
+static void launch_stage(const gx_ctx *c, int nblk, hipStream_t st, const StageArgs<R> &a) {
+  if (c->params.ReconstructionType == GX_PARABOLIC)
+    stage_kernel_parabolic<R, STAGE><<<nblk, NT, 0, st>>>(a);
+  else
+    stage_kernel<R, STAGE><<<nblk, NT, 0, st>>>(a);
+}
+
+template <typename R, int STAGE>
+static void launch_boundary(const gx_ctx *c, hipStream_t st, const StageArgs<R> &a,
+                            const BoundaryArgs &B) {
+  if (c->params.ReconstructionType == GX_PARABOLIC)
+    boundary_flux_kernel<R, STAGE, RECON_PARABOLIC><<<c->n_bpartial, TALLY_BLOCK, 0, st>>>(a, B);
+  else
+    boundary_flux_kernel<R, STAGE, RECON_LINEAR><<<c->n_bpartial, TALLY_BLOCK, 0, st>>>(a, B);
+}
+
 extern "C" {
 
 void gx_default_params(gx_params *p) {
@@ -93,6 +112,7 @@ void gx_default_params(gx_params *p) {
   p->UseLimiter = 1;
   p->RiemannSolverType = GX_HLLC;
   p->Precision = GX_FP64;
+  p->ReconstructionType = GX_LINEAR;
 }
 
 const char *gx_last_error(void) { return g_last_error.c_str(); }
@@ -517,12 +537,12 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     a1.nbx = nbx;
     a1.nby = nby;
     a1.nblk = nblk;
-    stage_kernel<R, 1><<<nblk, block, 0, st>>>(a1);
+    launch_stage<R, 1>(c, nblk, st, a1);
     HIP_TRY(hipGetLastError());
     if (prof) HIP_TRY(hipEventRecord(c->ev[1], st));
     // Stage-1 boundary fluxes from the stage-1 input (stage 2 overwrites the
     // Solution).
-    boundary_flux_kernel<R, 1><<<c->n_bpartial, TALLY_BLOCK, 0, st>>>(a1, B);
+    launch_boundary<R, 1>(c, st, a1, B);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), st));
     if (prof) HIP_TRY(hipEventRecord(c->ev[2], st));
@@ -535,12 +555,12 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     a2.nbx = nbx;
     a2.nby = nby;
     a2.nblk = nblk;
-    stage_kernel<R, 2><<<nblk, block, 0, st>>>(a2);
+    launch_stage<R, 2>(c, nblk, st, a2);
     HIP_TRY(hipGetLastError());
     if (prof) HIP_TRY(hipEventRecord(c->ev[3], st));
     // Stage-2 boundary fluxes (input Y, the step's flags) and the step's
     // surface integral into the boundary tally.
-    boundary_flux_kernel<R, 2><<<c->n_bpartial, TALLY_BLOCK, 0, st>>>(a2, B);
+    launch_boundary<R, 2>(c, st, a2, B);
     HIP_TRY(hipGetLastError());
     boundary_tally_finish<<<1, TALLY_BLOCK, 0, st>>>(c->bpartial, c->n_bpartial,
                                                      c->tally_local);

@@ -35,6 +35,9 @@ inline int check_params(const gx_params &p, const char *who) {
     return fail(GX_E_ARG, std::string(who) + ": Precision must be GX_FP64 or GX_FP32");
   if (p.RiemannSolverType != GX_HLL && p.RiemannSolverType != GX_HLLC)
     return fail(GX_E_ARG, std::string(who) + ": RiemannSolverType must be GX_HLL or GX_HLLC");
+  if (p.ReconstructionType != GX_LINEAR && p.ReconstructionType != GX_PARABOLIC)
+    return fail(GX_E_ARG,
+                std::string(who) + ": ReconstructionType must be GX_LINEAR or GX_PARABOLIC");
   return GX_OK;
 }
 
@@ -102,6 +105,7 @@ inline void geometry_1d(int n, double lo, double hi, std::vector<double> out[N_G
     out[G_RDXL][p] = 1.0 / out[G_DXL][p];
     out[G_RDXR][p] = 1.0 / out[G_DXR][p];
     out[G_RDEN][p] = 1.0 / out[G_DEN][p];
+    out[G_CENTER][p] = c[p];
   }
 }
 

@@ -32,6 +32,7 @@ enum GeoArray {
   G_RDXL,    // RN(1/dxl)
   G_RDXR,    // RN(1/dxr)
   G_RDEN,    // RN(1/den)
+  G_CENTER,  // Center (Chart_Template.f90:827-841), for PARABOLIC
   N_GEO
 };
 
@@ -129,6 +130,90 @@ __device__ __forceinline__ R slope(R vm, R v0, R vp, R dxl, R dxr, R dxl2, R dxr
          fmin(fmin(fabs(Arith<R>::rdiv(theta * dVL, dxl, rdxl)),
                    fabs(Arith<R>::rdiv(theta * dVR, dxr, rdxr))),
               fabs(central));
+}
+
+// ReconstructionType (IncrementDivergence_FV__Form.f90:239-241, 637-655).
+enum Recon { RECON_LINEAR = 0, RECON_PARABOLIC = 1 };
+
+// ComputeReconstructionParabolic_CSL_Kernel (EV/Increments/IncrementDivergence
+// _FV__Kernel.f90:296-596) for one cell, with XA = X and X2A = X**2 as
+// ComputeReconstructionParabolic_CSL passes them (IncrementDivergence_FV__
+// Form.f90:852-931): the values at the cell's inner face xC - dXL (*vI) and
+// outer face xC + dXR (*vO) from the values and centres of the cell and its
+// neighbours.  A local extremum of the three values gives vC on both faces;
+// otherwise the parabola through the three points, replaced on the side of
+// a vertex inside the stencil by the parabola flat at the neighbour; then
+// the parabola through (xI, vI), (xC, vC), (xO, vO), likewise replaced by one
+// flat at xI or xO.  The reference's expression order: a*b*c = (a*b)*c,
+// a+b+c = (a+b)+c, x**2 = x*x; -2*a*b/D rounds as ((-2*a)*b)/D.
+//
+// FP64 keeps the reference's absolute coordinates (bit parity).  FP32 takes
+// coordinates relative to the cell centre: the parabolas and the vertex
+// tests are invariant under the shift, and the reference's determinants,
+// differences of O(x^3) terms that are O(dx^3), would otherwise lose about
+// log10(x^2 / dx^3) digits, more than single precision holds.
+template <typename R>
+__device__ __forceinline__ void parabolic_faces(R vM, R vC, R vP, R xM, R xC, R xP, R dXL,
+                                                R dXR, R *pI, R *pO) {
+  if constexpr (sizeof(R) == sizeof(float)) {
+    xM = xM - xC;
+    xP = xP - xC;
+    xC = R(0);
+  }
+  const R tiny = Arith<R>::tiny;  // SqrtTiny = tiny ( 0.0_KDR )
+  const R xAM = xM, xAC = xC, xAP = xP;
+  const R x2AM = xM * xM, x2AC = xC * xC, x2AP = xP * xP;
+  const R xI = xC - dXL, xO = xC + dXR;
+  R vI = vC, vO = vC;
+  if (!((vC - vM) * (vP - vC) <= R(0))) {
+    R D = ((x2AM - x2AP) * xAC + (x2AP - x2AC) * xAM) + (x2AC - x2AM) * xAP;
+    R c0 = Arith<R>::div((vP * (x2AM * xAC - x2AC * xAM) + vM * (x2AC * xAP - x2AP * xAC)) +
+                             vC * (x2AP * xAM - x2AM * xAP),
+                         D);
+    R c1 = Arith<R>::div((vP * (x2AC - x2AM) + vC * (x2AM - x2AP)) + vM * (x2AP - x2AC), D);
+    R c2 = Arith<R>::div((vP * (xAM - xAC) + vM * (xAC - xAP)) + vC * (xAP - xAM), D);
+    R xE = -Arith<R>::div(c1, R(2) * copysign(fmax(fabs(c2), tiny), c2));
+    vI = (c0 + c1 * xI) + c2 * (xI * xI);
+    vO = (c0 + c1 * xO) + c2 * (xO * xO);
+    if (xE > xM && xE <= xC) {
+      D = (x2AC - x2AM) + (R(2) * (xAM - xAC)) * xAM;
+      c0 = Arith<R>::div((vM * x2AC - vC * x2AM) + (R(2) * (vC * xAM - vM * xAC)) * xAM, D);
+      c1 = Arith<R>::div((R(-2) * (vC - vM)) * xAM, D);
+      c2 = Arith<R>::div(vC - vM, D);
+      vI = (c0 + c1 * xI) + c2 * (xI * xI);
+    } else if (xE > xC && xE < xP) {
+      D = (x2AC - x2AP) + (R(2) * (xAP - xAC)) * xAP;
+      c0 = Arith<R>::div((vP * x2AC - vC * x2AP) + (R(2) * (vC * xAP - vP * xAC)) * xAP, D);
+      c1 = Arith<R>::div((R(-2) * (vC - vP)) * xAP, D);
+      c2 = Arith<R>::div(vC - vP, D);
+      vO = (c0 + c1 * xO) + c2 * (xO * xO);
+    }
+    D = (xI - xO) * ((x2AC + xI * xO) - xAC * (xI + xO));
+    c0 = Arith<R>::div(((vO * xI) * (x2AC - xAC * xI) + ((vC * xI) * xO) * (xI - xO)) +
+                           (vI * xO) * (xAC * xO - x2AC),
+                       D);
+    c1 = Arith<R>::div((vO * (xI * xI - x2AC) + vI * (x2AC - xO * xO)) + vC * (xO * xO - xI * xI),
+                       D);
+    c2 = Arith<R>::div((vO * (xAC - xI) + vC * (xI - xO)) + vI * (xO - xAC), D);
+    xE = -Arith<R>::div(c1, R(2) * copysign(fmax(fabs(c2), tiny), c2));
+    vI = (c0 + c1 * xI) + c2 * (xI * xI);
+    vO = (c0 + c1 * xO) + c2 * (xO * xO);
+    if (xE > xI && xE <= xC) {
+      D = (x2AC - xO * xO) + (R(2) * xI) * (xO - xAC);
+      c0 = Arith<R>::div((vO * x2AC - vC * (xO * xO)) + (R(2) * xI) * (vC * xO - vO * xAC), D);
+      c1 = Arith<R>::div((R(-2) * (vC - vO)) * xI, D);
+      c2 = Arith<R>::div(vC - vO, D);
+      vI = (c0 + c1 * xI) + c2 * (xI * xI);
+    } else if (xE > xC && xE < xO) {
+      D = (x2AC - xI * xI) + (R(2) * xO) * (xI - xAC);
+      c0 = Arith<R>::div((vI * x2AC - vC * (xI * xI)) + (R(2) * xO) * (vC * xI - vI * xAC), D);
+      c1 = Arith<R>::div((R(-2) * (vC - vI)) * xO, D);
+      c2 = Arith<R>::div(vC - vI, D);
+      vO = (c0 + c1 * xO) + c2 * (xO * xO);
+    }
+  }
+  *pI = vI;
+  *pO = vO;
 }
 
 // One reconstructed face state after ComputeFromPrimitiveCommon

@@ -111,20 +111,30 @@ __device__ inline unsigned line_cell(const int n[3], int u, int v, int c) {
 
 // HLL(C) flux of the face between cells with primitives Vm1 | V0, from the
 // four-cell stencil, as the stage kernel computes it (slopes of the two
-// cells beside the face, reconstruction, then the face flux).
-template <typename R, int D>
+// cells beside the face and the linear reconstruction, or the two cells'
+// parabolic face values; then the face flux).
+template <typename R, int D, int RECON>
 __device__ inline void stencil_flux(const StageArgs<R> &a, int cl, const R Vm2[5],
                                     const R Vm1[5], const R V0[5], const R Vp1[5], bool df,
                                     R F[5]) {
   const int cr = cl + 1;
-  const R wr = geo(a, D, G_WR, cl + 2), wl = geo(a, D, G_WL, cr + 2);
   R WL[5], WR[5];
+  if constexpr (RECON == RECON_PARABOLIC) {
 #pragma unroll
-  for (int f = 0; f < 5; f++) {
-    const R sl = slope_at(a, D, cl, Vm2[f], Vm1[f], V0[f]);
-    const R sr = slope_at(a, D, cr, Vm1[f], V0[f], Vp1[f]);
-    WL[f] = Vm1[f] + wr * sl;
-    WR[f] = V0[f] - wl * sr;
+    for (int f = 0; f < 5; f++) {
+      R iL, iR, oR;
+      parabolic_at(a, D, cl, Vm2[f], Vm1[f], V0[f], &iL, &WL[f]);
+      parabolic_at(a, D, cr, Vm1[f], V0[f], Vp1[f], &WR[f], &oR);
+    }
+  } else {
+    const R wr = geo(a, D, G_WR, cl + 2), wl = geo(a, D, G_WL, cr + 2);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      const R sl = slope_at(a, D, cl, Vm2[f], Vm1[f], V0[f]);
+      const R sr = slope_at(a, D, cr, Vm1[f], V0[f], Vp1[f]);
+      WL[f] = Vm1[f] + wr * sl;
+      WR[f] = V0[f] - wl * sr;
+    }
   }
   face_flux(WL, WR, D, df, a.ph, F);
 }
@@ -140,7 +150,7 @@ __device__ inline void prim_of(const StageArgs<R> &a, const R raw[5], bool flip,
 // rule maps the ghosts onto them), so each raw value is loaded once.  The
 // inner face takes the DiffusiveFlux flag of the line's first cell; the
 // outer face, between the last cell and a ghost, takes none.
-template <typename R, int D>
+template <typename R, int D, int RECON>
 __device__ inline void line_fluxes(const StageArgs<R> &a, long long idx, R Fin[5],
                                    R Fout[5]) {
   int u, v;
@@ -171,7 +181,7 @@ __device__ inline void line_fluxes(const StageArgs<R> &a, long long idx, R Fin[5
     prim_of<R, D>(a, r0, bin == BC_REFLECTING, G1);
     prim_of<R, D>(a, r1, bin == BC_REFLECTING, G2);
   }
-  stencil_flux<R, D>(a, -1, G2, G1, P0, P1, df, Fin);
+  stencil_flux<R, D, RECON>(a, -1, G2, G1, P0, P1, df, Fin);
   // outer ghosts n, n+1: periodic images 0, 1, or mirrors of n-1, n-2
   if (bout == BC_PERIODIC) {
     prim_of<R, D>(a, r0, false, G1);
@@ -180,7 +190,7 @@ __device__ inline void line_fluxes(const StageArgs<R> &a, long long idx, R Fin[5
     prim_of<R, D>(a, q1, bout == BC_REFLECTING, G1);
     prim_of<R, D>(a, q2, bout == BC_REFLECTING, G2);
   }
-  stencil_flux<R, D>(a, nd - 1, Q2, Q1, G1, G2, false, Fout);
+  stencil_flux<R, D, RECON>(a, nd - 1, Q2, Q1, G1, G2, false, Fout);
 }
 
 // BF = BF + fw * F for the five fields of one face value (stage 1 adds to
@@ -221,11 +231,11 @@ __device__ inline void add_integrands(const BoundaryArgs &B, int gs, int side, i
   for (int i = 0; i < NB_SUM; i++) I[i] += dir * J[i] * dA;
 }
 
-template <typename R, int STAGE, int D>
+template <typename R, int STAGE, int D, int RECON>
 __device__ inline void boundary_line(const StageArgs<R> &a, const BoundaryArgs &B,
                                      const int n[3], long long idx, double I[NB_SUM]) {
   R Fin[5], Fout[5];
-  line_fluxes<R, D>(a, idx, Fin, Fout);
+  line_fluxes<R, D, RECON>(a, idx, Fin, Fout);
   double vin[5], vout[5];
   record_face<STAGE>(B, n, 2 * D, idx, Fin, vin);
   record_face<STAGE>(B, n, 2 * D + 1, idx, Fout, vout);
@@ -240,7 +250,7 @@ __device__ inline void boundary_line(const StageArgs<R> &a, const BoundaryArgs &
 // One stage of fluence recording, one boundary line (two face values) per
 // thread; stage 2 also sums the boundary integrands of the finished fluence
 // into per-block partials.
-template <typename R, int STAGE>
+template <typename R, int STAGE, int RECON>
 __global__ __launch_bounds__(TALLY_BLOCK) void boundary_flux_kernel(StageArgs<R> a,
                                                                     BoundaryArgs B) {
   const int n[3] = {a.n[0], a.n[1], a.n[2]};
@@ -253,11 +263,11 @@ __global__ __launch_bounds__(TALLY_BLOCK) void boundary_flux_kernel(StageArgs<R>
     long long idx;
     boundary_line_of(n, e, &d, &idx);
     if (d == 0)
-      boundary_line<R, STAGE, 0>(a, B, n, idx, I);
+      boundary_line<R, STAGE, 0, RECON>(a, B, n, idx, I);
     else if (d == 1)
-      boundary_line<R, STAGE, 1>(a, B, n, idx, I);
+      boundary_line<R, STAGE, 1, RECON>(a, B, n, idx, I);
     else
-      boundary_line<R, STAGE, 2>(a, B, n, idx, I);
+      boundary_line<R, STAGE, 2, RECON>(a, B, n, idx, I);
   }
   if (STAGE == 2) {
     __shared__ double red[NB_SUM][TALLY_BLOCK];

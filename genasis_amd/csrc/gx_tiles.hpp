@@ -171,6 +171,16 @@ __device__ __forceinline__ R slope_at(const StageArgs<R> &a, int d, int c, R vm,
                R(a.ph.theta), a.ph.use_limiter);
 }
 
+// Parabolic face values of cell c of dimension d from the global geometry.
+template <typename R>
+__device__ __forceinline__ void parabolic_at(const StageArgs<R> &a, int d, int c, R vm,
+                                             R v0, R vp, R *vI, R *vO) {
+  const R *g = a.geo + d * N_GEO * a.gs + (c + 2);
+  const int s = a.gs;
+  parabolic_faces(vm, v0, vp, g[G_CENTER * s - 1], g[G_CENTER * s], g[G_CENTER * s + 1],
+                  g[G_WL * s], g[G_WR * s], vI, vO);
+}
+
 // XCD-aware tile order.  The dispatcher deals consecutive blocks to the XCDs
 // in turn; block b is given tile t so that every XCD owns one contiguous run
 // of tiles (x fastest, then y, then z-chunk).  Tiles that share halo cache
@@ -208,6 +218,24 @@ __device__ __forceinline__ R slope_lds(const TileGeo<R> &sG, int d, int l, R vm,
   return slope(vm, v0, vp, sG[d][G_DXL][l], sG[d][G_DXR][l], sG[d][G_DXL2][l],
                sG[d][G_DXR2][l], sG[d][G_DEN][l], sG[d][G_RDXL][l],
                sG[d][G_RDXR][l], sG[d][G_RDEN][l], theta, use_limiter);
+}
+
+// Parabolic face values of the cell at local geometry index l.
+template <typename R>
+__device__ __forceinline__ void parabolic_lds(const TileGeo<R> &sG, int d, int l, R vm,
+                                              R v0, R vp, R *vI, R *vO) {
+  parabolic_faces(vm, v0, vp, sG[d][G_CENTER][l - 1], sG[d][G_CENTER][l],
+                  sG[d][G_CENTER][l + 1], sG[d][G_WL][l], sG[d][G_WR][l], vI, vO);
+}
+
+// Face flux from given face states into slot of sF.
+template <typename R, typename SF>
+__device__ __forceinline__ void face_to_lds(const Physics &ph, const R WL[5], const R WR[5],
+                                            int d, bool df, SF sF, int slot) {
+  R F[5];
+  face_flux(WL, WR, d, df, ph, F);
+#pragma unroll
+  for (int f = 0; f < 5; f++) sF[f][slot] = F[f];
 }
 
 // x-face i (between cells x0+i-1 and x0+i) of row j into sFx.
@@ -264,9 +292,17 @@ template <typename R> struct StageShared {
   TileGeo<R> sG;
 };
 
-template <typename R, int STAGE>
+// PARABOLIC only: the inner-face values of the cells just beyond the tile's
+// right column and top row (the slopes' LDS then holds outer-face values).
+template <typename R> struct EdgeShared {
+  R sEx[5][TY];
+  R sEy[5][TX];
+};
+
+template <typename R, int STAGE, int RECON = RECON_LINEAR>
 __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0, int z0,
-                                           StageShared<R> &L) {
+                                           StageShared<R> &L, EdgeShared<R> *E = nullptr) {
+  constexpr bool PARA = RECON == RECON_PARABOLIC;
   auto &sP = L.sP;
   auto &sSx = L.sSx;
   auto &sSy = L.sSy;
@@ -307,12 +343,24 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
     load_prim(a, x, y, z0 + 1, Vk1);
     load_raw(a, x, y, z0 + 2, zr, &zflip);
     flk = ld(a.flags, lin(a.n, x, y, z0));
+    if constexpr (PARA) {
+      // sk holds the outer-face value of plane k
 #pragma unroll
-    for (int f = 0; f < 5; f++) {
-      sm1[f] = slope_at(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f]);
-      sk[f] = slope_at(a, 2, z0, Vm1[f], Vk[f], Vk1[f]);
-      WL[f] = Vm1[f] + geo(a, 2, G_WR, z0 - 1 + 2) * sm1[f];
-      WR[f] = Vk[f] - geo(a, 2, G_WL, z0 + 2) * sk[f];
+      for (int f = 0; f < 5; f++) {
+        R iM1, iK;
+        parabolic_at(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f], &iM1, &sm1[f]);
+        parabolic_at(a, 2, z0, Vm1[f], Vk[f], Vk1[f], &iK, &sk[f]);
+        WL[f] = sm1[f];
+        WR[f] = iK;
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        sm1[f] = slope_at(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f]);
+        sk[f] = slope_at(a, 2, z0, Vm1[f], Vk[f], Vk1[f]);
+        WL[f] = Vm1[f] + geo(a, 2, G_WR, z0 - 1 + 2) * sm1[f];
+        WR[f] = Vk[f] - geo(a, 2, G_WL, z0 + 2) * sk[f];
+      }
     }
     face_flux(WL, WR, 2, (flk >> 2) & 1, ph, Flo);
   }
@@ -357,19 +405,33 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
     }
     __syncthreads();
 
-    // Slopes: own cell (x and y), then the tile-edge cells.
+    // Slopes: own cell (x and y), then the tile-edge cells.  PARABOLIC: the
+    // outer-face values go where the slopes go, the own inner-face values
+    // stay in registers, the edge cells' inner-face values go to E.
+    R ix[5], iy[5];
     if (active) {
       const int bx = ctr;
+      if constexpr (PARA) {
 #pragma unroll
-      for (int f = 0; f < 5; f++)
-        sSx[f][ty * SXW + tx + 1] = slope_lds(sG, 0, tx + 2, sP[f][bx - 1], sP[f][bx],
-                                              sP[f][bx + 1], theta, ph.use_limiter);
-      __builtin_amdgcn_sched_barrier(0);
+        for (int f = 0; f < 5; f++)
+          parabolic_lds(sG, 0, tx + 2, sP[f][bx - 1], sP[f][bx], sP[f][bx + 1], &ix[f],
+                        &sSx[f][ty * SXW + tx + 1]);
 #pragma unroll
-      for (int f = 0; f < 5; f++)
-        sSy[f][(ty + 1) * TX + tx] = slope_lds(sG, 1, ty + 2, sP[f][bx - HX], sP[f][bx],
-                                               sP[f][bx + HX], theta, ph.use_limiter);
-      __builtin_amdgcn_sched_barrier(0);
+        for (int f = 0; f < 5; f++)
+          parabolic_lds(sG, 1, ty + 2, sP[f][bx - HX], sP[f][bx], sP[f][bx + HX], &iy[f],
+                        &sSy[f][(ty + 1) * TX + tx]);
+      } else {
+#pragma unroll
+        for (int f = 0; f < 5; f++)
+          sSx[f][ty * SXW + tx + 1] = slope_lds(sG, 0, tx + 2, sP[f][bx - 1], sP[f][bx],
+                                                sP[f][bx + 1], theta, ph.use_limiter);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int f = 0; f < 5; f++)
+          sSy[f][(ty + 1) * TX + tx] = slope_lds(sG, 1, ty + 2, sP[f][bx - HX], sP[f][bx],
+                                                 sP[f][bx + HX], theta, ph.use_limiter);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
     if (tid < nxs + nys) {
       int base, stride, l, out, gd;
@@ -392,29 +454,87 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         gd = 1;
         out = j * TX + i;
       }
+      if constexpr (PARA) {
+        // left/bottom edge cell: its outer-face value; right/top: its inner
+        const bool hi = isx ? (tid & 1) : ((tid - nxs) & 1);
+        const int e = isx ? (tid >> 1) : ((tid - nxs) >> 1);
 #pragma unroll
-      for (int f = 0; f < 5; f++) {
-        const R sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
-                               sP[f][base + stride], theta, ph.use_limiter);
-        if (isx)
-          sSx[f][out] = sv;
-        else
-          sSy[f][out] = sv;
+        for (int f = 0; f < 5; f++) {
+          R vI, vO;
+          parabolic_lds(sG, gd, l, sP[f][base - stride], sP[f][base], sP[f][base + stride],
+                        &vI, &vO);
+          if (!hi) {
+            if (isx)
+              sSx[f][out] = vO;
+            else
+              sSy[f][out] = vO;
+          } else {
+            if (isx)
+              E->sEx[f][e] = vI;
+            else
+              E->sEy[f][e] = vI;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          const R sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
+                                 sP[f][base + stride], theta, ph.use_limiter);
+          if (isx)
+            sSx[f][out] = sv;
+          else
+            sSy[f][out] = sv;
+        }
       }
     }
     __syncthreads();
 
     // Faces: left and bottom faces of the own cell, then the right column of
     // x-faces (wave 0) and the top row of y-faces (wave 1).
-    if (active) {
-      x_face(ph, sP, sSx, sFx, sG, tx, ty, flk & 1u);
+    if constexpr (PARA) {
+      R WL[5], WR[5];
+      if (active) {
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          WL[f] = sSx[f][ty * SXW + tx];
+          WR[f] = ix[f];
+        }
+        face_to_lds(ph, WL, WR, 0, flk & 1u, sFx, ty * FXW + tx);
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          WL[f] = sSy[f][ty * TX + tx];
+          WR[f] = iy[f];
+        }
+        face_to_lds(ph, WL, WR, 1, (flk >> 1) & 1u, sFy, ty * TX + tx);
+      }
+      if (tid < th) {
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          WL[f] = sSx[f][tid * SXW + tw];
+          WR[f] = E->sEx[f][tid];
+        }
+        face_to_lds(ph, WL, WR, 0, flx, sFx, tid * FXW + tw);
+      }
+      if (tid >= WAVE && tid - WAVE < tw) {
+        const int i = tid - WAVE;
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          WL[f] = sSy[f][th * TX + i];
+          WR[f] = E->sEy[f][i];
+        }
+        face_to_lds(ph, WL, WR, 1, flx, sFy, th * TX + i);
+      }
+    } else {
+      if (active) {
+        x_face(ph, sP, sSx, sFx, sG, tx, ty, flk & 1u);
+        __builtin_amdgcn_sched_barrier(0);
+        y_face(ph, sP, sSy, sFy, sG, tx, ty, (flk >> 1) & 1u);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      if (tid < th) x_face(ph, sP, sSx, sFx, sG, tw, tid, flx);
       __builtin_amdgcn_sched_barrier(0);
-      y_face(ph, sP, sSy, sFy, sG, tx, ty, (flk >> 1) & 1u);
-      __builtin_amdgcn_sched_barrier(0);
+      if (tid >= WAVE && tid - WAVE < tw) y_face(ph, sP, sSy, sFy, sG, tid - WAVE, th, flx);
     }
-    if (tid < th) x_face(ph, sP, sSx, sFx, sG, tw, tid, flx);
-    __builtin_amdgcn_sched_barrier(0);
-    if (tid >= WAVE && tid - WAVE < tw) y_face(ph, sP, sSy, sFy, sG, tid - WAVE, th, flx);
     __syncthreads();
 
     if (active) {
@@ -423,12 +543,22 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
       {
         R WL[5], WR[5];
         raw_to_prim(a, zr, zflip, Vk2);
+        if constexpr (PARA) {
 #pragma unroll
-        for (int f = 0; f < 5; f++) {
-          const R vk = sP[f][ctr];
-          sk1[f] = slope_at(a, 2, k + 1, vk, Vk1[f], Vk2[f]);
-          WL[f] = vk + geo(a, 2, G_WR, k + 2) * sk[f];
-          WR[f] = Vk1[f] - geo(a, 2, G_WL, k + 1 + 2) * sk1[f];
+          for (int f = 0; f < 5; f++) {
+            R iK1;
+            parabolic_at(a, 2, k + 1, sP[f][ctr], Vk1[f], Vk2[f], &iK1, &sk1[f]);
+            WL[f] = sk[f];
+            WR[f] = iK1;
+          }
+        } else {
+#pragma unroll
+          for (int f = 0; f < 5; f++) {
+            const R vk = sP[f][ctr];
+            sk1[f] = slope_at(a, 2, k + 1, vk, Vk1[f], Vk2[f]);
+            WL[f] = vk + geo(a, 2, G_WR, k + 2) * sk[f];
+            WR[f] = Vk1[f] - geo(a, 2, G_WL, k + 1 + 2) * sk1[f];
+          }
         }
         face_flux(WL, WR, 2, (fln >> 2) & 1u, ph, Fhi);
       }
@@ -519,6 +649,17 @@ __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel(
   const int tile = tile_of_block(blockIdx.x, a.nblk);
   stage_tile<R, STAGE>(a, (tile % a.nbx) * TX, ((tile / a.nbx) % a.nby) * TY,
                        (tile / (a.nbx * a.nby)) * a.tz, L);
+}
+
+// The same stage with PARABOLIC reconstruction.
+template <typename R, int STAGE>
+__global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel_parabolic(
+    StageArgs<R> a) {
+  __shared__ StageShared<R> L;
+  __shared__ EdgeShared<R> E;
+  const int tile = tile_of_block(blockIdx.x, a.nblk);
+  stage_tile<R, STAGE, RECON_PARABOLIC>(a, (tile % a.nbx) * TX, ((tile / a.nbx) % a.nby) * TY,
+                                        (tile / (a.nbx * a.nby)) * a.tz, L, &E);
 }
 
 // Max CFL term over the current state (the first step after an upload, or

@@ -33,12 +33,15 @@ class FluidParameters:
     UseLimiter: bool = True
     RiemannSolverType: str = "HLLC"
     Precision: str = "FP64"
+    ReconstructionType: str = "LINEAR"
 
     def to_c(self) -> _capi.GxParams:
         if self.RiemannSolverType not in ("HLL", "HLLC"):
             raise ValueError(f"RiemannSolverType not recognized: {self.RiemannSolverType}")
         if self.Precision not in ("FP64", "FP32"):
             raise ValueError(f"Precision not recognized: {self.Precision}")
+        if self.ReconstructionType not in ("LINEAR", "PARABOLIC"):
+            raise ValueError(f"ReconstructionType not recognized: {self.ReconstructionType}")
         p = _capi.GxParams()
         p.AdiabaticIndex = self.AdiabaticIndex
         p.LimiterParameter = self.LimiterParameter
@@ -49,6 +52,8 @@ class FluidParameters:
         p.UseLimiter = int(bool(self.UseLimiter))
         p.RiemannSolverType = _capi.GX_HLLC if self.RiemannSolverType == "HLLC" else _capi.GX_HLL
         p.Precision = _capi.GX_FP64 if self.Precision == "FP64" else _capi.GX_FP32
+        p.ReconstructionType = (_capi.GX_PARABOLIC if self.ReconstructionType == "PARABOLIC"
+                                else _capi.GX_LINEAR)
         return p
 
 

@@ -33,6 +33,7 @@ module GX_Fluid_Binding
     integer ( c_int ) :: UseLimiter
     integer ( c_int ) :: RiemannSolverType
     integer ( c_int ) :: Precision
+    integer ( c_int ) :: ReconstructionType
   end type gx_params
 
   public :: &

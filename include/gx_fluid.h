@@ -66,6 +66,10 @@ extern "C" {
 #define GX_HLL 0
 #define GX_HLLC 1
 
+/* ReconstructionType (IncrementDivergence_FV__Form.f90:239-241, 637-655) */
+#define GX_LINEAR 0
+#define GX_PARABOLIC 1
+
 /* Precision of the state and sweep arithmetic */
 #define GX_FP64 0
 #define GX_FP32 1
@@ -120,6 +124,7 @@ typedef struct {
   int UseLimiter;             /* 1 */
   int RiemannSolverType;      /* GX_HLLC */
   int Precision;              /* GX_FP64 */
+  int ReconstructionType;     /* GX_LINEAR (IncrementDivergence_FV__Form.f90:239) */
 } gx_params;
 
 typedef struct gx_ctx gx_ctx;

@@ -67,6 +67,8 @@ def _load():
         lib.gxo_set_boundary_fluence.argtypes = [ctypes.c_void_p, ctypes.c_int,
                                                  ctypes.c_int, _PD]
         lib.gxo_surface_integral.argtypes = [ctypes.c_void_p, _PD]
+        lib.gxo_set_reconstruction.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.gxo_parabolic_face_values.argtypes = [_D] * 8 + [_PD]
         _lib = lib
     return _lib
 
@@ -77,6 +79,13 @@ def _ptr(a: np.ndarray):
 
 def limited_slope(dv_l, dv_r, dx_l, dx_r, theta=1.4) -> float:
     return _load().gxo_limited_slope(dv_l, dv_r, dx_l, dx_r, theta)
+
+
+def parabolic_face_values(vm, vc, vp, xm, xc, xp, dxl, dxr):
+    """(vI, vO) of the parabolic reconstruction of one cell."""
+    out = np.zeros(2)
+    _load().gxo_parabolic_face_values(vm, vc, vp, xm, xc, xp, dxl, dxr, _ptr(out))
+    return float(out[0]), float(out[1])
 
 
 def face_flux(prim_l, prim_r, d, gamma=1.4, riemann="HLLC", df=0.0):
@@ -93,7 +102,7 @@ class OracleFluid:
 
     def __init__(self, chart, gamma=1.4, theta=1.4, use_limiter=True,
                  riemann="HLLC", shock_threshold=0.1, courant=0.7,
-                 n_min=0.0, n_threads=0):
+                 n_min=0.0, n_threads=0, reconstruction="LINEAR"):
         lib = _load()
         self.chart = chart
         self.shape = chart.m[::-1]
@@ -107,6 +116,9 @@ class OracleFluid:
         if not self._h:
             raise MemoryError("gxo_create failed")
         self._lib = lib
+        if reconstruction not in ("LINEAR", "PARABOLIC"):
+            raise ValueError(f"ReconstructionType not recognized: {reconstruction}")
+        lib.gxo_set_reconstruction(self._h, int(reconstruction == "PARABOLIC"))
 
     def close(self):
         if self._h:

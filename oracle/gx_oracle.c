@@ -47,6 +47,7 @@ struct gxo_ctx {
   int bc[3][2];
   double gamma, theta, st, courant, n_min, c_v, n0, p0, mass;
   int use_limiter, riemann;
+  int parabolic; /* ReconstructionType = 'PARABOLIC' */
   double *edge[3], *center[3], *wl[3], *wr[3];
   double *cur[GXO_N_FIELDS];
   double *df[3];
@@ -180,6 +181,8 @@ gxo_ctx *gxo_create(const int n[3], const double lo[3], const double hi[3],
   }
   return c;
 }
+
+void gxo_set_reconstruction(gxo_ctx *c, int parabolic) { c->parabolic = parabolic != 0; }
 
 void gxo_destroy(gxo_ctx *c) {
   if (!c) return;
@@ -451,6 +454,106 @@ static void gradient(gxo_ctx *c, int d) {
   }
 }
 
+/* ComputeReconstructionParabolic_CSL_Kernel (EV/Increments/IncrementDivergence_
+ * FV__Kernel.f90:296-596) for one cell, with XA = X and X2A = X**2 as
+ * ComputeReconstructionParabolic_CSL passes them (IncrementDivergence_FV__
+ * Form.f90:852-931): the cell's inner and outer face values vI, vO from the
+ * values and centres of the cell and its two neighbours.  A local extremum of
+ * the three values gives the constant vC; otherwise the parabola through the
+ * three points, replaced on the side of an interior vertex by the parabola
+ * flat at the neighbour; then the parabola through (xI, vI), (xC, vC),
+ * (xO, vO), likewise replaced by one flat at xI or xO.  Fortran's a*b*c is
+ * (a*b)*c, a+b+c is (a+b)+c, x**2 is x*x; -2*a*b/D is -((2*a*b)/D), which
+ * rounds as ((-2*a)*b)/D. */
+void gxo_parabolic_face_values(double vM, double vC, double vP, double xM,
+                               double xC, double xP, double dX_L, double dX_R,
+                               double out[2]) {
+  const double sqrt_tiny = DBL_MIN; /* SqrtTiny = tiny ( 0.0_KDR ) */
+  const double xAM = xM, xAC = xC, xAP = xP;
+  const double x2AM = xM * xM, x2AC = xC * xC, x2AP = xP * xP;
+  const double xI = xC - dX_L, xO = xC + dX_R;
+  double vI, vO, D, c0, c1, c2, c2s, xE;
+  if ((vC - vM) * (vP - vC) <= 0.0) {
+    vI = vC;
+    vO = vC;
+  } else {
+    /* first parabola */
+    D = ((x2AM - x2AP) * xAC + (x2AP - x2AC) * xAM) + (x2AC - x2AM) * xAP;
+    c0 = ((vP * (x2AM * xAC - x2AC * xAM) + vM * (x2AC * xAP - x2AP * xAC)) +
+          vC * (x2AP * xAM - x2AM * xAP)) / D;
+    c1 = ((vP * (x2AC - x2AM) + vC * (x2AM - x2AP)) + vM * (x2AP - x2AC)) / D;
+    c2 = ((vP * (xAM - xAC) + vM * (xAC - xAP)) + vC * (xAP - xAM)) / D;
+    c2s = copysign(fmax(fabs(c2), sqrt_tiny), c2);
+    xE = -c1 / (2.0 * c2s);
+    vI = (c0 + c1 * xI) + c2 * (xI * xI);
+    vO = (c0 + c1 * xO) + c2 * (xO * xO);
+    if (xE > xM && xE <= xC) {
+      D = (x2AC - x2AM) + (2 * (xAM - xAC)) * xAM;
+      c0 = ((vM * x2AC - vC * x2AM) + (2 * (vC * xAM - vM * xAC)) * xAM) / D;
+      c1 = ((-2 * (vC - vM)) * xAM) / D;
+      c2 = (vC - vM) / D;
+      vI = (c0 + c1 * xI) + c2 * (xI * xI);
+    } else if (xE > xC && xE < xP) {
+      D = (x2AC - x2AP) + (2 * (xAP - xAC)) * xAP;
+      c0 = ((vP * x2AC - vC * x2AP) + (2 * (vC * xAP - vP * xAC)) * xAP) / D;
+      c1 = ((-2 * (vC - vP)) * xAP) / D;
+      c2 = (vC - vP) / D;
+      vO = (c0 + c1 * xO) + c2 * (xO * xO);
+    }
+    /* second parabola */
+    D = (xI - xO) * ((x2AC + xI * xO) - xAC * (xI + xO));
+    c0 = (((vO * xI) * (x2AC - xAC * xI) + ((vC * xI) * xO) * (xI - xO)) +
+          (vI * xO) * (xAC * xO - x2AC)) / D;
+    c1 = ((vO * (xI * xI - x2AC) + vI * (x2AC - xO * xO)) + vC * (xO * xO - xI * xI)) / D;
+    c2 = ((vO * (xAC - xI) + vC * (xI - xO)) + vI * (xO - xAC)) / D;
+    c2s = copysign(fmax(fabs(c2), sqrt_tiny), c2);
+    xE = -c1 / (2.0 * c2s);
+    vI = (c0 + c1 * xI) + c2 * (xI * xI);
+    vO = (c0 + c1 * xO) + c2 * (xO * xO);
+    if (xE > xI && xE <= xC) {
+      D = (x2AC - xO * xO) + (2 * xI) * (xO - xAC);
+      c0 = ((vO * x2AC - vC * (xO * xO)) + (2 * xI) * (vC * xO - vO * xAC)) / D;
+      c1 = ((-2 * (vC - vO)) * xI) / D;
+      c2 = (vC - vO) / D;
+      vI = (c0 + c1 * xI) + c2 * (xI * xI);
+    } else if (xE > xC && xE < xO) {
+      D = (x2AC - xI * xI) + (2 * xO) * (xI - xAC);
+      c0 = ((vI * x2AC - vC * (xI * xI)) + (2 * xO) * (vC * xI - vI * xAC)) / D;
+      c1 = ((-2 * (vC - vI)) * xO) / D;
+      c2 = (vC - vI) / D;
+      vO = (c0 + c1 * xO) + c2 * (xO * xO);
+    }
+  }
+  out[0] = vI;
+  out[1] = vO;
+}
+
+/* The parabolic reconstruction of dimension d: cells p_d in [NG-1, m_d-NG]
+ * (the proper cells and one ghost on each side), transverse interior;
+ * V_IR(p) = vI, V_IL(p+e_d) = vO. */
+static void reconstruct_parabolic(gxo_ctx *c, int d) {
+  int lo[3], hi[3];
+  for (int e = 0; e < 3; e++) { lo[e] = NG; hi[e] = c->m[e] - NG; }
+  lo[d] = NG - 1;
+  hi[d] = c->m[d] - NG + 1;
+  long s = (d == 0) ? 1 : (d == 1) ? c->m[0] : (long)c->m[0] * c->m[1];
+  const double *X = c->center[d], *WL = c->wl[d], *WR = c->wr[d];
+  for (int f = 0; f < 5; f++) {
+    const double *V = c->cur[IA_PRIMITIVE[f]];
+    double *VL = c->il[IA_PRIMITIVE[f]], *VR = c->ir[IA_PRIMITIVE[f]];
+    LOOP3(c, lo, hi) {
+      int p[3] = {p0, p1, p2};
+      long iv = IDX(c, p0, p1, p2);
+      const int q = p[d];
+      double out[2];
+      gxo_parabolic_face_values(V[iv - s], V[iv], V[iv + s], X[q - 1], X[q], X[q + 1],
+                                WL[q], WR[q], out);
+      VR[iv] = out[0];
+      VL[iv + s] = out[1];
+    }
+  }
+}
+
 /* ComputeReconstructionLinear_CSL_Kernel (EV/Increments/IncrementDivergence_
  * FV__Kernel.f90:11-94): V_IL(p) = V(p-e) + dX_R(p-e)*dVdX(p-e),
  * V_IR(p) = V(p) - dX_L(p)*dVdX(p); p_d in [NG, m_d-NG], transverse interior. */
@@ -643,8 +746,12 @@ static void record_fluence(gxo_ctx *c, double dt, int d) {
 static void increment_divergence(gxo_ctx *c, double **K, double dt) {
   for (int d = 0; d < 3; d++) {
     apply_bc(c, d);
-    gradient(c, d);
-    reconstruct(c, d);
+    if (c->parabolic) {
+      reconstruct_parabolic(c, d);
+    } else {
+      gradient(c, d);
+      reconstruct(c, d);
+    }
     int lo[3], hi[3];
     face_range(c, d, lo, hi);
     LOOP3(c, lo, hi) {

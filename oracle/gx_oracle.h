@@ -63,6 +63,10 @@ gxo_ctx *gxo_create(const int n[3], const double lo[3], const double hi[3],
                     double courant, double n_min, int n_threads);
 void gxo_destroy(gxo_ctx *c);
 
+/* ReconstructionType: 0 'LINEAR' (default), 1 'PARABOLIC'
+ * (EV/Increments/IncrementDivergence_FV__Form.f90:239-241, 637-655). */
+void gxo_set_reconstruction(gxo_ctx *c, int parabolic);
+
 /* Padded extents and value count of one field. */
 void gxo_extents(const gxo_ctx *c, int m[3], long *n_values);
 
@@ -121,6 +125,14 @@ void gxo_surface_integral(const gxo_ctx *c, double *out);
  * Gradient_Kernel.f90:166-190). */
 double gxo_limited_slope(double dV_L, double dV_R, double dX_L, double dX_R,
                          double theta);
+
+/* Parabolic face values of one cell (EV/Increments/IncrementDivergence_FV__
+ * Kernel.f90:296-596): out[0] = value at the inner face xC - dX_L, out[1] at
+ * the outer face xC + dX_R, from the values vM, vC, vP at the centres xM, xC,
+ * xP. */
+void gxo_parabolic_face_values(double vM, double vC, double vP, double xM,
+                               double xC, double xP, double dX_L, double dX_R,
+                               double out[2]);
 
 /* HLL(+HLLC) flux of one face from reconstructed primitive states
  * (N, V1, V2, V3, E) on the left and right, sweep dimension d (0..2),

@@ -1,0 +1,112 @@
+"""PARABOLIC reconstruction on the GPU against the oracle (ReconstructionType
+'PARABOLIC', EV/Increments/IncrementDivergence_FV__Kernel.f90:296-596): FP64
+states, CFL candidates, shock flags and boundary fluence identical bit for
+bit after every step; FP32 close to FP64; batches and the adjoint refuse it."""
+import numpy as np
+import pytest
+
+from genasis_amd._capi import GxError
+from genasis_amd.adjoint import DiscreteAdjoint
+from genasis_amd.batch import Batch_RK2_C_ASC
+from genasis_amd.chart import Chart
+from genasis_amd.compare import field_errors
+from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
+from genasis_amd import problems
+
+from ._helpers import make_oracle, oracle_conserved, oracle_flag_bits
+
+pytestmark = pytest.mark.gpu
+
+OUT = (("OUTFLOW", "OUTFLOW"),) * 3
+PARA = "PARABOLIC"
+
+
+def _pair(chart, prim, riemann="HLLC"):
+    o = make_oracle(chart, prim, courant=0.8, riemann=riemann, reconstruction=PARA)
+    g = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, RiemannSolverType=riemann,
+                                              ReconstructionType=PARA))
+    g.set_primitive(*prim)
+    return o, g
+
+
+def _lockstep(o, g, chart, n_steps, tag, fluence=False):
+    for s in range(n_steps):
+        c_o, c_g = o.time_step_candidate()[0], g.time_step_candidate()
+        assert c_o == c_g, f"{tag} step {s}: candidate {c_o!r} vs {c_g!r}"
+        dt = min((s + 1) / 100.0, 1.0) * c_o
+        o.step(dt)
+        g.step(0.0, dt)
+        uo, ug = oracle_conserved(o, chart), g.conserved()
+        if not np.array_equal(uo, ug):
+            bad = np.argwhere(uo != ug)[:5].tolist()
+            raise AssertionError(f"{tag} step {s}: {int((uo != ug).sum())} values differ, {bad}")
+        assert np.array_equal(oracle_flag_bits(o, chart), g.flags()), f"{tag} step {s}: flags"
+        if fluence:
+            faces = g.boundary_fluence()
+            for b in range(6):
+                assert np.array_equal(faces[b], o.boundary_fluence(b)), f"{tag} step {s} face {b}"
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_sod_bitwise(axis):
+    n = [8, 8, 8]
+    n[axis] = 64
+    chart = Chart(tuple(n), bc=OUT)
+    o, g = _pair(chart, problems.sod(chart, axis=axis))
+    _lockstep(o, g, chart, 12, f"sod {axis}", fluence=True)
+
+
+@pytest.mark.parametrize("riemann", ["HLLC", "HLL"])
+def test_random_state_mixed_boundaries_bitwise(riemann):
+    bc = (("REFLECTING", "OUTFLOW"), ("PERIODIC", "PERIODIC"), ("OUTFLOW", "REFLECTING"))
+    chart = Chart((37, 11, 19), bc=bc)
+    rng = np.random.default_rng(8)
+    shape = chart.m[::-1]
+    x, _, _ = chart.padded_centers()
+    N = np.exp(rng.uniform(np.log(0.3), np.log(3.0), shape)) * (1 + 2 * (x > 0.5))
+    V = [rng.uniform(-0.6, 0.6, shape) for _ in range(3)]
+    E = np.exp(rng.uniform(np.log(0.3), np.log(3.0), shape)) / 0.4 * (1 + 5 * (x > 0.5))
+    o, g = _pair(chart, (N, V[0], V[1], V[2], E), riemann)
+    g.set_tuning(5)  # partial tiles and short z-chunks
+    _lockstep(o, g, chart, 8, f"random {riemann}", fluence=True)
+    to, tg = o.compute_tally(), g.compute_tally()
+    for key in ("interior", "boundary", "total", "change"):
+        scale = max(np.abs(to["interior"]).max(), np.abs(to["boundary"]).max())
+        assert np.abs(to[key] - tg[key]).max() <= 1e-13 * scale, key
+
+
+def test_kelvin_helmholtz_bitwise():
+    chart = Chart((24, 24, 16))
+    prim = problems.kelvin_helmholtz(chart, seed=7)
+    o, g = _pair(chart, prim)
+    _lockstep(o, g, chart, 10, "kh")
+
+
+def test_fp32_close_to_fp64():
+    chart = Chart((48, 48, 32))
+    prim = problems.kelvin_helmholtz(chart, seed=7, padded=False)
+    runs, dts = {}, []
+    for prec in ("FP64", "FP32"):
+        g = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, Precision=prec,
+                                                  ReconstructionType=PARA))
+        g.set_primitive(*prim)
+        for s in range(20):
+            if prec == "FP64":
+                dts.append(min((s + 1) / 100.0, 1.0) * g.time_step_candidate())
+            g.step(0.0, dts[s])
+        runs[prec] = g.conserved()
+    err = field_errors(runs["FP32"], runs["FP64"], ("D", "S_1", "S_2", "S_3", "G"))
+    assert err["D"]["L1"] <= 1e-5, err["D"]
+    for f in ("S_1", "S_2", "S_3", "G"):
+        assert err[f]["L1"] <= 1e-5 * max(1.0, np.abs(runs["FP64"]).max()), (f, err[f])
+
+
+def test_batches_and_adjoint_refuse_parabolic():
+    chart = Chart((8, 8, 8))
+    params = FluidParameters(ReconstructionType=PARA)
+    with pytest.raises(GxError, match="GX_LINEAR"):
+        Batch_RK2_C_ASC([chart], params)
+    g = Step_RK2_C_ASC(chart, params)
+    g.set_primitive(*problems.kelvin_helmholtz(chart, seed=1, padded=False))
+    with pytest.raises(GxError, match="GX_LINEAR"):
+        DiscreteAdjoint(g).forward(2, 1)
