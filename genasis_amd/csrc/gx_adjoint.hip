@@ -285,6 +285,293 @@ __global__ __launch_bounds__(AB) void adj_gather_kernel(StageArgs<double> a, Pad
   }
 }
 
+// ---- fused transposed residual, one wave per line segment -------------------
+//
+// A wave holds 64 consecutive cells c = c0 - 3 + lane of a line along D
+// (cells -2 .. n+1 exist; others are zero).  Neighbour values come by lane
+// shuffles: the slope of each cell, the face (c-1 | c) of each lane with its
+// flux's reverse sweep, the slope bar s_b(c) = wr plus - wl minus and the
+// limiter's reverse, and the cell bar: own face-state bars plus the slope
+// partials of cells c-1, c, c+1.  Lanes 3 .. 60 (58 cells) are complete.  It
+// is the slope, face and gather passes of the reverse in one step, with the
+// face bars never leaving registers.  A face counts into the dt and gamma
+// bars in the segment that owns its cell c.
+constexpr int LINE_OUT = 58;  // complete cells per wave
+
+template <int D>
+__device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int c, int lane,
+                                         bool own, double A, double dt, const double V[5],
+                                         const double Kb[5], bool df, double volc,
+                                         double volm, double Vb[5], double &dtb,
+                                         double &gbs) {
+  double Vm[5], Vq[5], s[5], sm[5], Kbm[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    Vm[f] = __shfl_up(V[f], 1, WAVE);
+    Vq[f] = __shfl_down(V[f], 1, WAVE);
+  }
+  const bool has_slope = c >= -1 && c <= n && lane >= 1 && lane <= 62;
+  {
+    const double dxl = has_slope ? geo(a, D, G_DXL, c + 2) : 1.0;
+    const double dxr = has_slope ? geo(a, D, G_DXR, c + 2) : 1.0;
+    const double dxl2 = has_slope ? geo(a, D, G_DXL2, c + 2) : 1.0;
+    const double dxr2 = has_slope ? geo(a, D, G_DXR2, c + 2) : 1.0;
+    const double den = has_slope ? geo(a, D, G_DEN, c + 2) : 1.0;
+    const double rdxl = has_slope ? geo(a, D, G_RDXL, c + 2) : 1.0;
+    const double rdxr = has_slope ? geo(a, D, G_RDXR, c + 2) : 1.0;
+    const double rden = has_slope ? geo(a, D, G_RDEN, c + 2) : 1.0;
+#pragma unroll
+    for (int f = 0; f < 5; f++)
+      s[f] = has_slope ? slope(Vm[f], V[f], Vq[f], dxl, dxr, dxl2, dxr2, den, rdxl, rdxr, rden,
+                               a.ph.theta, a.ph.use_limiter)
+                       : 0.0;
+  }
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    sm[f] = __shfl_up(s[f], 1, WAVE);
+    Kbm[f] = __shfl_up(Kb[f], 1, WAVE);
+  }
+  // face (c-1 | c)
+  double WLb[5] = {0, 0, 0, 0, 0}, WRb[5] = {0, 0, 0, 0, 0};
+  if (c >= 0 && c <= n && lane >= 2 && lane <= 62) {
+    const double wr = geo(a, D, G_WR, c + 1), wl = geo(a, D, G_WL, c + 2);
+    double WL[5], WR[5], Gf[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      WL[f] = Vm[f] + wr * sm[f];
+      WR[f] = V[f] - wl * s[f];
+      Gf[f] = 0.0;
+    }
+    if (c < n) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) Gf[f] += A * Kb[f] / volc;
+    }
+    if (c >= 1) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) Gf[f] -= A * Kbm[f] / volm;
+    }
+    double Fb[5], F[5], gl = 0.0;
+#pragma unroll
+    for (int f = 0; f < 5; f++) Fb[f] = dt * Gf[f];
+    adj::face_flux_vjp(WL, WR, D, df && c < n, a.ph.gamma, a.ph.mass, a.ph.hllc != 0, Fb, WLb,
+                       WRb, gl, F);
+    if (own) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) dtb += F[f] * Gf[f];
+      gbs += gl;
+    }
+  }
+  // slope bar of cell c and the limiter's reverse onto c-1, c, c+1
+  double dm[5], d0[5], dp[5], up[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    up[f] = __shfl_down(WLb[f], 1, WAVE);  // bar of c as the left state of face c+1
+    dm[f] = d0[f] = dp[f] = 0.0;
+  }
+  if (has_slope && lane >= 2 && lane <= 61) {
+    const int q = c + 2;
+    const double wr = geo(a, D, G_WR, q), wl = geo(a, D, G_WL, q);
+    const double dxl = geo(a, D, G_DXL, q), dxr = geo(a, D, G_DXR, q);
+    const double dxl2 = geo(a, D, G_DXL2, q), dxr2 = geo(a, D, G_DXR2, q);
+    const double den = geo(a, D, G_DEN, q);
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      double sb = 0.0;
+      if (c <= n - 1) sb += wr * up[f];
+      if (c >= 0) sb -= wl * WRb[f];
+      adj::slope_vjp(Vm[f], V[f], Vq[f], dxl, dxr, dxl2, dxr2, den, a.ph.theta,
+                     a.ph.use_limiter, sb, dm[f], d0[f], dp[f]);
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    const double from_left = __shfl_up(dp[f], 1, WAVE);    // cell c-1's partial on c
+    const double from_right = __shfl_down(dm[f], 1, WAVE); // cell c+1's partial on c
+    double acc = 0.0;
+    if (c >= -1 && c <= n - 1) acc += up[f];
+    if (c >= 0 && c <= n) acc += WRb[f];
+    Vb[f] = ((acc + from_left) + d0[f]) + from_right;
+  }
+}
+
+// D = 0: one wave per segment of a row (lanes along x, coalesced).
+__global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, PadGeom g,
+                                                        const double *Vp, const double *Kb,
+                                                        double dt, double *Vb,
+                                                        double *partial) {
+  __shared__ double sh[AB];
+  const int n = g.n[0];
+  const int nseg = (n + 4 + LINE_OUT - 1) / LINE_OUT;
+  const int lane = threadIdx.x % WAVE;
+  const long long w = blockIdx.x * (long long)(AB / WAVE) + threadIdx.x / WAVE;
+  const long long nw = (long long)g.n[1] * g.n[2] * nseg;
+  double dtb = 0.0, gbs = 0.0;
+  if (w < nw) {  // uniform over the wave
+    const int sgi = (int)(w % nseg);
+    const long long r = w / nseg;
+    const int y = (int)(r % g.n[1]), z = (int)(r / g.n[1]);
+    const int c0 = -2 + sgi * LINE_OUT;
+    const int c = c0 - 3 + lane;
+    const long long pl = plin(g, 0, y, z);
+    double V[5], K[5];
+    bool df = false;
+    double volc = 0.0, volm = 0.0;
+    const double wy = cell_width(a, 1, y), wz = cell_width(a, 2, z);
+    const double A = wy * wz;
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      V[f] = (c >= -2 && c <= n + 1) ? Vp[f * g.np + pl + c] : 0.0;
+      K[f] = 0.0;
+    }
+    if (c >= 0 && c < n) {
+      const unsigned ic = lin(a.n, c, y, z);
+#pragma unroll
+      for (int f = 0; f < 5; f++) K[f] = Kb[f * a.fs + ic];
+      df = (a.flags[ic] >> 0) & 1;
+      volc = cell_width(a, 0, c) * wy * wz;
+    }
+    if (c >= 1 && c <= n) volm = cell_width(a, 0, c - 1) * wy * wz;
+    const bool own = lane >= 3 && lane <= 60 && c <= n + 1;
+    double out[5];
+    line_vjp<0>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
+    if (own) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) Vb[f * g.np + pl + c] += out[f];
+    }
+  }
+  const double s1 = block_sum(dtb, sh);
+  const double s2 = block_sum(gbs, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s1;
+    partial[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// D = 1, 2: TRW columns of 64 cells along D per block, loaded and stored
+// along x through LDS (64-byte row pieces); each of the four waves works on
+// two columns in turn.
+constexpr int TRW = 8;          // columns (x) per block
+constexpr int TRP = TRW + 1;    // padded LDS row
+constexpr int TRWAVES = 4;      // waves per block
+template <int D>
+__global__ __launch_bounds__(TRWAVES * WAVE) void adj_line_yz_kernel(StageArgs<double> a,
+                                                                      PadGeom g,
+                                                                      const double *Vp,
+                                                                      const double *Kb,
+                                                                      double dt, double *Vb,
+                                                                      double *partial) {
+  constexpr int NTH = TRWAVES * WAVE;
+  constexpr int O = D == 1 ? 2 : 1;  // the other transverse dimension
+  __shared__ double sV[5][WAVE][TRP];
+  __shared__ double sK[5][WAVE][TRP];
+  __shared__ unsigned char sF[WAVE][TRP];
+  __shared__ double sh[NTH];
+  const int n = g.n[D];
+  const int nseg = (n + 4 + LINE_OUT - 1) / LINE_OUT;
+  const int nxb = (g.n[0] + TRW - 1) / TRW;
+  // block -> (x block, segment, other coordinate), XCD-aware: neighbouring
+  // x blocks share the cache lines of their rows and run on one L2
+  const int b = tile_of_block(blockIdx.x, gridDim.x);
+  const int xb = b % nxb, sgi = (b / nxb) % nseg, o = b / (nxb * nseg);
+  const int x0 = xb * TRW, c0 = -2 + sgi * LINE_OUT;
+  const int tid = threadIdx.x;
+  auto pidx = [&](int x, int c) {
+    int q[3];
+    q[0] = x;
+    q[D] = c;
+    q[O] = o;
+    return plin(g, q[0], q[1], q[2]);
+  };
+  auto lidx = [&](int x, int c) {
+    int q[3];
+    q[0] = x;
+    q[D] = c;
+    q[O] = o;
+    return lin(a.n, q[0], q[1], q[2]);
+  };
+  // cooperative loads: item -> (x = item % TRW, lane row l = item / TRW)
+  for (int item = tid; item < TRW * WAVE; item += NTH) {
+    const int xi = item % TRW, l = item / TRW;
+    const int x = x0 + xi, c = c0 - 3 + l;
+    const bool okx = x < g.n[0];
+    const bool okV = okx && c >= -2 && c <= n + 1;
+    const bool okK = okx && c >= 0 && c < n;
+    const long long pv = okV ? pidx(x, c) : 0;
+    const unsigned ik = okK ? lidx(x, c) : 0u;
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      sV[f][l][xi] = okV ? Vp[f * g.np + pv] : 0.0;
+      sK[f][l][xi] = okK ? Kb[f * a.fs + ik] : 0.0;
+    }
+    sF[l][xi] = okK ? (unsigned char)((a.flags[ik] >> D) & 1) : (unsigned char)0;
+  }
+  __syncthreads();
+  const int wv = tid / WAVE, lane = tid % WAVE;
+  const int c = c0 - 3 + lane;
+  double dtb = 0.0, gbs = 0.0;
+  const double wo = cell_width(a, O, o);
+  for (int col = wv; col < TRW; col += TRWAVES) {
+    const int x = x0 + col;
+    double out[5] = {0, 0, 0, 0, 0};
+    if (x < g.n[0]) {  // uniform over the wave
+      double V[5], K[5];
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        V[f] = sV[f][lane][col];
+        K[f] = sK[f][lane][col];
+      }
+      const bool df = sF[lane][col] != 0;
+      const double wx = cell_width(a, 0, x);
+      const double A = D == 1 ? wo * wx : wx * wo;  // D=1: wz*wx; D=2: wx*wy
+      double volc = 0.0, volm = 0.0;
+      auto vol_of = [&](int cc) {
+        const double wd = cell_width(a, D, cc);
+        return D == 1 ? (wx * wd) * wo : (wx * wo) * wd;
+      };
+      if (c >= 0 && c < n) volc = vol_of(c);
+      if (c >= 1 && c <= n) volm = vol_of(c - 1);
+      const bool own = lane >= 3 && lane <= 60 && c <= n + 1;
+      line_vjp<D>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
+      if (!own) {
+#pragma unroll
+        for (int f = 0; f < 5; f++) out[f] = 0.0;
+      }
+    }
+    // this wave alone reads and writes column col
+#pragma unroll
+    for (int f = 0; f < 5; f++) sV[f][lane][col] = out[f];
+  }
+  __syncthreads();
+  for (int item = tid; item < TRW * WAVE; item += NTH) {
+    const int xi = item % TRW, l = item / TRW;
+    const int xx = x0 + xi, cc = c0 - 3 + l;
+    if (xx < g.n[0] && l >= 3 && l <= 60 && cc <= n + 1) {
+      const long long pv = pidx(xx, cc);
+#pragma unroll
+      for (int f = 0; f < 5; f++) Vb[f * g.np + pv] += sV[f][l][xi];
+    }
+  }
+  // block sums of the dt and gamma bars
+  sh[tid] = dtb;
+  __syncthreads();
+  for (int s2 = NTH / 2; s2 > 0; s2 >>= 1) {
+    if (tid < s2) sh[tid] += sh[tid + s2];
+    __syncthreads();
+  }
+  const double t1 = sh[0];
+  __syncthreads();
+  sh[tid] = gbs;
+  __syncthreads();
+  for (int s2 = NTH / 2; s2 > 0; s2 >>= 1) {
+    if (tid < s2) sh[tid] += sh[tid + s2];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    partial[2 * blockIdx.x] = t1;
+    partial[2 * blockIdx.x + 1] = sh[0];
+  }
+}
+
 // Interior cells: the primitive bar of the cell plus the bars of the ghost
 // values copied from it (normal velocity negated across a REFLECTING face),
 // then the conserved -> primitive reverse (or the primitive bar itself for an
@@ -572,6 +859,17 @@ static int launch_stage2(gx_ctx *c, const double *Y, const double *W, double *ou
   return GX_OK;
 }
 
+// Blocks of the line kernel of dimension d.
+static int line_blocks(const PadGeom &g, int d) {
+  const long long nseg = (g.n[d] + 4 + LINE_OUT - 1) / LINE_OUT;
+  if (d == 0) {
+    const long long waves = (long long)g.n[1] * g.n[2] * nseg;
+    return (int)((waves + AB / WAVE - 1) / (AB / WAVE));
+  }
+  const long long nxb = (g.n[0] + TRW - 1) / TRW;
+  return (int)(nxb * nseg * g.n[d == 1 ? 2 : 1]);
+}
+
 // Transposed residual: out += (dt R(src))^T Kb; dt and gamma bar partials at
 // part (3 face passes).
 static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, double dt,
@@ -595,13 +893,17 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, 
     }
     const int fb = (int)((nf + AB - 1) / AB);
     double *pp = part + 2 * (size_t)d * A->part_stride;
+    (void)fb;
+    (void)ng;
+    (void)ns;
     auto pass = [&](auto dim) {
       constexpr int D = decltype(dim)::value;
-      adj_slope_kernel<D><<<blocks_for(ns), AB, 0, c->stream>>>(a, g, A->Vp, A->S, A->Dp);
-      adj_face_kernel<D><<<fb, AB, 0, c->stream>>>(a, g, A->Vp, A->S, Kb, dt, A->plus,
-                                                   A->minus, pp);
-      adj_gather_kernel<D><<<blocks_for(ng), AB, 0, c->stream>>>(a, g, A->Dp, A->plus,
-                                                                 A->minus, A->Vb);
+      const int nb = line_blocks(g, D);
+      if (D == 0)
+        adj_line_x_kernel<<<nb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb, pp);
+      else
+        adj_line_yz_kernel<D == 0 ? 1 : D><<<nb, TRWAVES * WAVE, 0, c->stream>>>(
+            a, g, A->Vp, Kb, dt, A->Vb, pp);
     };
     if (d == 0)
       pass(std::integral_constant<int, 0>());
@@ -652,6 +954,7 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
     maxf = std::max(maxf, nf);
   }
   A->part_stride = (int)((maxf + AB - 1) / AB);
+  for (int d = 0; d < 3; d++) A->part_stride = std::max(A->part_stride, line_blocks(g, d));
   HIP_TRY(hipMalloc(&A->partials, sizeof(double) * 2 * 6 * (size_t)A->part_stride));
   HIP_TRY(hipMemset(A->partials, 0, sizeof(double) * 2 * 6 * (size_t)A->part_stride));
   HIP_TRY(hipMalloc(&A->gb, sizeof(double)));
