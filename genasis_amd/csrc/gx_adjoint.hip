@@ -10,14 +10,14 @@
 //   U' = fix(Z), Z = W + K2/2, K2 = dt R(Y), Y = U + K1, W = U + K1/2,
 //   K1 = dt R(U or the uploaded primitives), dt = dt(CFL maximum of U),
 //
-// where R(S) = -div F / Vol of the stage input S.  The transpose of R is one
-// face pass per dimension (the face flux's reverse sweep, writing the bars
-// of the two reconstructed face states), a gather pass (reconstruction and
-// limiter reverse), a fold of the ghost bars onto the cells they copy, and
-// the conserved -> primitive reverse.  Work arrays that take ghosts use the
-// padded layout (n + 4 per dimension).  Shock flags are recomputed from each
-// step's input and replayed as constants; dt and gamma bars are reduced per
-// block in a fixed order.
+// where R(S) = -div F / Vol of the stage input S.  The transpose of R is a
+// primitives pass onto the padded layout (n + 4 per dimension, ghosts by the
+// ghost rule), one line pass per dimension (slopes, the face flux's reverse
+// sweep and the limiter's reverse along lines, neighbours by lane shuffles),
+// a fold of the ghost bars onto the cells they copy, and the conserved ->
+// primitive reverse.  Shock flags are recomputed from each step's input and
+// replayed as constants; dt and gamma bars are reduced per block, then in
+// two levels, in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -59,13 +59,12 @@ struct AdjointState {
   double *seg = nullptr, *segY = nullptr, *segW = nullptr, *segZ = nullptr;
   uint8_t *segF = nullptr;
   double *trash = nullptr;
-  double *Vp = nullptr, *Vb = nullptr, *plus = nullptr, *minus = nullptr;  // 5 np
-  double *S = nullptr;         // slopes, 5 np
-  double *Dp = nullptr;        // slope partials d s / d(V_{j-1}, V_j, V_{j+1}), 15 np
+  double *Vp = nullptr, *Vb = nullptr;  // primitives and their bars, 5 np
   double *Ub = nullptr, *Yb = nullptr, *Wb = nullptr, *K1b = nullptr, *K2b = nullptr,
          *P0b = nullptr;       // 5 ncell
-  double *partials = nullptr;  // 2 per face block, both stages, three dimensions
-  int part_stride = 0;         // blocks of the largest face pass
+  double *partials = nullptr;  // 2 per line block, both stages, three dimensions
+  int part_stride = 0;         // blocks of the largest line pass
+  double *partials2 = nullptr; // 2 per block of the first reduction level
   double *gb = nullptr;        // gamma bar
   int *argmax = nullptr;
   uint8_t *flags = nullptr;
@@ -118,173 +117,6 @@ __device__ inline double cell_width(const StageArgs<double> &a, int e, int c) {
   return geo(a, e, G_WIDTH, c + 2);
 }
 
-// Slope pass of dimension D over cells -1 .. n along D (transverse
-// interior): the forward's limited slope of every field and its three
-// partial derivatives (the limiter's branch, replayed).
-template <int D>
-__global__ __launch_bounds__(AB) void adj_slope_kernel(StageArgs<double> a, PadGeom g,
-                                                       const double *Vp, double *S,
-                                                       double *Dp) {
-  const int e0 = D == 0 ? g.n[0] + 2 : g.n[0];
-  const int e1 = D == 1 ? g.n[1] + 2 : g.n[1];
-  const int e2 = D == 2 ? g.n[2] + 2 : g.n[2];
-  const long long nc = (long long)e0 * e1 * e2;
-  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < nc;
-       t += (long long)gridDim.x * AB) {
-    int c3[3];
-    c3[0] = (int)(t % e0);
-    c3[1] = (int)((t / e0) % e1);
-    c3[2] = (int)(t / ((long long)e0 * e1));
-    c3[D] -= 1;
-    const int q = c3[D] + 2;  // padded geometry index
-    const long long sd = g.s[D];
-    const long long p = plin(g, c3[0], c3[1], c3[2]);
-    const double dxl = geo(a, D, G_DXL, q), dxr = geo(a, D, G_DXR, q);
-    const double dxl2 = geo(a, D, G_DXL2, q), dxr2 = geo(a, D, G_DXR2, q);
-    const double den = geo(a, D, G_DEN, q);
-    const double rdxl = geo(a, D, G_RDXL, q), rdxr = geo(a, D, G_RDXR, q);
-    const double rden = geo(a, D, G_RDEN, q);
-#pragma unroll
-    for (int f = 0; f < 5; f++) {
-      const double *v = Vp + f * g.np + p;
-      const double vm = v[-sd], v0 = v[0], vp = v[sd];
-      S[f * g.np + p] = slope(vm, v0, vp, dxl, dxr, dxl2, dxr2, den, rdxl, rdxr, rden,
-                              a.ph.theta, a.ph.use_limiter);
-      double dm = 0.0, d0 = 0.0, dp = 0.0;
-      adj::slope_vjp(vm, v0, vp, dxl, dxr, dxl2, dxr2, den, a.ph.theta, a.ph.use_limiter, 1.0,
-                     dm, d0, dp);
-      Dp[(3 * f) * g.np + p] = dm;
-      Dp[(3 * f + 1) * g.np + p] = d0;
-      Dp[(3 * f + 2) * g.np + p] = dp;
-    }
-  }
-}
-
-// Face pass of dimension D: the reverse of the face flux for every face of
-// the sweep (faces 0 .. n_D between cells i-1 and i, transverse interior).
-// Fb = dt * A * (Kb_i / Vol_i - Kb_{i-1} / Vol_{i-1}); writes the bar of the
-// left state to plus[i-1] and of the right state to minus[i], and per block
-// the sums of F . Fb / dt (the dt bar) and of the gamma bar.
-template <int D>
-__global__ __launch_bounds__(AB, 3) void adj_face_kernel(StageArgs<double> a, PadGeom g,
-                                                      const double *Vp, const double *S,
-                                                      const double *Kb, double dt,
-                                                      double *plus, double *minus,
-                                                      double *partial) {
-  __shared__ double sh[AB];
-  const int e0 = g.n[0] + (D == 0), e1 = g.n[1] + (D == 1), e2 = g.n[2] + (D == 2);
-  const long long nf = (long long)e0 * e1 * e2;
-  const long long t = blockIdx.x * (long long)AB + threadIdx.x;
-  double dtb = 0.0, gb = 0.0;
-  if (t < nf) {
-    int c3[3];
-    c3[0] = (int)(t % e0);
-    c3[1] = (int)((t / e0) % e1);
-    c3[2] = (int)(t / ((long long)e0 * e1));
-    const int i = c3[D], n = g.n[D];
-    const long long sd = g.s[D];
-    const long long p0 = plin(g, c3[0], c3[1], c3[2]);
-    const double wr = geo(a, D, G_WR, i + 1), wl = geo(a, D, G_WL, i + 2);
-    double WL[5], WR[5];
-#pragma unroll
-    for (int f = 0; f < 5; f++) {
-      const long long o = f * g.np + p0;
-      WL[f] = Vp[o - sd] + wr * S[o - sd];
-      WR[f] = Vp[o] - wl * S[o];
-    }
-    bool df = false;
-    if (i < n) df = (a.flags[lin(a.n, c3[0], c3[1], c3[2])] >> D) & 1;
-    double Gf[5] = {0, 0, 0, 0, 0};
-    const double wx = cell_width(a, 0, c3[0]), wy = cell_width(a, 1, c3[1]),
-                 wz = cell_width(a, 2, c3[2]);
-    if (i < n) {
-      const double vol = wx * wy * wz;
-      const double A = D == 0 ? wy * wz : (D == 1 ? wz * wx : wx * wy);
-      const unsigned ic = lin(a.n, c3[0], c3[1], c3[2]);
-#pragma unroll
-      for (int f = 0; f < 5; f++) Gf[f] += A * Kb[f * a.fs + ic] / vol;
-    }
-    if (i >= 1) {
-      int q[3] = {c3[0], c3[1], c3[2]};
-      q[D] = i - 1;
-      const double vx = cell_width(a, 0, q[0]), vy = cell_width(a, 1, q[1]),
-                   vz = cell_width(a, 2, q[2]);
-      const double vol = vx * vy * vz;
-      const double A = D == 0 ? vy * vz : (D == 1 ? vz * vx : vx * vy);
-      const unsigned ic = lin(a.n, q[0], q[1], q[2]);
-#pragma unroll
-      for (int f = 0; f < 5; f++) Gf[f] -= A * Kb[f * a.fs + ic] / vol;
-    }
-    double Fb[5], F[5], WLb[5] = {0, 0, 0, 0, 0}, WRb[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-    for (int f = 0; f < 5; f++) Fb[f] = dt * Gf[f];
-    adj::face_flux_vjp(WL, WR, D, df, a.ph.gamma, a.ph.mass, a.ph.hllc != 0, Fb, WLb, WRb,
-                       gb, F);
-#pragma unroll
-    for (int f = 0; f < 5; f++) {
-      dtb += F[f] * Gf[f];
-      plus[f * g.np + p0 - sd] = WLb[f];
-      minus[f * g.np + p0] = WRb[f];
-    }
-  }
-  const double s1 = block_sum(dtb, sh);
-  const double s2 = block_sum(gb, sh);
-  if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = s1;
-    partial[2 * blockIdx.x + 1] = s2;
-  }
-}
-
-// Gather pass of dimension D over cells -2 .. n+1 along D (transverse
-// interior): Vb += bars of the cell's own face states, and the slope partials
-// of cells c-1, c, c+1 with respect to cell c times their slope bars
-// (s_j bar = wr_j plus_j - wl_j minus_j).
-template <int D>
-__global__ __launch_bounds__(AB) void adj_gather_kernel(StageArgs<double> a, PadGeom g,
-                                                        const double *Dp, const double *plus,
-                                                        const double *minus, double *Vb) {
-  const int e0 = D == 0 ? g.n[0] + 4 : g.n[0];
-  const int e1 = D == 1 ? g.n[1] + 4 : g.n[1];
-  const int e2 = D == 2 ? g.n[2] + 4 : g.n[2];
-  const long long nc = (long long)e0 * e1 * e2;
-  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < nc;
-       t += (long long)gridDim.x * AB) {
-    int c3[3];
-    c3[0] = (int)(t % e0);
-    c3[1] = (int)((t / e0) % e1);
-    c3[2] = (int)(t / ((long long)e0 * e1));
-    c3[D] -= 2;
-    const int c = c3[D], n = g.n[D];
-    const long long sd = g.s[D];
-    const long long p = plin(g, c3[0], c3[1], c3[2]);
-    double acc[5] = {0, 0, 0, 0, 0};
-    const bool hasp = c >= -1 && c <= n - 1, hasm = c >= 0 && c <= n;
-#pragma unroll
-    for (int f = 0; f < 5; f++) {
-      if (hasp) acc[f] += plus[f * g.np + p];
-      if (hasm) acc[f] += minus[f * g.np + p];
-    }
-#pragma unroll
-    for (int o = -1; o <= 1; o++) {
-      const int j = c + o;
-      if (j < -1 || j > n) continue;
-      const long long pj = p + o * sd;
-      const int q = j + 2;
-      const double wr = geo(a, D, G_WR, q), wl = geo(a, D, G_WL, q);
-      const int role = 1 - o;  // c is j-1 (0), j (1) or j+1 (2)
-#pragma unroll
-      for (int f = 0; f < 5; f++) {
-        double sb = 0.0;
-        if (j <= n - 1) sb += wr * plus[f * g.np + pj];
-        if (j >= 0) sb -= wl * minus[f * g.np + pj];
-        acc[f] += Dp[(3 * f + role) * g.np + pj] * sb;
-      }
-    }
-#pragma unroll
-    for (int f = 0; f < 5; f++) Vb[f * g.np + p] += acc[f];
-  }
-}
-
 // ---- fused transposed residual, one wave per line segment -------------------
 //
 // A wave holds 64 consecutive cells c = c0 - 3 + lane of a line along D
@@ -297,6 +129,49 @@ __global__ __launch_bounds__(AB) void adj_gather_kernel(StageArgs<double> a, Pad
 // face bars never leaving registers.  A face counts into the dt and gamma
 // bars in the segment that owns its cell c.
 constexpr int LINE_OUT = 58;  // complete cells per wave
+
+// The limiter's reverse with the geometry reciprocals: the forward values
+// that pick the branch (central slope, the two one-sided quotients) are the
+// forward's own exactly rounded quotients (div_rcp); the bars multiply by
+// the reciprocals (a gradient needs no exactly rounded quotient).
+__device__ __forceinline__ void slope_vjp_r(double vm, double v0, double vp, double dxl2,
+                                            double dxr2, double dxl, double dxr, double den,
+                                            double rdxl, double rdxr, double rden,
+                                            double theta, int use_limiter, double sb,
+                                            double &vmb, double &v0b, double &vpb) {
+  const double dVL = v0 - vm, dVR = vp - v0;
+  double dVLb = 0.0, dVRb = 0.0;
+  if (!use_limiter) {
+    dVLb = (sb * dxr2) * rden;
+    dVRb = (sb * dxl2) * rden;
+  } else {
+    const bool zL = dVL == 0.0, zR = dVR == 0.0;
+    if (zL && !zR) {
+      dVLb = (sb * theta) * (0.5 * rdxl);
+    } else if (zR && !zL) {
+      dVRb = (sb * theta) * (0.5 * rdxr);
+    } else {
+      const double c = div_rcp(dxr2 * dVL + dxl2 * dVR, den, rden);
+      const double ql = div_rcp(theta * dVL, dxl, rdxl), qr = div_rcp(theta * dVR, dxr, rdxr);
+      const double tl = fabs(ql), tr = fabs(qr);
+      const double m1 = tl <= tr ? tl : tr;
+      const double factor = (signbit(dVL) ? -0.5 : 0.5) + (signbit(dVR) ? -0.5 : 0.5);
+      const double mb = sb * factor;
+      if (fabs(c) <= m1) {
+        const double cb = mb * adj::sgn1(c);
+        dVLb = (cb * dxr2) * rden;
+        dVRb = (cb * dxl2) * rden;
+      } else if (tl <= tr) {
+        dVLb = ((mb * adj::sgn1(ql)) * theta) * rdxl;
+      } else {
+        dVRb = ((mb * adj::sgn1(qr)) * theta) * rdxr;
+      }
+    }
+  }
+  vmb -= dVLb;
+  v0b += dVLb - dVRb;
+  vpb += dVRb;
+}
 
 template <int D>
 __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int c, int lane,
@@ -343,12 +218,14 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
       Gf[f] = 0.0;
     }
     if (c < n) {
+      const double r = A / volc;
 #pragma unroll
-      for (int f = 0; f < 5; f++) Gf[f] += A * Kb[f] / volc;
+      for (int f = 0; f < 5; f++) Gf[f] += r * Kb[f];
     }
     if (c >= 1) {
+      const double r = A / volm;
 #pragma unroll
-      for (int f = 0; f < 5; f++) Gf[f] -= A * Kbm[f] / volm;
+      for (int f = 0; f < 5; f++) Gf[f] -= r * Kbm[f];
     }
     double Fb[5], F[5], gl = 0.0;
 #pragma unroll
@@ -374,13 +251,15 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
     const double dxl = geo(a, D, G_DXL, q), dxr = geo(a, D, G_DXR, q);
     const double dxl2 = geo(a, D, G_DXL2, q), dxr2 = geo(a, D, G_DXR2, q);
     const double den = geo(a, D, G_DEN, q);
+    const double rdxl = geo(a, D, G_RDXL, q), rdxr = geo(a, D, G_RDXR, q);
+    const double rden = geo(a, D, G_RDEN, q);
 #pragma unroll
     for (int f = 0; f < 5; f++) {
       double sb = 0.0;
       if (c <= n - 1) sb += wr * up[f];
       if (c >= 0) sb -= wl * WRb[f];
-      adj::slope_vjp(Vm[f], V[f], Vq[f], dxl, dxr, dxl2, dxr2, den, a.ph.theta,
-                     a.ph.use_limiter, sb, dm[f], d0[f], dp[f]);
+      slope_vjp_r(Vm[f], V[f], Vq[f], dxl2, dxr2, dxl, dxr, den, rdxl, rdxr, rden, a.ph.theta,
+                  a.ph.use_limiter, sb, dm[f], d0[f], dp[f]);
     }
   }
 #pragma unroll
@@ -673,6 +552,28 @@ __global__ __launch_bounds__(AB) void adj_argmax_kernel(StageArgs<double> a,
   }
 }
 
+// First level of the dt and gamma bar sums: block b adds the pairs of its
+// contiguous chunk of the per-block partials (fixed order).
+constexpr int PART_BLOCKS = 128;
+
+__global__ __launch_bounds__(AB) void adj_partsum_kernel(const double *partial, int nparts,
+                                                         double *out) {
+  __shared__ double sh[AB];
+  const int chunk = (nparts + PART_BLOCKS - 1) / PART_BLOCKS;
+  const int i0 = blockIdx.x * chunk, i1 = min(nparts, i0 + chunk);
+  double sd = 0.0, sg = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += AB) {
+    sd += partial[2 * i];
+    sg += partial[2 * i + 1];
+  }
+  const double a = block_sum(sd, sh);
+  const double b = block_sum(sg, sh);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = a;
+    out[2 * blockIdx.x + 1] = b;
+  }
+}
+
 // The dt bar of one step: sum of the per-block partials (fixed order), the
 // CFL maximum's reverse at its cell (dt = ts = ramp * C / invdt), added to
 // out through the conserved -> primitive reverse; gamma bars into *gbar.
@@ -783,10 +684,10 @@ void release_adjoint(gx_ctx *c) {
   if (!A) return;
   for (double *p : A->ckpt) (void)hipFree(p);
   for (void *p : {(void *)A->seg, (void *)A->segY, (void *)A->segW, (void *)A->segZ,
-                  (void *)A->segF, (void *)A->trash, (void *)A->Vp, (void *)A->S,
-                  (void *)A->Dp, (void *)A->Vb, (void *)A->plus, (void *)A->minus, (void *)A->Ub,
+                  (void *)A->segF, (void *)A->trash, (void *)A->Vp, (void *)A->Vb,
+                  (void *)A->Ub,
                   (void *)A->Yb, (void *)A->Wb, (void *)A->K1b, (void *)A->K2b,
-                  (void *)A->P0b, (void *)A->partials, (void *)A->gb, (void *)A->argmax,
+                  (void *)A->P0b, (void *)A->partials, (void *)A->partials2, (void *)A->gb, (void *)A->argmax,
                   (void *)A->flags, (void *)A->band, (void *)A->lossp})
     (void)hipFree(p);
   delete A;
@@ -885,32 +786,16 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, 
   int rc = fill(c, A->Vb, 5 * g.np, 0.0);
   if (rc) return rc;
   for (int d = 0; d < 3; d++) {
-    long long nf = 1, ng = 1, ns = 1;
-    for (int e = 0; e < 3; e++) {
-      nf *= g.n[e] + (e == d);
-      ng *= e == d ? g.n[e] + 4 : g.n[e];
-      ns *= e == d ? g.n[e] + 2 : g.n[e];
-    }
-    const int fb = (int)((nf + AB - 1) / AB);
     double *pp = part + 2 * (size_t)d * A->part_stride;
-    (void)fb;
-    (void)ng;
-    (void)ns;
-    auto pass = [&](auto dim) {
-      constexpr int D = decltype(dim)::value;
-      const int nb = line_blocks(g, D);
-      if (D == 0)
-        adj_line_x_kernel<<<nb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb, pp);
-      else
-        adj_line_yz_kernel<D == 0 ? 1 : D><<<nb, TRWAVES * WAVE, 0, c->stream>>>(
-            a, g, A->Vp, Kb, dt, A->Vb, pp);
-    };
+    const int nb = line_blocks(g, d);
     if (d == 0)
-      pass(std::integral_constant<int, 0>());
+      adj_line_x_kernel<<<nb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb, pp);
     else if (d == 1)
-      pass(std::integral_constant<int, 1>());
+      adj_line_yz_kernel<1><<<nb, TRWAVES * WAVE, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb,
+                                                                  pp);
     else
-      pass(std::integral_constant<int, 2>());
+      adj_line_yz_kernel<2><<<nb, TRWAVES * WAVE, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb,
+                                                                  pp);
     HIP_TRY(hipGetLastError());
   }
   adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, g, A->Vb, out);
@@ -945,18 +830,13 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   HIP_TRY(hipMalloc(&A->segF, (size_t)c->ncell * every));
   for (double **p : {&A->trash, &A->Ub, &A->Yb, &A->Wb, &A->K1b, &A->K2b, &A->P0b})
     HIP_TRY(hipMalloc(p, s5));
-  for (double **p : {&A->Vp, &A->Vb, &A->plus, &A->minus, &A->S}) HIP_TRY(hipMalloc(p, p5));
-  HIP_TRY(hipMalloc(&A->Dp, 3 * p5));
-  long long maxf = 0;
-  for (int d = 0; d < 3; d++) {
-    long long nf = 1;
-    for (int e = 0; e < 3; e++) nf *= c->n[e] + (e == d);
-    maxf = std::max(maxf, nf);
-  }
-  A->part_stride = (int)((maxf + AB - 1) / AB);
+  for (double **p : {&A->Vp, &A->Vb}) HIP_TRY(hipMalloc(p, p5));
+  A->part_stride = 0;
   for (int d = 0; d < 3; d++) A->part_stride = std::max(A->part_stride, line_blocks(g, d));
+  // every line block writes its pair; the tails of the shorter passes stay 0
   HIP_TRY(hipMalloc(&A->partials, sizeof(double) * 2 * 6 * (size_t)A->part_stride));
   HIP_TRY(hipMemset(A->partials, 0, sizeof(double) * 2 * 6 * (size_t)A->part_stride));
+  HIP_TRY(hipMalloc(&A->partials2, sizeof(double) * 2 * PART_BLOCKS));
   HIP_TRY(hipMalloc(&A->gb, sizeof(double)));
   HIP_TRY(hipMalloc(&A->argmax, sizeof(int)));
   HIP_TRY(hipMalloc(&A->flags, (size_t)c->ncell));
@@ -1053,8 +933,6 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       const uint8_t *Fn = A->segF + (size_t)j * c->ncell;
       const bool first = n == 0;
       const double dt = A->dt[n];
-      rc = fill(c, A->partials, 12 * (long long)A->part_stride, 0.0);
-      if (rc) return rc;
       adj_fix_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
           c->ncell, c->params.BaryonMassReference, c->params.BaryonDensityMin,
           A->segZ + (size_t)j * n5, A->Ub, A->Wb, A->K2b);
@@ -1079,7 +957,10 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       memcpy(&bits, &A->invdt[n], sizeof(bits));
       adj_argmax_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, bits, A->argmax);
       HIP_TRY(hipGetLastError());
-      adj_dt_kernel<<<1, AB, 0, c->stream>>>(a, A->partials, 6 * A->part_stride, A->ts[n],
+      adj_partsum_kernel<<<PART_BLOCKS, AB, 0, c->stream>>>(A->partials, 6 * A->part_stride,
+                                                            A->partials2);
+      HIP_TRY(hipGetLastError());
+      adj_dt_kernel<<<1, AB, 0, c->stream>>>(a, A->partials2, PART_BLOCKS, A->ts[n],
                                              A->invdt[n], A->argmax, out, A->gb);
       HIP_TRY(hipGetLastError());
     }
