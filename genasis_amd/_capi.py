@@ -55,7 +55,7 @@ EXPORTS = (
     "gx_batch_set_tuning", "gx_batch_get_tuning", "gx_batch_autotune",
     "gx_batch_cells", "gx_batch_stream",
     "gx_adjoint_forward", "gx_adjoint_backward", "gx_adjoint_kinetic_energy",
-    "gx_adjoint_face_vjp", "gx_adjoint_slope_vjp",
+    "gx_adjoint_get_bar", "gx_adjoint_face_vjp", "gx_adjoint_slope_vjp",
 )
 
 
@@ -149,6 +149,7 @@ def load(path: str = LIB_PATH):
         "gx_adjoint_forward": (i, [vp, i, i, i]),
         "gx_adjoint_backward": (i, [vp, pd, pd, pd]),
         "gx_adjoint_kinetic_energy": (i, [vp, i, d, d, pd, pd]),
+        "gx_adjoint_get_bar": (i, [vp, pd]),
         "gx_adjoint_face_vjp": (None, [pd, pd, i, i, d, d, i, pd, pd, pd, pd, pd]),
         "gx_adjoint_slope_vjp": (None, [d, d, d, d, d, d, i, d, pd]),
     }

@@ -31,21 +31,37 @@ class DiscreteAdjoint:
                                                 int(checkpoint_every), int(n_ramp)),
                     "gx_adjoint_forward")
 
-    def backward(self, Ubar: np.ndarray):
-        ub = np.ascontiguousarray(Ubar, dtype=np.float64).reshape((5,) + self.shape)
-        pb = np.empty((5,) + self.shape)
+    def backward(self, Ubar: np.ndarray | None = None, fetch: bool = True):
+        """(P0bar, gamma bar) for Ubar = dJ/dU_final.  Ubar None takes the seed
+        that ``kinetic_energy(keep_on_device=True)`` left on the device;
+        fetch False leaves P0bar there (returned as None; ``get_bar`` copies
+        it later)."""
+        ub = None
+        if Ubar is not None:
+            ub = np.ascontiguousarray(Ubar, dtype=np.float64).reshape((5,) + self.shape)
+        pb = np.empty((5,) + self.shape) if fetch else None
         gb = ctypes.c_double()
-        _capi.check(self.lib.gx_adjoint_backward(self.st._h, _pd(ub), _pd(pb),
+        _capi.check(self.lib.gx_adjoint_backward(self.st._h, None if ub is None else _pd(ub),
+                                                 None if pb is None else _pd(pb),
                                                  ctypes.byref(gb)), "gx_adjoint_backward")
         return pb, gb.value
 
-    def kinetic_energy(self, component: int = 1, lo: float = 0.2, hi: float = 0.8):
+    def get_bar(self):
+        """P0bar of the last reverse pass."""
+        pb = np.empty((5,) + self.shape)
+        _capi.check(self.lib.gx_adjoint_get_bar(self.st._h, _pd(pb)), "gx_adjoint_get_bar")
+        return pb
+
+    def kinetic_energy(self, component: int = 1, lo: float = 0.2, hi: float = 0.8,
+                       keep_on_device: bool = False):
         """(J, dJ/dU) for J = sum over lo < y_c < hi of 1/2 N V_(component+1)^2 dV
-        on the current state."""
+        on the current state.  dJ/dU also stays on the device as the seed of
+        the next ``backward(None)``; keep_on_device True returns None for it."""
         loss = ctypes.c_double()
-        ub = np.empty((5,) + self.shape)
+        ub = None if keep_on_device else np.empty((5,) + self.shape)
         _capi.check(self.lib.gx_adjoint_kinetic_energy(self.st._h, int(component), float(lo),
-                                                       float(hi), ctypes.byref(loss), _pd(ub)),
+                                                       float(hi), ctypes.byref(loss),
+                                                       None if ub is None else _pd(ub)),
                     "gx_adjoint_kinetic_energy")
         return loss.value, ub
 

@@ -55,13 +55,14 @@ struct AdjointState {
   std::vector<double> dt, ts, invdt, ramp;
   std::vector<double *> ckpt;
   // Per step of one segment (every steps): the input U^n, the stage
-  // values Y and W, the state before the fix Z, and the shock flags.
+  // values Y and W, and the state before the fix Z.
   double *seg = nullptr, *segY = nullptr, *segW = nullptr, *segZ = nullptr;
-  uint8_t *segF = nullptr;
+  uint8_t *stepF = nullptr;    // the shock flags each step used, nsteps ncell
   double *trash = nullptr;
   double *Vp = nullptr, *Vb = nullptr;  // primitives and their bars, 5 np
-  double *Ub = nullptr, *Yb = nullptr, *Wb = nullptr, *K1b = nullptr, *K2b = nullptr,
-         *P0b = nullptr;       // 5 ncell
+  double *Ub = nullptr, *K1b = nullptr, *K2b = nullptr, *P0b = nullptr;  // 5 ncell
+  bool seed_ready = false;     // Ub holds dJ/dU_final (gx_adjoint_kinetic_energy)
+  bool bar_ready = false;      // P0b holds the last reverse pass's result
   double *partials = nullptr;  // 2 per line block, both stages, three dimensions
   int part_stride = 0;         // blocks of the largest line pass
   double *partials2 = nullptr; // 2 per block of the first reduction level
@@ -273,24 +274,28 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   }
 }
 
-// D = 0: one wave per segment of a row (lanes along x, coalesced).
+// D = 0: the rows (y, z) of the padded layout, cells -2 .. n+1 each, are
+// consecutive in memory; waves take 58 positions each of that sequence
+// (lanes along x, coalesced), so a wave may hold the end of one row and the
+// start of the next.  Every use of a neighbour lane is guarded by the own
+// row's cell range, and the lanes of another row contribute zeros there.
 __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, PadGeom g,
                                                         const double *Vp, const double *Kb,
                                                         double dt, double *Vb,
                                                         double *partial) {
   __shared__ double sh[AB];
   const int n = g.n[0];
-  const int nseg = (n + 4 + LINE_OUT - 1) / LINE_OUT;
+  const int L = n + 4;
+  const long long T = (long long)L * g.n[1] * g.n[2];
   const int lane = threadIdx.x % WAVE;
   const long long w = blockIdx.x * (long long)(AB / WAVE) + threadIdx.x / WAVE;
-  const long long nw = (long long)g.n[1] * g.n[2] * nseg;
+  const long long p = w * LINE_OUT - 3 + lane;
   double dtb = 0.0, gbs = 0.0;
-  if (w < nw) {  // uniform over the wave
-    const int sgi = (int)(w % nseg);
-    const long long r = w / nseg;
+  if (w * LINE_OUT < T) {  // uniform over the wave
+    const bool valid = p >= 0 && p < T;
+    const long long r = valid ? p / L : 0;
+    const int c = valid ? (int)(p - r * L) - 2 : -100;  // -100: in no cell range
     const int y = (int)(r % g.n[1]), z = (int)(r / g.n[1]);
-    const int c0 = -2 + sgi * LINE_OUT;
-    const int c = c0 - 3 + lane;
     const long long pl = plin(g, 0, y, z);
     double V[5], K[5];
     bool df = false;
@@ -299,7 +304,7 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
     const double A = wy * wz;
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      V[f] = (c >= -2 && c <= n + 1) ? Vp[f * g.np + pl + c] : 0.0;
+      V[f] = valid ? Vp[f * g.np + pl + c] : 0.0;
       K[f] = 0.0;
     }
     if (c >= 0 && c < n) {
@@ -310,12 +315,13 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
       volc = cell_width(a, 0, c) * wy * wz;
     }
     if (c >= 1 && c <= n) volm = cell_width(a, 0, c - 1) * wy * wz;
-    const bool own = lane >= 3 && lane <= 60 && c <= n + 1;
+    const bool own = valid && lane >= 3 && lane <= 60;
     double out[5];
     line_vjp<0>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
     if (own) {
+      // the x pass is the first to touch the interior rows: it sets them
 #pragma unroll
-      for (int f = 0; f < 5; f++) Vb[f * g.np + pl + c] += out[f];
+      for (int f = 0; f < 5; f++) Vb[f * g.np + pl + c] = out[f];
     }
   }
   const double s1 = block_sum(dtb, sh);
@@ -326,9 +332,12 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
   }
 }
 
-// D = 1, 2: TRW columns of 64 cells along D per block, loaded and stored
-// along x through LDS (64-byte row pieces); each of the four waves works on
-// two columns in turn.
+// D = 1, 2: TRW columns (x) per block, each holding 64 consecutive positions
+// of the sequence of lines along D (cells -2 .. n+1 of line o, then of line
+// o+1, ...), loaded and stored along x through LDS (64-byte row pieces);
+// each of the four waves works on two columns in turn.  Ghost positions
+// along D are touched by this pass alone and are set; interior cells add to
+// the x pass.
 constexpr int TRW = 8;          // columns (x) per block
 constexpr int TRP = TRW + 1;    // padded LDS row
 constexpr int TRWAVES = 4;      // waves per block
@@ -346,22 +355,23 @@ __global__ __launch_bounds__(TRWAVES * WAVE) void adj_line_yz_kernel(StageArgs<d
   __shared__ unsigned char sF[WAVE][TRP];
   __shared__ double sh[NTH];
   const int n = g.n[D];
-  const int nseg = (n + 4 + LINE_OUT - 1) / LINE_OUT;
+  const int L = n + 4;
+  const int T = L * g.n[O];
   const int nxb = (g.n[0] + TRW - 1) / TRW;
-  // block -> (x block, segment, other coordinate), XCD-aware: neighbouring
-  // x blocks share the cache lines of their rows and run on one L2
+  // block -> (x block, segment), XCD-aware: neighbouring x blocks share the
+  // cache lines of their rows and run on one L2
   const int b = tile_of_block(blockIdx.x, gridDim.x);
-  const int xb = b % nxb, sgi = (b / nxb) % nseg, o = b / (nxb * nseg);
-  const int x0 = xb * TRW, c0 = -2 + sgi * LINE_OUT;
+  const int xb = b % nxb, sgi = b / nxb;
+  const int x0 = xb * TRW, p0 = sgi * LINE_OUT - 3;
   const int tid = threadIdx.x;
-  auto pidx = [&](int x, int c) {
+  auto pidx = [&](int x, int c, int o) {
     int q[3];
     q[0] = x;
     q[D] = c;
     q[O] = o;
     return plin(g, q[0], q[1], q[2]);
   };
-  auto lidx = [&](int x, int c) {
+  auto lidx = [&](int x, int c, int o) {
     int q[3];
     q[0] = x;
     q[D] = c;
@@ -371,12 +381,12 @@ __global__ __launch_bounds__(TRWAVES * WAVE) void adj_line_yz_kernel(StageArgs<d
   // cooperative loads: item -> (x = item % TRW, lane row l = item / TRW)
   for (int item = tid; item < TRW * WAVE; item += NTH) {
     const int xi = item % TRW, l = item / TRW;
-    const int x = x0 + xi, c = c0 - 3 + l;
-    const bool okx = x < g.n[0];
-    const bool okV = okx && c >= -2 && c <= n + 1;
-    const bool okK = okx && c >= 0 && c < n;
-    const long long pv = okV ? pidx(x, c) : 0;
-    const unsigned ik = okK ? lidx(x, c) : 0u;
+    const int x = x0 + xi, p = p0 + l;
+    const bool okV = x < g.n[0] && p >= 0 && p < T;
+    const int o = okV ? p / L : 0, c = okV ? p - o * L - 2 : -100;
+    const bool okK = okV && c >= 0 && c < n;
+    const long long pv = okV ? pidx(x, c, o) : 0;
+    const unsigned ik = okK ? lidx(x, c, o) : 0u;
 #pragma unroll
     for (int f = 0; f < 5; f++) {
       sV[f][l][xi] = okV ? Vp[f * g.np + pv] : 0.0;
@@ -386,7 +396,10 @@ __global__ __launch_bounds__(TRWAVES * WAVE) void adj_line_yz_kernel(StageArgs<d
   }
   __syncthreads();
   const int wv = tid / WAVE, lane = tid % WAVE;
-  const int c = c0 - 3 + lane;
+  const int p = p0 + lane;
+  const bool valid = p >= 0 && p < T;
+  const int o = valid ? p / L : 0, c = valid ? p - o * L - 2 : -100;
+  const bool own = valid && lane >= 3 && lane <= 60;
   double dtb = 0.0, gbs = 0.0;
   const double wo = cell_width(a, O, o);
   for (int col = wv; col < TRW; col += TRWAVES) {
@@ -409,12 +422,7 @@ __global__ __launch_bounds__(TRWAVES * WAVE) void adj_line_yz_kernel(StageArgs<d
       };
       if (c >= 0 && c < n) volc = vol_of(c);
       if (c >= 1 && c <= n) volm = vol_of(c - 1);
-      const bool own = lane >= 3 && lane <= 60 && c <= n + 1;
       line_vjp<D>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
-      if (!own) {
-#pragma unroll
-        for (int f = 0; f < 5; f++) out[f] = 0.0;
-      }
     }
     // this wave alone reads and writes column col
 #pragma unroll
@@ -423,11 +431,17 @@ __global__ __launch_bounds__(TRWAVES * WAVE) void adj_line_yz_kernel(StageArgs<d
   __syncthreads();
   for (int item = tid; item < TRW * WAVE; item += NTH) {
     const int xi = item % TRW, l = item / TRW;
-    const int xx = x0 + xi, cc = c0 - 3 + l;
-    if (xx < g.n[0] && l >= 3 && l <= 60 && cc <= n + 1) {
-      const long long pv = pidx(xx, cc);
+    const int xx = x0 + xi, pp = p0 + l;
+    if (xx < g.n[0] && l >= 3 && l <= 60 && pp >= 0 && pp < T) {
+      const int oo = pp / L, cc = pp - oo * L - 2;
+      const long long pv = pidx(xx, cc, oo);
+      if (cc >= 0 && cc < n) {
 #pragma unroll
-      for (int f = 0; f < 5; f++) Vb[f * g.np + pv] += sV[f][l][xi];
+        for (int f = 0; f < 5; f++) Vb[f * g.np + pv] += sV[f][l][xi];
+      } else {
+#pragma unroll
+        for (int f = 0; f < 5; f++) Vb[f * g.np + pv] = sV[f][l][xi];
+      }
     }
   }
   // block sums of the dt and gamma bars
@@ -454,9 +468,12 @@ __global__ __launch_bounds__(TRWAVES * WAVE) void adj_line_yz_kernel(StageArgs<d
 // Interior cells: the primitive bar of the cell plus the bars of the ghost
 // values copied from it (normal velocity negated across a REFLECTING face),
 // then the conserved -> primitive reverse (or the primitive bar itself for an
-// uploaded-primitive input), added to out.
+// uploaded-primitive input): y.  Stage 1 adds y to out.  Stage 2 (K2b given)
+// ends the RK reverse too: with Yb = y, Wb = 2 K2b (Y = U + K1, W = U + K1/2,
+// Z = W + K2/2), out = Ub = Yb + Wb and K1b = Yb + Wb/2.
 __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGeom g,
-                                                      const double *Vb, double *out) {
+                                                      const double *Vb, double *out,
+                                                      const double *K2b, double *K1b) {
   const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
   for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
        t += (long long)gridDim.x * AB) {
@@ -491,24 +508,34 @@ __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGe
         }
       }
     }
+    double y[5] = {0, 0, 0, 0, 0};
     if (a.src_is_prim) {
 #pragma unroll
-      for (int f = 0; f < 5; f++) out[f * ncell + t] += V[f];
+      for (int f = 0; f < 5; f++) y[f] = V[f];
     } else {
-      double U[5], Ub[5] = {0, 0, 0, 0, 0};
+      double U[5];
 #pragma unroll
       for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];
-      adj::prim_vjp(U, a.ph.mass, a.ph.nmin, V, Ub);
+      adj::prim_vjp(U, a.ph.mass, a.ph.nmin, V, y);
+    }
+    if (K2b) {
 #pragma unroll
-      for (int f = 0; f < 5; f++) out[f * ncell + t] += Ub[f];
+      for (int f = 0; f < 5; f++) {
+        const double k2 = K2b[f * ncell + t];
+        out[f * ncell + t] = y[f] + 2.0 * k2;
+        K1b[f * ncell + t] = y[f] + k2;
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 5; f++) out[f * ncell + t] += y[f];
     }
   }
 }
 
-// Zb = fix^T Ub' ; Wb = Zb, K2b = Zb/2.
+// Zb = fix^T Ub' ; K2b = Zb/2 (and Wb = Zb = 2 K2b, formed by the fold).
 __global__ __launch_bounds__(AB) void adj_fix_kernel(long long ncell, double mass, double nmin,
                                                      const double *Z, const double *Ubn,
-                                                     double *Wb, double *K2b) {
+                                                     double *K2b) {
   for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
        t += (long long)gridDim.x * AB) {
     double z[5], ub[5], zb[5];
@@ -519,20 +546,7 @@ __global__ __launch_bounds__(AB) void adj_fix_kernel(long long ncell, double mas
     }
     adj::fix_vjp(z, mass, nmin, ub, zb);
 #pragma unroll
-    for (int f = 0; f < 5; f++) {
-      Wb[f * ncell + t] = zb[f];
-      K2b[f * ncell + t] = 0.5 * zb[f];
-    }
-  }
-}
-
-// Y = U + K1, W = U + K1/2: Ub = Yb + Wb, K1b = Yb + Wb/2.
-__global__ __launch_bounds__(AB) void adj_rk_kernel(long long n5, const double *Yb,
-                                                    const double *Wb, double *Ub, double *K1b) {
-  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < n5;
-       t += (long long)gridDim.x * AB) {
-    Ub[t] = Yb[t] + Wb[t];
-    K1b[t] = Yb[t] + 0.5 * Wb[t];
+    for (int f = 0; f < 5; f++) K2b[f * ncell + t] = 0.5 * zb[f];
   }
 }
 
@@ -684,9 +698,8 @@ void release_adjoint(gx_ctx *c) {
   if (!A) return;
   for (double *p : A->ckpt) (void)hipFree(p);
   for (void *p : {(void *)A->seg, (void *)A->segY, (void *)A->segW, (void *)A->segZ,
-                  (void *)A->segF, (void *)A->trash, (void *)A->Vp, (void *)A->Vb,
-                  (void *)A->Ub,
-                  (void *)A->Yb, (void *)A->Wb, (void *)A->K1b, (void *)A->K2b,
+                  (void *)A->stepF, (void *)A->trash, (void *)A->Vp, (void *)A->Vb,
+                  (void *)A->Ub, (void *)A->K1b, (void *)A->K2b,
                   (void *)A->P0b, (void *)A->partials, (void *)A->partials2, (void *)A->gb, (void *)A->argmax,
                   (void *)A->flags, (void *)A->band, (void *)A->lossp})
     (void)hipFree(p);
@@ -713,18 +726,6 @@ static void stage_geometry(const gx_ctx *c, StageArgs<double> &a) {
   a.nbx = (c->n[0] + TX - 1) / TX;
   a.nby = (c->n[1] + TY - 1) / TY;
   a.nblk = a.nbx * a.nby * ((c->n[2] + c->tz - 1) / c->tz);
-}
-
-static int launch_flags(gx_ctx *c, const double *U, uint8_t *flags) {
-  StageArgs<double> a = base_args<double>(c);
-  a.src = U;
-  a.tz = FLAG_TZ;
-  a.nbx = (c->n[0] + TX - 1) / TX;
-  a.nby = (c->n[1] + TY - 1) / TY;
-  a.nblk = a.nbx * a.nby * ((c->n[2] + FLAG_TZ - 1) / FLAG_TZ);
-  shock_flags_kernel<double><<<a.nblk, NT, 0, c->stream>>>(a, flags);
-  HIP_TRY(hipGetLastError());
-  return GX_OK;
 }
 
 // One stage-1 launch: out1 = aux + K(src), out2 = aux + K(src)/2.
@@ -762,19 +763,18 @@ static int launch_stage2(gx_ctx *c, const double *Y, const double *W, double *ou
 
 // Blocks of the line kernel of dimension d.
 static int line_blocks(const PadGeom &g, int d) {
-  const long long nseg = (g.n[d] + 4 + LINE_OUT - 1) / LINE_OUT;
-  if (d == 0) {
-    const long long waves = (long long)g.n[1] * g.n[2] * nseg;
-    return (int)((waves + AB / WAVE - 1) / (AB / WAVE));
-  }
+  const long long other = d == 0 ? (long long)g.n[1] * g.n[2] : g.n[d == 1 ? 2 : 1];
+  const long long nseg = ((g.n[d] + 4) * other + LINE_OUT - 1) / LINE_OUT;
+  if (d == 0) return (int)((nseg + AB / WAVE - 1) / (AB / WAVE));
   const long long nxb = (g.n[0] + TRW - 1) / TRW;
-  return (int)(nxb * nseg * g.n[d == 1 ? 2 : 1]);
+  return (int)(nxb * nseg);
 }
 
 // Transposed residual: out += (dt R(src))^T Kb; dt and gamma bar partials at
 // part (3 face passes).
 static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, double dt,
-                     const uint8_t *flags, double *out, double *part) {
+                     const uint8_t *flags, double *out, double *part,
+                     const double *K2b = nullptr, double *K1b = nullptr) {
   AdjointState *A = c->adj;
   const PadGeom &g = A->pg;
   StageArgs<double> a = base_args<double>(c);
@@ -783,8 +783,8 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, 
   a.flags = flags;
   adj_prims_kernel<<<blocks_for(g.np), AB, 0, c->stream>>>(a, g, A->Vp);
   HIP_TRY(hipGetLastError());
-  int rc = fill(c, A->Vb, 5 * g.np, 0.0);
-  if (rc) return rc;
+  // the x pass sets Vb on the interior rows, the y and z passes on their
+  // ghost positions; cells with two coordinates outside stay unused
   for (int d = 0; d < 3; d++) {
     double *pp = part + 2 * (size_t)d * A->part_stride;
     const int nb = line_blocks(g, d);
@@ -798,7 +798,7 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, 
                                                                   pp);
     HIP_TRY(hipGetLastError());
   }
-  adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, g, A->Vb, out);
+  adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, g, A->Vb, out, K2b, K1b);
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }
@@ -827,8 +827,8 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   A->ckpt.assign(nck, nullptr);
   for (int i = 0; i < nck; i++) HIP_TRY(hipMalloc(&A->ckpt[i], s5));
   for (double **p : {&A->seg, &A->segY, &A->segW, &A->segZ}) HIP_TRY(hipMalloc(p, s5 * every));
-  HIP_TRY(hipMalloc(&A->segF, (size_t)c->ncell * every));
-  for (double **p : {&A->trash, &A->Ub, &A->Yb, &A->Wb, &A->K1b, &A->K2b, &A->P0b})
+  HIP_TRY(hipMalloc(&A->stepF, (size_t)c->ncell * nsteps));
+  for (double **p : {&A->trash, &A->Ub, &A->K1b, &A->K2b, &A->P0b})
     HIP_TRY(hipMalloc(p, s5));
   for (double **p : {&A->Vp, &A->Vb}) HIP_TRY(hipMalloc(p, p5));
   A->part_stride = 0;
@@ -863,6 +863,9 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
     if (n % checkpointEvery == 0)
       HIP_TRY(hipMemcpyAsync(A->ckpt[n / checkpointEvery], c->U, s5, hipMemcpyDeviceToDevice,
                              c->stream));
+    // the flags this step uses (those of the upload, then of the last step)
+    HIP_TRY(hipMemcpyAsync(A->stepF + (size_t)n * c->ncell, c->flags, (size_t)c->ncell,
+                           hipMemcpyDeviceToDevice, c->stream));
     double cand;
     rc = gx_compute_time_step(c, &cand);
     if (rc) return rc;
@@ -889,14 +892,19 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
 }
 
 int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *gammaBar) {
-  if (!c || !Ubar || !P0bar || !gammaBar)
-    return fail(GX_E_ARG, "gx_adjoint_backward: null argument");
+  if (!c || !gammaBar) return fail(GX_E_ARG, "gx_adjoint_backward: null argument");
   AdjointState *A = c->adj;
   if (!A || !A->forward_done)
     return fail(GX_E_STATE, "gx_adjoint_backward: no forward run recorded");
+  if (!Ubar && !A->seed_ready)
+    return fail(GX_E_STATE,
+                "gx_adjoint_backward: Ubar is NULL and no device seed is held "
+                "(gx_adjoint_kinetic_energy with Ubar = NULL leaves one)");
   const long long n5 = 5 * c->ncell;
   const size_t s5 = sizeof(double) * (size_t)n5;
-  HIP_TRY(hipMemcpyAsync(A->Ub, Ubar, s5, hipMemcpyHostToDevice, c->stream));
+  if (Ubar) HIP_TRY(hipMemcpyAsync(A->Ub, Ubar, s5, hipMemcpyHostToDevice, c->stream));
+  A->seed_ready = false;  // the reverse pass overwrites Ub
+  A->bar_ready = false;
   int rc = fill(c, A->P0b, n5, 0.0);
   if (rc) return rc;
   rc = fill(c, A->gb, 1, 0.0);
@@ -905,21 +913,15 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   const int every = A->every, N = A->nsteps;
   for (int seg0 = ((N - 1) / every) * every; seg0 >= 0; seg0 -= every) {
     const int len = std::min(every, N - seg0);
-    // Recompute the segment from its checkpoint, keeping per step the input
-    // U^n, the flags, Y, W and the state before the fix Z.
+    // Recompute the segment from its checkpoint with the recorded flags,
+    // keeping per step the input U^n, Y, W and the state before the fix Z.
     HIP_TRY(hipMemcpyAsync(A->seg, A->ckpt[seg0 / every], s5, hipMemcpyDeviceToDevice,
                            c->stream));
     for (int j = 0; j < len; j++) {
       const int n = seg0 + j;
       const double *Un = A->seg + (size_t)j * n5;
       double *Un1 = j + 1 < len ? A->seg + (size_t)(j + 1) * n5 : A->trash;
-      uint8_t *Fn = A->segF + (size_t)j * c->ncell;
-      if (n == 0) {
-        HIP_TRY(hipMemsetAsync(Fn, 0, (size_t)c->ncell, c->stream));
-      } else {
-        rc = launch_flags(c, Un, Fn);
-        if (rc) return rc;
-      }
+      const uint8_t *Fn = A->stepF + (size_t)n * c->ncell;
       double *Yn = A->segY + (size_t)j * n5, *Wn = A->segW + (size_t)j * n5;
       rc = launch_stage1(c, n == 0 ? P0 : Un, n == 0, Un, Yn, Wn, A->dt[n], Fn);
       if (rc) return rc;
@@ -930,20 +932,17 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       const int n = seg0 + j;
       const double *Un = A->seg + (size_t)j * n5;
       const double *Yn = A->segY + (size_t)j * n5;
-      const uint8_t *Fn = A->segF + (size_t)j * c->ncell;
+      const uint8_t *Fn = A->stepF + (size_t)n * c->ncell;
       const bool first = n == 0;
       const double dt = A->dt[n];
       adj_fix_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
           c->ncell, c->params.BaryonMassReference, c->params.BaryonDensityMin,
-          A->segZ + (size_t)j * n5, A->Ub, A->Wb, A->K2b);
+          A->segZ + (size_t)j * n5, A->Ub, A->K2b);
       HIP_TRY(hipGetLastError());
-      rc = fill(c, A->Yb, n5, 0.0);
+      // stage 2's transpose, ending with Ub = Yb + Wb and K1b = Yb + Wb/2
+      rc = stage_vjp(c, Yn, false, A->K2b, dt, Fn, A->Ub,
+                     A->partials + 6 * (size_t)A->part_stride, A->K2b, A->K1b);
       if (rc) return rc;
-      rc = stage_vjp(c, Yn, false, A->K2b, dt, Fn, A->Yb,
-                     A->partials + 6 * (size_t)A->part_stride);
-      if (rc) return rc;
-      adj_rk_kernel<<<blocks_for(n5), AB, 0, c->stream>>>(n5, A->Yb, A->Wb, A->Ub, A->K1b);
-      HIP_TRY(hipGetLastError());
       double *out = first ? A->P0b : A->Ub;
       rc = stage_vjp(c, first ? P0 : Un, first, A->K1b, dt, Fn, out, A->partials);
       if (rc) return rc;
@@ -969,16 +968,28 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   adj_from_prim_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
       c->ncell, c->params.BaryonMassReference, P0, A->Ub, A->P0b);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(P0bar, A->P0b, s5, hipMemcpyDeviceToHost, c->stream));
+  if (P0bar) HIP_TRY(hipMemcpyAsync(P0bar, A->P0b, s5, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(gammaBar, A->gb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  A->bar_ready = true;
   c->invdt_valid = false;  // the recomputation reused the CFL accumulator
+  return GX_OK;
+}
+
+int gx_adjoint_get_bar(gx_ctx *c, double *P0bar) {
+  if (!c || !P0bar) return fail(GX_E_ARG, "gx_adjoint_get_bar: null argument");
+  AdjointState *A = c->adj;
+  if (!A || !A->bar_ready)
+    return fail(GX_E_STATE, "gx_adjoint_get_bar: no reverse pass has completed");
+  HIP_TRY(hipMemcpyAsync(P0bar, A->P0b, 5 * sizeof(double) * (size_t)c->ncell,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
 }
 
 int gx_adjoint_kinetic_energy(gx_ctx *c, int component, double lo, double hi,
                               double *loss, double *Ubar) {
-  if (!c || !loss || !Ubar || component < 0 || component > 2)
+  if (!c || !loss || component < 0 || component > 2)
     return fail(GX_E_ARG, "gx_adjoint_kinetic_energy: bad argument");
   if (!c->adj) return fail(GX_E_STATE, "gx_adjoint_kinetic_energy: no forward run recorded");
   AdjointState *A = c->adj;
@@ -1005,9 +1016,11 @@ int gx_adjoint_kinetic_energy(gx_ctx *c, int component, double lo, double hi,
   std::vector<double> part(nb);
   HIP_TRY(hipMemcpyAsync(part.data(), A->lossp, sizeof(double) * nb, hipMemcpyDeviceToHost,
                          c->stream));
-  HIP_TRY(hipMemcpyAsync(Ubar, A->Ub, 5 * sizeof(double) * (size_t)c->ncell,
-                         hipMemcpyDeviceToHost, c->stream));
+  if (Ubar)
+    HIP_TRY(hipMemcpyAsync(Ubar, A->Ub, 5 * sizeof(double) * (size_t)c->ncell,
+                           hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  A->seed_ready = true;  // the seed stays on the device for gx_adjoint_backward
   double s = 0.0;
   for (double v : part) s += v;
   *loss = s;

@@ -182,99 +182,85 @@ GX_HD void face_flux_vjp(const double WL[5], const double WR[5], int d, bool df,
     const double dd = MD * MD >= kTiny ? MD * MD : kTiny;
     const double MDinv = nn / dd;
     const double AC = 1.0 * SN * MDinv;
-    double ACb = 0.0, Dsb = 0.0, Ssb[3] = {0, 0, 0}, Gsb = 0.0, Psb = 0.0;
-    if (AC >= 0.0) {
-      const double AM_VL = AM + vdL, AM_AC = AM + AC;
-      const double aa = fabs(AM_AC);
+    double ACb = 0.0;
+    // The star state on the side of the contact speed, written once for
+    // both sides: side X = L with A = AM and s = +1 when AC >= 0, else X = R
+    // with A = AP and s = -1.  A + s v is AM + vL or AP - vR, and
+    // (P + s a) - s b is (P + a) - b or (P - a) + b, bit for bit.
+    {
+      const bool lft = AC >= 0.0;
+      const double s = lft ? 1.0 : -1.0;
+      const double A = lft ? AM : AP;
+      const double v = lft ? vdL : vdR;
+      const double XD = lft ? L.D : R.D, XP = lft ? L.P : R.P, XG = lft ? L.G : R.G;
+      const double XSd = lft ? SdL : SdR;
+      double XV[3];
+      for (int k = 0; k < 3; k++) XV[k] = lft ? L.V[k] : R.V[k];
+      const double AV = A + s * v, AAC = A + s * AC;
+      const double aa = fabs(AAC);
       const double den = aa >= kSqrtTiny ? aa : kSqrtTiny;
       const double inv = 1.0 / den;
-      const double Ds = L.D * AM_VL * inv;
+      const double Ds = XD * AV * inv;
       double Vs[3], Ss[3];
       for (int k = 0; k < 3; k++) {
-        Vs[k] = k == d ? AC : L.V[k];
+        Vs[k] = k == d ? AC : XV[k];
         Ss[k] = mass * Ds * Vs[k];
       }
-      const double Ps = L.P + SdL * AM_VL - Ss[d] * AM_AC;
-      const double t = L.G * AM_VL + vdL * L.P - AC * Ps;
+      const double Ps = (XP + s * (XSd * AV)) - s * (Ss[d] * AAC);
+      const double t = (XG * AV + s * (v * XP)) - s * (AC * Ps);
       const double Gs = t * inv;
       raw(Ds, Ss, Gs, Ps, AC, d, F);
+      double Dsb = 0.0, Ssb[3] = {0, 0, 0}, Gsb = 0.0, Psb = 0.0;
       raw_vjp(Ds, Ss, Gs, Ps, AC, d, Fb, Dsb, Ssb, Gsb, Psb, ACb);
-      double AM_VLb = 0.0, AM_ACb = 0.0;
+      FBar Xb;
+      double AVb = 0.0, AACb = 0.0, vb = 0.0;
       const double tb = Gsb * inv;
       double invb = Gsb * t;
-      Lb.G += tb * AM_VL;
-      AM_VLb += tb * L.G;
-      vdLb += tb * L.P;
-      Lb.P += tb * vdL;
-      ACb -= tb * Ps;
-      Psb -= tb * AC;
-      Lb.P += Psb;
-      Lb.S[d] += Psb * AM_VL;
-      AM_VLb += Psb * SdL;
-      Ssb[d] -= Psb * AM_AC;
-      AM_ACb -= Psb * Ss[d];
+      Xb.G += tb * AV;
+      AVb += tb * XG;
+      vb += s * (tb * XP);
+      Xb.P += s * (tb * v);
+      ACb -= s * (tb * Ps);
+      Psb -= s * (tb * AC);
+      Xb.P += Psb;
+      Xb.S[d] += s * (Psb * AV);
+      AVb += s * (Psb * XSd);
+      Ssb[d] -= s * (Psb * AAC);
+      AACb -= s * (Psb * Ss[d]);
       for (int k = 0; k < 3; k++) {
         Dsb += Ssb[k] * mass * Vs[k];
-        const double vb = Ssb[k] * mass * Ds;
+        const double vk = Ssb[k] * mass * Ds;
         if (k == d)
-          ACb += vb;
+          ACb += vk;
         else
-          Lb.V[k] += vb;
+          Xb.V[k] += vk;
       }
-      Lb.D += Dsb * AM_VL * inv;
-      AM_VLb += Dsb * L.D * inv;
-      invb += Dsb * L.D * AM_VL;
-      if (aa >= kSqrtTiny) AM_ACb += -invb * inv * inv * sgn1(AM_AC);
-      AMb += AM_ACb;
-      ACb += AM_ACb;
-      AMb += AM_VLb;
-      vdLb += AM_VLb;
-    } else {
-      const double AP_VR = AP - vdR, AP_AC = AP - AC;
-      const double aa = fabs(AP_AC);
-      const double den = aa >= kSqrtTiny ? aa : kSqrtTiny;
-      const double inv = 1.0 / den;
-      const double Ds = R.D * AP_VR * inv;
-      double Vs[3], Ss[3];
+      Xb.D += Dsb * AV * inv;
+      AVb += Dsb * XD * inv;
+      invb += Dsb * XD * AV;
+      if (aa >= kSqrtTiny) AACb += -invb * inv * inv * sgn1(AAC);
+      ACb += s * AACb;
+      vb += s * AVb;
+      const double Ab = AACb + AVb;
+      // scatter onto the side's bars
+      auto onL = [lft](double x) { return lft ? x : 0.0; };
+      auto onR = [lft](double x) { return lft ? 0.0 : x; };
+      AMb += onL(Ab);
+      APb += onR(Ab);
+      vdLb += onL(vb);
+      vdRb += onR(vb);
+      Lb.D += onL(Xb.D);
+      Rb.D += onR(Xb.D);
+      Lb.P += onL(Xb.P);
+      Rb.P += onR(Xb.P);
+      Lb.G += onL(Xb.G);
+      Rb.G += onR(Xb.G);
       for (int k = 0; k < 3; k++) {
-        Vs[k] = k == d ? AC : R.V[k];
-        Ss[k] = mass * Ds * Vs[k];
+        Lb.S[k] += onL(Xb.S[k]);
+        Rb.S[k] += onR(Xb.S[k]);
+        Lb.V[k] += onL(Xb.V[k]);
+        Rb.V[k] += onR(Xb.V[k]);
       }
-      const double Ps = R.P - SdR * AP_VR + Ss[d] * AP_AC;
-      const double t = R.G * AP_VR - vdR * R.P + AC * Ps;
-      const double Gs = t * inv;
-      raw(Ds, Ss, Gs, Ps, AC, d, F);
-      raw_vjp(Ds, Ss, Gs, Ps, AC, d, Fb, Dsb, Ssb, Gsb, Psb, ACb);
-      double AP_VRb = 0.0, AP_ACb = 0.0;
-      const double tb = Gsb * inv;
-      double invb = Gsb * t;
-      Rb.G += tb * AP_VR;
-      AP_VRb += tb * R.G;
-      vdRb -= tb * R.P;
-      Rb.P -= tb * vdR;
-      ACb += tb * Ps;
-      Psb += tb * AC;
-      Rb.P += Psb;
-      Rb.S[d] -= Psb * AP_VR;
-      AP_VRb -= Psb * SdR;
-      Ssb[d] += Psb * AP_AC;
-      AP_ACb += Psb * Ss[d];
-      for (int k = 0; k < 3; k++) {
-        Dsb += Ssb[k] * mass * Vs[k];
-        const double vb = Ssb[k] * mass * Ds;
-        if (k == d)
-          ACb += vb;
-        else
-          Rb.V[k] += vb;
-      }
-      Rb.D += Dsb * AP_VR * inv;
-      AP_VRb += Dsb * R.D * inv;
-      invb += Dsb * R.D * AP_VR;
-      if (aa >= kSqrtTiny) AP_ACb += -invb * inv * inv * sgn1(AP_AC);
-      APb += AP_ACb;
-      ACb -= AP_ACb;
-      APb += AP_VRb;
-      vdRb -= AP_VRb;
     }
     // AC = (1*SN)*MDinv, MDinv = max(MD, 0)/max(MD^2, tiny)
     const double SNb = ACb * MDinv;

@@ -569,10 +569,10 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     // Shock flags of the next step from the new state.
     StageArgs<R> a3 = base_args<R>(c);
     a3.src = (const R *)c->U;
-    a3.tz = FLAG_TZ;
+    a3.tz = flag_tz(nbx * nby, c->n[2]);
     a3.nbx = nbx;
     a3.nby = nby;
-    a3.nblk = nbx * nby * ((c->n[2] + FLAG_TZ - 1) / FLAG_TZ);
+    a3.nblk = nbx * nby * ((c->n[2] + a3.tz - 1) / a3.tz);
     shock_flags_kernel<R><<<a3.nblk, NT, 0, st>>>(a3, c->flags);
     HIP_TRY(hipGetLastError());
     return GX_OK;

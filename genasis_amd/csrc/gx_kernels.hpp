@@ -296,37 +296,31 @@ __device__ __forceinline__ void face_flux(const R WL[5], const R WR[5], int d, b
     const R SN = AP * SdR + AM * SdL - FSdR + FSdL;
     const R MDinv = Arith<R>::div(fmax(MD, R(0)), fmax(MD * MD, tiny));
     const R AC = R(1) * SN * MDinv;
-    if (AC >= R(0)) {
-      const R AM_VL = AM + vdL;
-      const R AM_AC = AM + AC;
-      const R inv = Arith<R>::div(R(1), fmax(fabs(AM_AC), sqrt_tiny));
-      const R Ds = L.D * AM_VL * inv;
-      const R V1 = d == 0 ? AC : L.V1;
-      const R V2 = d == 1 ? AC : L.V2;
-      const R V3 = d == 2 ? AC : L.V3;
-      const R S1 = mass * Ds * V1;
-      const R S2 = mass * Ds * V2;
-      const R S3 = mass * Ds * V3;
-      const R Sds = sel3(d, S1, S2, S3);
-      const R Ps = L.P + SdL * AM_VL - Sds * AM_AC;
-      const R Gs = (L.G * AM_VL + vdL * L.P - AC * Ps) * inv;
-      raw_flux(Ds, S1, S2, S3, Gs, Ps, AC, d, F);
-    } else {
-      const R AP_VR = AP - vdR;
-      const R AP_AC = AP - AC;
-      const R inv = Arith<R>::div(R(1), fmax(fabs(AP_AC), sqrt_tiny));
-      const R Ds = Rt.D * AP_VR * inv;
-      const R V1 = d == 0 ? AC : Rt.V1;
-      const R V2 = d == 1 ? AC : Rt.V2;
-      const R V3 = d == 2 ? AC : Rt.V3;
-      const R S1 = mass * Ds * V1;
-      const R S2 = mass * Ds * V2;
-      const R S3 = mass * Ds * V3;
-      const R Sds = sel3(d, S1, S2, S3);
-      const R Ps = Rt.P - SdR * AP_VR + Sds * AP_AC;
-      const R Gs = (Rt.G * AP_VR - vdR * Rt.P + AC * Ps) * inv;
-      raw_flux(Ds, S1, S2, S3, Gs, Ps, AC, d, F);
-    }
+    // The star state on the side of the contact speed, written once for both
+    // sides: side X = L with A = AM and s = +1 when AC >= 0, else X = R with
+    // A = AP and s = -1.  A + s v is AM + vL or AP - vR, and (P + s a) - s b
+    // is (P + a) - b or (P - a) + b, bit for bit (s x is exact), so a wave
+    // whose faces take both sides runs one star state, not two.
+    const bool lft = AC >= R(0);
+    const R s = lft ? R(1) : R(-1);
+    const R A = lft ? AM : AP;
+    const R v = lft ? vdL : vdR;
+    const R XD = lft ? L.D : Rt.D, XP = lft ? L.P : Rt.P, XG = lft ? L.G : Rt.G;
+    const R XSd = lft ? SdL : SdR;
+    const R AV = A + s * v;
+    const R AAC = A + s * AC;
+    const R inv = Arith<R>::div(R(1), fmax(fabs(AAC), sqrt_tiny));
+    const R Ds = XD * AV * inv;
+    const R V1 = d == 0 ? AC : (lft ? L.V1 : Rt.V1);
+    const R V2 = d == 1 ? AC : (lft ? L.V2 : Rt.V2);
+    const R V3 = d == 2 ? AC : (lft ? L.V3 : Rt.V3);
+    const R S1 = mass * Ds * V1;
+    const R S2 = mass * Ds * V2;
+    const R S3 = mass * Ds * V3;
+    const R Sds = sel3(d, S1, S2, S3);
+    const R Ps = (XP + s * (XSd * AV)) - s * (Sds * AAC);
+    const R Gs = ((XG * AV + s * (v * XP)) - s * (AC * Ps)) * inv;
+    raw_flux(Ds, S1, S2, S3, Gs, Ps, AC, d, F);
     return;
   }
   const R UL[5] = {L.D, L.S1, L.S2, L.S3, L.G};

@@ -715,6 +715,15 @@ constexpr int QW = TX + 3, QH = TY + 3;  // primitives of x0-2 .. x0+TX, y0-2 ..
 constexpr int BW = TX + 2, BH = TY + 2;  // tests of x0-1 .. x0+TX, y0-1 .. y0+TY
 constexpr int FLAG_TZ = 32;              // planes per block
 
+// Planes per single-chart shock-flag block: FLAG_TZ, halved (down to 4) while
+// the launch would hold fewer than 1536 blocks -- six per CU, what its LDS
+// allows -- so small charts still fill the chip.
+inline int flag_tz(int ntile_xy, int nz) {
+  int tz = FLAG_TZ;
+  while (tz > 4 && (long long)ntile_xy * ((nz + tz - 1) / tz) < 1536) tz /= 2;
+  return tz;
+}
+
 template <typename R> struct FlagShared {
   R sQ[2][4][QH * QW];     // P, V_1, V_2, V_3 of planes k and k-1
   uint8_t sB[3][BH * BW];  // tests of plane p at (p + 3) % 3

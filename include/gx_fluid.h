@@ -354,11 +354,20 @@ int gx_adjoint_forward(gx_ctx *ctx, int nSteps, int checkpointEvery, int nRampCy
 /* Reverse pass of the recorded run: given Ubar = dJ/dU_final (five proper-cell
  * fields D, S_1..3, G), returns P0bar = dJ/d(N, V_1..3, E) of the uploaded
  * primitives and *gammaBar = dJ/dAdiabaticIndex.  Recomputes each segment
- * from its checkpoint.  Synchronises. */
+ * from its checkpoint, replaying the shock flags the forward run recorded.
+ * Ubar = NULL takes the seed that gx_adjoint_kinetic_energy left on the
+ * device (GX_E_STATE if there is none); P0bar = NULL leaves the result on
+ * the device for gx_adjoint_get_bar.  Synchronises. */
 int gx_adjoint_backward(gx_ctx *ctx, const double *Ubar, double *P0bar, double *gammaBar);
 
+/* Copies the result of the last completed gx_adjoint_backward (P0bar, five
+ * proper-cell fields) to the host.  Synchronises. */
+int gx_adjoint_get_bar(gx_ctx *ctx, double *P0bar);
+
 /* The observable J = sum over cells with lo < y_c < hi of 1/2 N V_comp^2 dV
- * (comp 0..2) on the current state, and Ubar = dJ/dU.  Synchronises. */
+ * (comp 0..2) on the current state, and Ubar = dJ/dU (copied to the host
+ * unless Ubar is NULL; it stays on the device as the seed of the next
+ * gx_adjoint_backward either way).  Synchronises. */
 int gx_adjoint_kinetic_energy(gx_ctx *ctx, int component, double lo, double hi,
                               double *loss, double *Ubar);
 

@@ -129,3 +129,26 @@ def test_kelvin_helmholtz_32_50_steps_matches_autograd():
     assert abs(loss - J) <= 1e-13 * abs(J)
     np.testing.assert_allclose(dN, gP[0], rtol=1e-8, atol=1e-11 * np.abs(gP[0]).max())
     assert abs(dg - gg) <= 1e-8 * abs(gg), (dg, gg)
+
+
+def test_device_resident_seed_and_result():
+    """The seed dJ/dU left on the device by the observable and the result
+    kept there give the same gradient as the host round trip; a consumed
+    seed is refused."""
+    from genasis_amd._capi import GxError
+    chart = Chart((12, 10, 8))
+    prim = problems.kelvin_helmholtz(chart, seed=5, padded=False)
+    st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8))
+    st.set_primitive(*prim)
+    adj = DiscreteAdjoint(st)
+    adj.forward(5, 2)
+    loss, ub = adj.kinetic_energy()
+    pb, gb = adj.backward(ub)
+    loss2, none = adj.kinetic_energy(keep_on_device=True)
+    assert none is None and loss2 == loss
+    pb2, gb2 = adj.backward(None, fetch=False)
+    assert pb2 is None and gb2 == gb
+    assert np.array_equal(adj.get_bar(), pb)
+    with pytest.raises(GxError):
+        adj.backward(None)
+    st.close()
