@@ -61,6 +61,7 @@ struct AdjointState {
   double *trash = nullptr;
   double *Vp = nullptr, *Vb = nullptr;  // primitives and their bars, 5 np
   double *Ub = nullptr, *K1b = nullptr, *K2b = nullptr, *P0b = nullptr;  // 5 ncell
+  int seg_held = -1;           // first step of the segment the buffers hold
   bool seed_ready = false;     // Ub holds dJ/dU_final (gx_adjoint_kinetic_energy)
   bool bar_ready = false;      // P0b holds the last reverse pass's result
   double *partials = nullptr;  // 2 per line block, both stages, three dimensions
@@ -82,8 +83,6 @@ __global__ __launch_bounds__(AB) void adj_fill_kernel(double *p, long long n, do
        i += (long long)gridDim.x * AB)
     p[i] = v;
 }
-
-__global__ void adj_fill_int_kernel(int *p, int v) { *p = v; }
 
 // Primitives of the stage input on the padded layout, ghosts by the ghost
 // rule (cells with two or more coordinates outside are unused and 0).
@@ -131,47 +130,45 @@ __device__ inline double cell_width(const StageArgs<double> &a, int e, int c) {
 // bars in the segment that owns its cell c.
 constexpr int LINE_OUT = 58;  // complete cells per wave
 
-// The limiter's reverse with the geometry reciprocals: the forward values
-// that pick the branch (central slope, the two one-sided quotients) are the
-// forward's own exactly rounded quotients (div_rcp); the bars multiply by
-// the reciprocals (a gradient needs no exactly rounded quotient).
-__device__ __forceinline__ void slope_vjp_r(double vm, double v0, double vp, double dxl2,
-                                            double dxr2, double dxl, double dxr, double den,
-                                            double rdxl, double rdxr, double rden,
-                                            double theta, int use_limiter, double sb,
-                                            double &vmb, double &v0b, double &vpb) {
+// The limited slope of one cell and its reverse multipliers: s = slope(...)
+// exactly as the forward computes it (the three exactly rounded quotients
+// div_rcp gives, which also pick the limiter's branch), and kl, kr with
+// ds = kl d(dVL) + kr d(dVR), dVL = v0 - vm, dVR = vp - v0.  Branch-free:
+// every case is formed and selected, so lanes on different limiter branches
+// do not serialise.  Multipliers use the geometry reciprocals (a gradient
+// needs no exactly rounded quotient).  Rules at ties as in the header of
+// gx_adjoint.hpp: the centred slope when it equals the smaller one-sided
+// one; where exactly one difference is zero, the mean of the two sides.
+__device__ __forceinline__ double slope_fwd_rev(double vm, double v0, double vp, double dxl,
+                                                double dxr, double dxl2, double dxr2,
+                                                double den, double rdxl, double rdxr,
+                                                double rden, double theta, int use_limiter,
+                                                double &kl, double &kr) {
   const double dVL = v0 - vm, dVR = vp - v0;
-  double dVLb = 0.0, dVRb = 0.0;
-  if (!use_limiter) {
-    dVLb = (sb * dxr2) * rden;
-    dVRb = (sb * dxl2) * rden;
-  } else {
-    const bool zL = dVL == 0.0, zR = dVR == 0.0;
-    if (zL && !zR) {
-      dVLb = (sb * theta) * (0.5 * rdxl);
-    } else if (zR && !zL) {
-      dVRb = (sb * theta) * (0.5 * rdxr);
-    } else {
-      const double c = div_rcp(dxr2 * dVL + dxl2 * dVR, den, rden);
-      const double ql = div_rcp(theta * dVL, dxl, rdxl), qr = div_rcp(theta * dVR, dxr, rdxr);
-      const double tl = fabs(ql), tr = fabs(qr);
-      const double m1 = tl <= tr ? tl : tr;
-      const double factor = (signbit(dVL) ? -0.5 : 0.5) + (signbit(dVR) ? -0.5 : 0.5);
-      const double mb = sb * factor;
-      if (fabs(c) <= m1) {
-        const double cb = mb * adj::sgn1(c);
-        dVLb = (cb * dxr2) * rden;
-        dVRb = (cb * dxl2) * rden;
-      } else if (tl <= tr) {
-        dVLb = ((mb * adj::sgn1(ql)) * theta) * rdxl;
-      } else {
-        dVRb = ((mb * adj::sgn1(qr)) * theta) * rdxr;
-      }
-    }
+  const double c = div_rcp(dxr2 * dVL + dxl2 * dVR, den, rden);
+  if (!use_limiter) {  // uniform over the launch
+    kl = dxr2 * rden;
+    kr = dxl2 * rden;
+    return c;
   }
-  vmb -= dVLb;
-  v0b += dVLb - dVRb;
-  vpb += dVRb;
+  const double ql = div_rcp(theta * dVL, dxl, rdxl), qr = div_rcp(theta * dVR, dxr, rdxr);
+  const double tl = fabs(ql), tr = fabs(qr), ac = fabs(c);
+  const double sf = copysign(0.5, dVL) + copysign(0.5, dVR);
+  const double s = sf * fmin(fmin(tl, tr), ac);
+  const double m1 = tl <= tr ? tl : tr;
+  const bool zL = dVL == 0.0, zR = dVR == 0.0;
+  const bool cen = ac <= m1, lef = !cen && tl <= tr;
+  const double factor = (signbit(dVL) ? -0.5 : 0.5) + (signbit(dVR) ? -0.5 : 0.5);
+  const double fc = factor * adj::sgn1(c);
+  double l = cen ? (fc * dxr2) * rden : (lef ? ((factor * adj::sgn1(ql)) * theta) * rdxl : 0.0);
+  double r = cen ? (fc * dxl2) * rden : (lef ? 0.0 : ((factor * adj::sgn1(qr)) * theta) * rdxr);
+  if (zL != zR) {  // exactly one difference is zero: one-sided, mean of the sides
+    l = zL ? theta * (0.5 * rdxl) : 0.0;
+    r = zL ? 0.0 : theta * (0.5 * rdxr);
+  }
+  kl = l;
+  kr = r;
+  return s;
 }
 
 template <int D>
@@ -187,6 +184,7 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
     Vq[f] = __shfl_down(V[f], 1, WAVE);
   }
   const bool has_slope = c >= -1 && c <= n && lane >= 1 && lane <= 62;
+  double kl[5], kr[5];  // the limiter's reverse multipliers of cell c
   {
     const double dxl = has_slope ? geo(a, D, G_DXL, c + 2) : 1.0;
     const double dxr = has_slope ? geo(a, D, G_DXR, c + 2) : 1.0;
@@ -197,10 +195,11 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
     const double rdxr = has_slope ? geo(a, D, G_RDXR, c + 2) : 1.0;
     const double rden = has_slope ? geo(a, D, G_RDEN, c + 2) : 1.0;
 #pragma unroll
-    for (int f = 0; f < 5; f++)
-      s[f] = has_slope ? slope(Vm[f], V[f], Vq[f], dxl, dxr, dxl2, dxr2, den, rdxl, rdxr, rden,
-                               a.ph.theta, a.ph.use_limiter)
-                       : 0.0;
+    for (int f = 0; f < 5; f++) {
+      s[f] = slope_fwd_rev(Vm[f], V[f], Vq[f], dxl, dxr, dxl2, dxr2, den, rdxl, rdxr, rden,
+                           a.ph.theta, a.ph.use_limiter, kl[f], kr[f]);
+      if (!has_slope) s[f] = kl[f] = kr[f] = 0.0;
+    }
   }
 #pragma unroll
   for (int f = 0; f < 5; f++) {
@@ -219,12 +218,12 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
       Gf[f] = 0.0;
     }
     if (c < n) {
-      const double r = A / volc;
+      const double r = A * adj::rcp_rev(volc);
 #pragma unroll
       for (int f = 0; f < 5; f++) Gf[f] += r * Kb[f];
     }
     if (c >= 1) {
-      const double r = A / volm;
+      const double r = A * adj::rcp_rev(volm);
 #pragma unroll
       for (int f = 0; f < 5; f++) Gf[f] -= r * Kbm[f];
     }
@@ -249,18 +248,15 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   if (has_slope && lane >= 2 && lane <= 61) {
     const int q = c + 2;
     const double wr = geo(a, D, G_WR, q), wl = geo(a, D, G_WL, q);
-    const double dxl = geo(a, D, G_DXL, q), dxr = geo(a, D, G_DXR, q);
-    const double dxl2 = geo(a, D, G_DXL2, q), dxr2 = geo(a, D, G_DXR2, q);
-    const double den = geo(a, D, G_DEN, q);
-    const double rdxl = geo(a, D, G_RDXL, q), rdxr = geo(a, D, G_RDXR, q);
-    const double rden = geo(a, D, G_RDEN, q);
 #pragma unroll
     for (int f = 0; f < 5; f++) {
       double sb = 0.0;
       if (c <= n - 1) sb += wr * up[f];
       if (c >= 0) sb -= wl * WRb[f];
-      slope_vjp_r(Vm[f], V[f], Vq[f], dxl2, dxr2, dxl, dxr, den, rdxl, rdxr, rden, a.ph.theta,
-                  a.ph.use_limiter, sb, dm[f], d0[f], dp[f]);
+      const double dVLb = sb * kl[f], dVRb = sb * kr[f];
+      dm[f] = -dVLb;
+      d0[f] = dVLb - dVRb;
+      dp[f] = dVRb;
     }
   }
 #pragma unroll
@@ -332,136 +328,78 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
   }
 }
 
-// D = 1, 2: TRW columns (x) per block, each holding 64 consecutive positions
+// D = 1, 2 without staging: wave (x, segment) holds 64 consecutive positions
 // of the sequence of lines along D (cells -2 .. n+1 of line o, then of line
-// o+1, ...), loaded and stored along x through LDS (64-byte row pieces);
-// each of the four waves works on two columns in turn.  Ghost positions
-// along D are touched by this pass alone and are set; interior cells add to
-// the x pass.
-constexpr int TRW = 8;          // columns (x) per block
-constexpr int TRP = TRW + 1;    // padded LDS row
-constexpr int TRWAVES = 4;      // waves per block
+// o+1, ...) in column x; the four waves of a block take four consecutive x
+// at the same positions, so each strided load of a wave shares its cache
+// lines with the other three.  Ghost positions along D are touched by this
+// pass alone and are set; interior cells add to the x pass.
+constexpr int DXW = AB / WAVE;  // x columns per block
 template <int D>
-__global__ __launch_bounds__(TRWAVES * WAVE) void adj_line_yz_kernel(StageArgs<double> a,
-                                                                      PadGeom g,
-                                                                      const double *Vp,
-                                                                      const double *Kb,
-                                                                      double dt, double *Vb,
-                                                                      double *partial) {
-  constexpr int NTH = TRWAVES * WAVE;
+__global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, PadGeom g,
+                                                        const double *Vp, const double *Kb,
+                                                        double dt, double *Vb,
+                                                        double *partial) {
   constexpr int O = D == 1 ? 2 : 1;  // the other transverse dimension
-  __shared__ double sV[5][WAVE][TRP];
-  __shared__ double sK[5][WAVE][TRP];
-  __shared__ unsigned char sF[WAVE][TRP];
-  __shared__ double sh[NTH];
+  __shared__ double sh[AB];
   const int n = g.n[D];
   const int L = n + 4;
   const int T = L * g.n[O];
-  const int nxb = (g.n[0] + TRW - 1) / TRW;
-  // block -> (x block, segment), XCD-aware: neighbouring x blocks share the
-  // cache lines of their rows and run on one L2
+  const int nxb = (g.n[0] + DXW - 1) / DXW;
   const int b = tile_of_block(blockIdx.x, gridDim.x);
   const int xb = b % nxb, sgi = b / nxb;
-  const int x0 = xb * TRW, p0 = sgi * LINE_OUT - 3;
-  const int tid = threadIdx.x;
-  auto pidx = [&](int x, int c, int o) {
+  const int lane = threadIdx.x % WAVE;
+  const int x = xb * DXW + threadIdx.x / WAVE;
+  const int p = sgi * LINE_OUT - 3 + lane;
+  double dtb = 0.0, gbs = 0.0;
+  if (x < g.n[0]) {  // uniform over the wave
+    const bool valid = p >= 0 && p < T;
+    const int o = valid ? p / L : 0;
+    const int c = valid ? p - o * L - 2 : -100;  // -100: in no cell range
     int q[3];
     q[0] = x;
-    q[D] = c;
+    q[D] = valid ? c : 0;
     q[O] = o;
-    return plin(g, q[0], q[1], q[2]);
-  };
-  auto lidx = [&](int x, int c, int o) {
-    int q[3];
-    q[0] = x;
-    q[D] = c;
-    q[O] = o;
-    return lin(a.n, q[0], q[1], q[2]);
-  };
-  // cooperative loads: item -> (x = item % TRW, lane row l = item / TRW)
-  for (int item = tid; item < TRW * WAVE; item += NTH) {
-    const int xi = item % TRW, l = item / TRW;
-    const int x = x0 + xi, p = p0 + l;
-    const bool okV = x < g.n[0] && p >= 0 && p < T;
-    const int o = okV ? p / L : 0, c = okV ? p - o * L - 2 : -100;
-    const bool okK = okV && c >= 0 && c < n;
-    const long long pv = okV ? pidx(x, c, o) : 0;
-    const unsigned ik = okK ? lidx(x, c, o) : 0u;
+    const long long pv = plin(g, q[0], q[1], q[2]);
+    double V[5], K[5], Vold[5];
+    bool df = false;
+    const bool interior = c >= 0 && c < n;
+    const bool own = valid && lane >= 3 && lane <= 60;
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      sV[f][l][xi] = okV ? Vp[f * g.np + pv] : 0.0;
-      sK[f][l][xi] = okK ? Kb[f * a.fs + ik] : 0.0;
+      V[f] = valid ? Vp[f * g.np + pv] : 0.0;
+      K[f] = 0.0;
+      // the x pass's bars of interior cells, fetched now so that their
+      // (strided) latency overlaps the sweep
+      Vold[f] = own && interior ? Vb[f * g.np + pv] : 0.0;
     }
-    sF[l][xi] = okK ? (unsigned char)((a.flags[ik] >> D) & 1) : (unsigned char)0;
-  }
-  __syncthreads();
-  const int wv = tid / WAVE, lane = tid % WAVE;
-  const int p = p0 + lane;
-  const bool valid = p >= 0 && p < T;
-  const int o = valid ? p / L : 0, c = valid ? p - o * L - 2 : -100;
-  const bool own = valid && lane >= 3 && lane <= 60;
-  double dtb = 0.0, gbs = 0.0;
-  const double wo = cell_width(a, O, o);
-  for (int col = wv; col < TRW; col += TRWAVES) {
-    const int x = x0 + col;
-    double out[5] = {0, 0, 0, 0, 0};
-    if (x < g.n[0]) {  // uniform over the wave
-      double V[5], K[5];
+    if (interior) {
+      const unsigned ik = lin(a.n, q[0], q[1], q[2]);
 #pragma unroll
-      for (int f = 0; f < 5; f++) {
-        V[f] = sV[f][lane][col];
-        K[f] = sK[f][lane][col];
-      }
-      const bool df = sF[lane][col] != 0;
-      const double wx = cell_width(a, 0, x);
-      const double A = D == 1 ? wo * wx : wx * wo;  // D=1: wz*wx; D=2: wx*wy
-      double volc = 0.0, volm = 0.0;
-      auto vol_of = [&](int cc) {
-        const double wd = cell_width(a, D, cc);
-        return D == 1 ? (wx * wd) * wo : (wx * wo) * wd;
-      };
-      if (c >= 0 && c < n) volc = vol_of(c);
-      if (c >= 1 && c <= n) volm = vol_of(c - 1);
-      line_vjp<D>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
+      for (int f = 0; f < 5; f++) K[f] = Kb[f * a.fs + ik];
+      df = (a.flags[ik] >> D) & 1;
     }
-    // this wave alone reads and writes column col
+    const double wo = cell_width(a, O, o);
+    const double wx = cell_width(a, 0, x);
+    const double A = D == 1 ? wo * wx : wx * wo;  // D=1: wz*wx; D=2: wx*wy
+    auto vol_of = [&](int cc) {
+      const double wd = cell_width(a, D, cc);
+      return D == 1 ? (wx * wd) * wo : (wx * wo) * wd;
+    };
+    const double volc = (c >= 0 && c < n) ? vol_of(c) : 0.0;
+    const double volm = (c >= 1 && c <= n) ? vol_of(c - 1) : 0.0;
+    double out[5];
+    line_vjp<D>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
+    if (own) {
 #pragma unroll
-    for (int f = 0; f < 5; f++) sV[f][lane][col] = out[f];
-  }
-  __syncthreads();
-  for (int item = tid; item < TRW * WAVE; item += NTH) {
-    const int xi = item % TRW, l = item / TRW;
-    const int xx = x0 + xi, pp = p0 + l;
-    if (xx < g.n[0] && l >= 3 && l <= 60 && pp >= 0 && pp < T) {
-      const int oo = pp / L, cc = pp - oo * L - 2;
-      const long long pv = pidx(xx, cc, oo);
-      if (cc >= 0 && cc < n) {
-#pragma unroll
-        for (int f = 0; f < 5; f++) Vb[f * g.np + pv] += sV[f][l][xi];
-      } else {
-#pragma unroll
-        for (int f = 0; f < 5; f++) Vb[f * g.np + pv] = sV[f][l][xi];
-      }
+      for (int f = 0; f < 5; f++) Vb[f * g.np + pv] = Vold[f] + out[f];
     }
   }
-  // block sums of the dt and gamma bars
-  sh[tid] = dtb;
-  __syncthreads();
-  for (int s2 = NTH / 2; s2 > 0; s2 >>= 1) {
-    if (tid < s2) sh[tid] += sh[tid + s2];
-    __syncthreads();
-  }
-  const double t1 = sh[0];
-  __syncthreads();
-  sh[tid] = gbs;
-  __syncthreads();
-  for (int s2 = NTH / 2; s2 > 0; s2 >>= 1) {
-    if (tid < s2) sh[tid] += sh[tid + s2];
-    __syncthreads();
-  }
-  if (tid == 0) {
-    partial[2 * blockIdx.x] = t1;
-    partial[2 * blockIdx.x + 1] = sh[0];
+  const double s1 = block_sum(dtb, sh);
+  const double s2 = block_sum(gbs, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s1;
+    partial[2 * blockIdx.x + 1] = s2;
   }
 }
 
@@ -470,10 +408,13 @@ __global__ __launch_bounds__(TRWAVES * WAVE) void adj_line_yz_kernel(StageArgs<d
 // then the conserved -> primitive reverse (or the primitive bar itself for an
 // uploaded-primitive input): y.  Stage 1 adds y to out.  Stage 2 (K2b given)
 // ends the RK reverse too: with Yb = y, Wb = 2 K2b (Y = U + K1, W = U + K1/2,
-// Z = W + K2/2), out = Ub = Yb + Wb and K1b = Yb + Wb/2.
+// Z = W + K2/2), out = Ub = Yb + Wb and K1b = Yb + Wb/2.  With idx given
+// (stage 1), it also finds the first cell (x fastest) whose CFL term has the
+// bits of the step's maximum (atomicMin; *idx starts at INT_MAX).
 __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGeom g,
                                                       const double *Vb, double *out,
-                                                      const double *K2b, double *K1b) {
+                                                      const double *K2b, double *K1b,
+                                                      unsigned long long bits, int *idx) {
   const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
   for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
        t += (long long)gridDim.x * AB) {
@@ -518,6 +459,13 @@ __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGe
       for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];
       adj::prim_vjp(U, a.ph.mass, a.ph.nmin, V, y);
     }
+    if (idx) {
+      double P[5];
+      load_prim(a, c3[0], c3[1], c3[2], P);
+      const double tc = cfl_term(P, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, c3[0] + 2),
+                                 geo(a, 1, G_WIDTH, c3[1] + 2), geo(a, 2, G_WIDTH, c3[2] + 2));
+      if ((unsigned long long)__double_as_longlong(tc) == bits) atomicMin(idx, (int)t);
+    }
     if (K2b) {
 #pragma unroll
       for (int f = 0; f < 5; f++) {
@@ -550,22 +498,6 @@ __global__ __launch_bounds__(AB) void adj_fix_kernel(long long ncell, double mas
   }
 }
 
-// First cell (x fastest) whose CFL term has the bits of the maximum.
-__global__ __launch_bounds__(AB) void adj_argmax_kernel(StageArgs<double> a,
-                                                        unsigned long long bits, int *idx) {
-  const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
-  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
-       t += (long long)gridDim.x * AB) {
-    const int x = (int)(t % a.n[0]), y = (int)((t / a.n[0]) % a.n[1]),
-              z = (int)(t / ((long long)a.n[0] * a.n[1]));
-    double V[5];
-    load_prim(a, x, y, z, V);
-    const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, x + 2),
-                               geo(a, 1, G_WIDTH, y + 2), geo(a, 2, G_WIDTH, z + 2));
-    if ((unsigned long long)__double_as_longlong(tc) == bits) atomicMin(idx, (int)t);
-  }
-}
-
 // First level of the dt and gamma bar sums: block b adds the pairs of its
 // contiguous chunk of the per-block partials (fixed order).
 constexpr int PART_BLOCKS = 128;
@@ -593,7 +525,7 @@ __global__ __launch_bounds__(AB) void adj_partsum_kernel(const double *partial, 
 // out through the conserved -> primitive reverse; gamma bars into *gbar.
 __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const double *partial,
                                                     int nparts, double ts, double invdt,
-                                                    const int *idx, double *out,
+                                                    int *idx, double *out,
                                                     double *gbar) {
   __shared__ double sh[AB];
   double sd = 0.0, sg = 0.0;
@@ -626,6 +558,7 @@ __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const d
     }
   }
   *gbar += gb;
+  *idx = 0x7fffffff;  // ready for the next step's fold
 }
 
 // ComputeFromInitial's reverse: D = N, S_k = (M N) V_k, G = E + S.V/2 with
@@ -766,7 +699,7 @@ static int line_blocks(const PadGeom &g, int d) {
   const long long other = d == 0 ? (long long)g.n[1] * g.n[2] : g.n[d == 1 ? 2 : 1];
   const long long nseg = ((g.n[d] + 4) * other + LINE_OUT - 1) / LINE_OUT;
   if (d == 0) return (int)((nseg + AB / WAVE - 1) / (AB / WAVE));
-  const long long nxb = (g.n[0] + TRW - 1) / TRW;
+  const long long nxb = (g.n[0] + DXW - 1) / DXW;
   return (int)(nxb * nseg);
 }
 
@@ -774,7 +707,8 @@ static int line_blocks(const PadGeom &g, int d) {
 // part (3 face passes).
 static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, double dt,
                      const uint8_t *flags, double *out, double *part,
-                     const double *K2b = nullptr, double *K1b = nullptr) {
+                     const double *K2b = nullptr, double *K1b = nullptr,
+                     unsigned long long cfl_bits = 0, int *argmax = nullptr) {
   AdjointState *A = c->adj;
   const PadGeom &g = A->pg;
   StageArgs<double> a = base_args<double>(c);
@@ -791,14 +725,13 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, 
     if (d == 0)
       adj_line_x_kernel<<<nb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb, pp);
     else if (d == 1)
-      adj_line_yz_kernel<1><<<nb, TRWAVES * WAVE, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb,
-                                                                  pp);
+      adj_line_t_kernel<1><<<nb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb, pp);
     else
-      adj_line_yz_kernel<2><<<nb, TRWAVES * WAVE, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb,
-                                                                  pp);
+      adj_line_t_kernel<2><<<nb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb, pp);
     HIP_TRY(hipGetLastError());
   }
-  adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, g, A->Vb, out, K2b, K1b);
+  adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, g, A->Vb, out, K2b, K1b,
+                                                                cfl_bits, argmax);
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }
@@ -839,6 +772,10 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   HIP_TRY(hipMalloc(&A->partials2, sizeof(double) * 2 * PART_BLOCKS));
   HIP_TRY(hipMalloc(&A->gb, sizeof(double)));
   HIP_TRY(hipMalloc(&A->argmax, sizeof(int)));
+  {
+    const int none = 0x7fffffff;  // the dt kernel restores it after each use
+    HIP_TRY(hipMemcpy(A->argmax, &none, sizeof(int), hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipMalloc(&A->flags, (size_t)c->ncell));
   HIP_TRY(hipMalloc(&A->band, sizeof(double) * c->n[1]));
   HIP_TRY(hipMalloc(&A->lossp, sizeof(double) * blocks_for(c->ncell)));
@@ -858,6 +795,10 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
   if (rc) return rc;
   AdjointState *A = c->adj;
   const size_t s5 = 5 * sizeof(double) * (size_t)c->ncell;
+  const long long n5 = 5 * c->ncell;
+  // the last segment, whose states the reverse pass needs first, is kept
+  // in the segment buffers as it is run, not recomputed
+  const int last0 = ((nSteps - 1) / checkpointEvery) * checkpointEvery;
   double time = 0.0;
   for (int n = 0; n < nSteps; n++) {
     if (n % checkpointEvery == 0)
@@ -882,11 +823,25 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
     A->ts.push_back(ts);
     A->invdt.push_back(invdt);
     A->ramp.push_back(ramp);
-    rc = gx_step_rk2(c, time, dt);
+    if (n < last0) {
+      rc = gx_step_rk2(c, time, dt);
+    } else {
+      const int j = n - last0;
+      if (j == 0)
+        HIP_TRY(hipMemcpyAsync(A->seg, c->U, s5, hipMemcpyDeviceToDevice, c->stream));
+      StepRecord rec;
+      rec.Uin = A->seg + (size_t)j * n5;
+      rec.Y = A->segY + (size_t)j * n5;
+      rec.W = A->segW + (size_t)j * n5;
+      rec.Z = A->segZ + (size_t)j * n5;
+      rec.Uout = n + 1 < nSteps ? A->seg + (size_t)(j + 1) * n5 : (double *)c->U;
+      rc = step_rk2(c, dt, &rec);
+    }
     if (rc) return rc;
     time = time + dt;
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
+  A->seg_held = last0;
   A->forward_done = true;
   return GX_OK;
 }
@@ -914,10 +869,12 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   for (int seg0 = ((N - 1) / every) * every; seg0 >= 0; seg0 -= every) {
     const int len = std::min(every, N - seg0);
     // Recompute the segment from its checkpoint with the recorded flags,
-    // keeping per step the input U^n, Y, W and the state before the fix Z.
-    HIP_TRY(hipMemcpyAsync(A->seg, A->ckpt[seg0 / every], s5, hipMemcpyDeviceToDevice,
-                           c->stream));
-    for (int j = 0; j < len; j++) {
+    // keeping per step the input U^n, Y, W and the state before the fix Z
+    // (unless the buffers hold it already).
+    if (A->seg_held != seg0)
+      HIP_TRY(hipMemcpyAsync(A->seg, A->ckpt[seg0 / every], s5, hipMemcpyDeviceToDevice,
+                             c->stream));
+    for (int j = 0; j < len && A->seg_held != seg0; j++) {
       const int n = seg0 + j;
       const double *Un = A->seg + (size_t)j * n5;
       double *Un1 = j + 1 < len ? A->seg + (size_t)(j + 1) * n5 : A->trash;
@@ -928,6 +885,7 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       rc = launch_stage2(c, Yn, Wn, Un1, A->segZ + (size_t)j * n5, A->dt[n], Fn);
       if (rc) return rc;
     }
+    A->seg_held = seg0;
     for (int j = len - 1; j >= 0; j--) {
       const int n = seg0 + j;
       const double *Un = A->seg + (size_t)j * n5;
@@ -944,18 +902,16 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
                      A->partials + 6 * (size_t)A->part_stride, A->K2b, A->K1b);
       if (rc) return rc;
       double *out = first ? A->P0b : A->Ub;
-      rc = stage_vjp(c, first ? P0 : Un, first, A->K1b, dt, Fn, out, A->partials);
+      // stage 1's transpose; its fold also finds the cell of the CFL maximum
+      unsigned long long bits;
+      memcpy(&bits, &A->invdt[n], sizeof(bits));
+      rc = stage_vjp(c, first ? P0 : Un, first, A->K1b, dt, Fn, out, A->partials, nullptr,
+                     nullptr, bits, A->argmax);
       if (rc) return rc;
       // dt = ts(invdt(U^n))
       StageArgs<double> a = base_args<double>(c);
       a.src = first ? P0 : Un;
       a.src_is_prim = first ? 1 : 0;
-      adj_fill_int_kernel<<<1, 1, 0, c->stream>>>(A->argmax, 0x7fffffff);
-      HIP_TRY(hipGetLastError());
-      unsigned long long bits;
-      memcpy(&bits, &A->invdt[n], sizeof(bits));
-      adj_argmax_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, bits, A->argmax);
-      HIP_TRY(hipGetLastError());
       adj_partsum_kernel<<<PART_BLOCKS, AB, 0, c->stream>>>(A->partials, 6 * A->part_stride,
                                                             A->partials2);
       HIP_TRY(hipGetLastError());

@@ -29,6 +29,20 @@ constexpr double kSqrtTiny = 1.4916681462400413e-154;
 
 GX_HD double sgn1(double x) { return std::signbit(x) ? -1.0 : 1.0; }
 
+// 1/x for the reverse sweeps' multipliers (x positive and normal), where a
+// gradient needs no correctly rounded quotient: on the device the hardware
+// reciprocal refined by two Newton steps (within an ulp or two), on the host
+// the division.
+GX_HD double rcp_rev(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return fma(y, fma(-x, y, 1.0), y);
+#else
+  return 1.0 / x;
+#endif
+}
+
 // ---- limited slope (Gradient_Kernel.f90:166-190) --------------------------
 // s = (sign(1/2, dVL) + sign(1/2, dVR)) * min(|theta dVL/dxl|, |theta dVR/dxr|,
 // |(dxr^2 dVL + dxl^2 dVR)/den|); adds sb * ds/d(vm, v0, vp) to the bars.
@@ -111,10 +125,12 @@ GX_HD void fstate_vjp(const FState &s, FBar b, double gamma, double mass, double
   }
   double Nb = s.N < 0.0 ? 0.0 : Ncb;
   if (s.ok && s.CS > 0.0) {
-    const double qb = b.CS * 0.5 / s.CS;
-    gb += qb * s.P / (mass * s.N);
-    b.P += qb * gamma / (mass * s.N);
-    Nb -= qb * s.q / s.N;
+    // q = gamma P / (M N), CS = sqrt(q)
+    const double qb = (b.CS * 0.5) * rcp_rev(s.CS);
+    const double rMN = rcp_rev(mass * s.N);
+    gb += (qb * s.P) * rMN;
+    b.P += (qb * gamma) * rMN;
+    Nb -= (qb * s.q) * rcp_rev(s.N);
   }
   b.E += b.P * (gamma - 1.0);
   gb += b.P * s.E;
@@ -265,8 +281,9 @@ GX_HD void face_flux_vjp(const double WL[5], const double WR[5], int d, bool df,
     // AC = (1*SN)*MDinv, MDinv = max(MD, 0)/max(MD^2, tiny)
     const double SNb = ACb * MDinv;
     const double MDinvb = ACb * SN;
-    const double nnb = MDinvb / dd;
-    const double ddb = -MDinvb * MDinv / dd;
+    const double rdd = rcp_rev(dd);
+    const double nnb = MDinvb * rdd;
+    const double ddb = (-MDinvb * MDinv) * rdd;
     double MDb = 0.0;
     if (MD >= 0.0) MDb += nnb;
     if (MD * MD >= kTiny) MDb += ddb * 2.0 * MD;
@@ -288,11 +305,12 @@ GX_HD void face_flux_vjp(const double WL[5], const double WR[5], int d, bool df,
     const double sum = AP + AM;
     const double den = sum >= kTiny ? sum : kTiny;
     double denb = 0.0, ULb[5], URb[5];
+    const double rden = rcp_rev(den);
     for (int f = 0; f < 5; f++) {
       const double dU = UR[f] - UL[f];
       F[f] = (AP * FL[f] + AM * FR[f] - 1.0 * AP * AM * dU) / den;
-      const double numb = Fb[f] / den;
-      denb -= Fb[f] * F[f] / den;
+      const double numb = Fb[f] * rden;
+      denb -= (Fb[f] * F[f]) * rden;
       APb += numb * (FL[f] - AM * dU);
       AMb += numb * (FR[f] - AP * dU);
       FLb[f] += numb * AP;

@@ -62,6 +62,19 @@ struct gx_ctx {
   gx::AdjointState *adj = nullptr;  // discrete adjoint (gx_adjoint_*), on demand
 };
 
+namespace gx {
+// Where one RK2 step reads and writes when a caller records it (the discrete
+// adjoint's forward run): the step input Uin (which it leaves intact), the
+// stage values Y and W, the new state Uout and the state before the
+// conserved fix Z.  Without a record the step works in the context's U, Y, W.
+struct StepRecord {
+  const double *Uin;
+  double *Y, *W, *Uout, *Z;
+};
+// gx_step_rk2 with an optional record (FP64 contexts only when rec is set).
+int step_rk2(gx_ctx *c, double dt, const StepRecord *rec);
+}  // namespace gx
+
 template <typename R>
 inline StageArgs<R> base_args(const gx_ctx *c) {
   StageArgs<R> a;

@@ -512,6 +512,15 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   (void)time;
   if (!c) return fail(GX_E_ARG, "gx_step_rk2: null context");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_step_rk2: no state uploaded");
+  return gx::step_rk2(c, dt, nullptr);
+}
+
+}  // extern "C"
+
+namespace gx {
+
+int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
+  if (rec && c->fp32) return fail(GX_E_ARG, "step_rk2: a recorded step is FP64");
   const dim3 block(NT);
   const int nbx = (c->n[0] + TX - 1) / TX, nby = (c->n[1] + TY - 1) / TY,
             nbz = (c->n[2] + c->tz - 1) / c->tz;
@@ -530,10 +539,13 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     StageArgs<R> a1 = base_args<R>(c);
     a1.dt = dt;
     a1.src_is_prim = c->prims_valid ? 1 : 0;
-    a1.src = (const R *)(c->prims_valid ? c->P0 : c->U);
-    a1.aux = (const R *)c->U;
-    a1.out1 = (R *)c->Y;
-    a1.out2 = (R *)c->W;
+    const R *Uin = rec ? (const R *)rec->Uin : (const R *)c->U;
+    R *Y = rec ? (R *)rec->Y : (R *)c->Y, *W = rec ? (R *)rec->W : (R *)c->W;
+    R *Uout = rec ? (R *)rec->Uout : (R *)c->U;
+    a1.src = c->prims_valid ? (const R *)c->P0 : Uin;
+    a1.aux = Uin;
+    a1.out1 = Y;
+    a1.out2 = W;
     a1.nbx = nbx;
     a1.nby = nby;
     a1.nblk = nblk;
@@ -549,9 +561,10 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     // Stage 2: Current = Y.
     StageArgs<R> a2 = base_args<R>(c);
     a2.dt = dt;
-    a2.src = (const R *)c->Y;
-    a2.aux = (const R *)c->W;
-    a2.out1 = (R *)c->U;
+    a2.src = Y;
+    a2.aux = W;
+    a2.out1 = Uout;
+    a2.out2 = rec ? (R *)rec->Z : nullptr;
     a2.nbx = nbx;
     a2.nby = nby;
     a2.nblk = nblk;
@@ -568,7 +581,7 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
     if (prof) HIP_TRY(hipEventRecord(c->ev[4], st));
     // Shock flags of the next step from the new state.
     StageArgs<R> a3 = base_args<R>(c);
-    a3.src = (const R *)c->U;
+    a3.src = Uout;
     a3.tz = flag_tz(nbx * nby, c->n[2]);
     a3.nbx = nbx;
     a3.nby = nby;
@@ -594,6 +607,11 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   c->invdt_valid = true;
   return GX_OK;
 }
+
+}  // namespace gx
+
+extern "C" {
+
 
 int gx_sync(gx_ctx *c) {
   if (!c) return fail(GX_E_ARG, "gx_sync: null context");
