@@ -410,11 +410,15 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
 // ends the RK reverse too: with Yb = y, Wb = 2 K2b (Y = U + K1, W = U + K1/2,
 // Z = W + K2/2), out = Ub = Yb + Wb and K1b = Yb + Wb/2.  With idx given
 // (stage 1), it also finds the first cell (x fastest) whose CFL term has the
-// bits of the step's maximum (atomicMin; *idx starts at INT_MAX).
+// bits of the step's maximum (atomicMin; *idx starts at INT_MAX).  With Zp
+// given (stage 1, the previous step's state before the fix), it also starts
+// that step's reverse: K2p = fix^T(out)/2 (the dt kernel redoes the one cell
+// it changes afterwards).
 __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGeom g,
                                                       const double *Vb, double *out,
                                                       const double *K2b, double *K1b,
-                                                      unsigned long long bits, int *idx) {
+                                                      unsigned long long bits, int *idx,
+                                                      const double *Zp, double *K2p) {
   const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
   for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
        t += (long long)gridDim.x * AB) {
@@ -474,8 +478,20 @@ __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGe
         K1b[f * ncell + t] = y[f] + k2;
       }
     } else {
+      double o[5];
 #pragma unroll
-      for (int f = 0; f < 5; f++) out[f * ncell + t] += y[f];
+      for (int f = 0; f < 5; f++) {
+        o[f] = out[f * ncell + t] + y[f];
+        out[f * ncell + t] = o[f];
+      }
+      if (Zp) {
+        double z[5], zb[5];
+#pragma unroll
+        for (int f = 0; f < 5; f++) z[f] = Zp[f * ncell + t];
+        adj::fix_vjp(z, a.ph.mass, a.ph.nmin, o, zb);
+#pragma unroll
+        for (int f = 0; f < 5; f++) K2p[f * ncell + t] = 0.5 * zb[f];
+      }
     }
   }
 }
@@ -526,7 +542,8 @@ __global__ __launch_bounds__(AB) void adj_partsum_kernel(const double *partial, 
 __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const double *partial,
                                                     int nparts, double ts, double invdt,
                                                     int *idx, double *out,
-                                                    double *gbar) {
+                                                    double *gbar, const double *Zp,
+                                                    double *K2p) {
   __shared__ double sh[AB];
   double sd = 0.0, sg = 0.0;
   for (int i = threadIdx.x; i < nparts; i += AB) {
@@ -555,6 +572,15 @@ __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const d
       for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];
       adj::prim_vjp(U, a.ph.mass, a.ph.nmin, Vb, Ub);
       for (int f = 0; f < 5; f++) out[f * ncell + t] += Ub[f];
+    }
+    if (Zp) {  // the fold's K2p of this cell predates the dt term
+      double z[5], o[5], zb[5];
+      for (int f = 0; f < 5; f++) {
+        z[f] = Zp[f * ncell + t];
+        o[f] = out[f * ncell + t];
+      }
+      adj::fix_vjp(z, a.ph.mass, a.ph.nmin, o, zb);
+      for (int f = 0; f < 5; f++) K2p[f * ncell + t] = 0.5 * zb[f];
     }
   }
   *gbar += gb;
@@ -708,7 +734,8 @@ static int line_blocks(const PadGeom &g, int d) {
 static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, double dt,
                      const uint8_t *flags, double *out, double *part,
                      const double *K2b = nullptr, double *K1b = nullptr,
-                     unsigned long long cfl_bits = 0, int *argmax = nullptr) {
+                     unsigned long long cfl_bits = 0, int *argmax = nullptr,
+                     const double *Zp = nullptr, double *K2p = nullptr) {
   AdjointState *A = c->adj;
   const PadGeom &g = A->pg;
   StageArgs<double> a = base_args<double>(c);
@@ -731,7 +758,7 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, 
     HIP_TRY(hipGetLastError());
   }
   adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, g, A->Vb, out, K2b, K1b,
-                                                                cfl_bits, argmax);
+                                                                cfl_bits, argmax, Zp, K2p);
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }
@@ -866,6 +893,7 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   if (rc) return rc;
   const double *P0 = (const double *)c->P0;
   const int every = A->every, N = A->nsteps;
+  bool k2_ready = false;  // K2b of the next step to reverse is formed already
   for (int seg0 = ((N - 1) / every) * every; seg0 >= 0; seg0 -= every) {
     const int len = std::min(every, N - seg0);
     // Recompute the segment from its checkpoint with the recorded flags,
@@ -893,21 +921,26 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       const uint8_t *Fn = A->stepF + (size_t)n * c->ncell;
       const bool first = n == 0;
       const double dt = A->dt[n];
-      adj_fix_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
-          c->ncell, c->params.BaryonMassReference, c->params.BaryonDensityMin,
-          A->segZ + (size_t)j * n5, A->Ub, A->K2b);
-      HIP_TRY(hipGetLastError());
+      if (!k2_ready) {
+        adj_fix_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
+            c->ncell, c->params.BaryonMassReference, c->params.BaryonDensityMin,
+            A->segZ + (size_t)j * n5, A->Ub, A->K2b);
+        HIP_TRY(hipGetLastError());
+      }
       // stage 2's transpose, ending with Ub = Yb + Wb and K1b = Yb + Wb/2
       rc = stage_vjp(c, Yn, false, A->K2b, dt, Fn, A->Ub,
                      A->partials + 6 * (size_t)A->part_stride, A->K2b, A->K1b);
       if (rc) return rc;
       double *out = first ? A->P0b : A->Ub;
       // stage 1's transpose; its fold also finds the cell of the CFL maximum
+      // and, when the previous step's Z is in the segment, forms its K2b
       unsigned long long bits;
       memcpy(&bits, &A->invdt[n], sizeof(bits));
+      const double *Zp = j > 0 ? A->segZ + (size_t)(j - 1) * n5 : nullptr;
       rc = stage_vjp(c, first ? P0 : Un, first, A->K1b, dt, Fn, out, A->partials, nullptr,
-                     nullptr, bits, A->argmax);
+                     nullptr, bits, A->argmax, Zp, Zp ? A->K2b : nullptr);
       if (rc) return rc;
+      k2_ready = Zp != nullptr;
       // dt = ts(invdt(U^n))
       StageArgs<double> a = base_args<double>(c);
       a.src = first ? P0 : Un;
@@ -916,7 +949,8 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
                                                             A->partials2);
       HIP_TRY(hipGetLastError());
       adj_dt_kernel<<<1, AB, 0, c->stream>>>(a, A->partials2, PART_BLOCKS, A->ts[n],
-                                             A->invdt[n], A->argmax, out, A->gb);
+                                             A->invdt[n], A->argmax, out, A->gb, Zp,
+                                             Zp ? A->K2b : nullptr);
       HIP_TRY(hipGetLastError());
     }
   }
