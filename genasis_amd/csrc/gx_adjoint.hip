@@ -171,6 +171,21 @@ __device__ __forceinline__ double slope_fwd_rev(double vm, double v0, double vp,
   return s;
 }
 
+// Neighbour lanes by DPP wavefront shifts (wave_shr:1, wave_shl:1), a VALU
+// move instead of an LDS permute: lane_up gives lane i the value of lane
+// i - 1, lane_down that of lane i + 1; the end lanes get 0 and are never
+// used (every use is guarded by the lane range).
+__device__ __forceinline__ double lane_up(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_down(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
 template <int D>
 __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int c, int lane,
                                          bool own, double A, double dt, const double V[5],
@@ -180,8 +195,8 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   double Vm[5], Vq[5], s[5], sm[5], Kbm[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
-    Vm[f] = __shfl_up(V[f], 1, WAVE);
-    Vq[f] = __shfl_down(V[f], 1, WAVE);
+    Vm[f] = lane_up(V[f]);
+    Vq[f] = lane_down(V[f]);
   }
   const bool has_slope = c >= -1 && c <= n && lane >= 1 && lane <= 62;
   double kl[5], kr[5];  // the limiter's reverse multipliers of cell c
@@ -203,8 +218,8 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   }
 #pragma unroll
   for (int f = 0; f < 5; f++) {
-    sm[f] = __shfl_up(s[f], 1, WAVE);
-    Kbm[f] = __shfl_up(Kb[f], 1, WAVE);
+    sm[f] = lane_up(s[f]);
+    Kbm[f] = lane_up(Kb[f]);
   }
   // face (c-1 | c)
   double WLb[5] = {0, 0, 0, 0, 0}, WRb[5] = {0, 0, 0, 0, 0};
@@ -242,7 +257,7 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   double dm[5], d0[5], dp[5], up[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
-    up[f] = __shfl_down(WLb[f], 1, WAVE);  // bar of c as the left state of face c+1
+    up[f] = lane_down(WLb[f]);  // bar of c as the left state of face c+1
     dm[f] = d0[f] = dp[f] = 0.0;
   }
   if (has_slope && lane >= 2 && lane <= 61) {
@@ -261,8 +276,8 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   }
 #pragma unroll
   for (int f = 0; f < 5; f++) {
-    const double from_left = __shfl_up(dp[f], 1, WAVE);    // cell c-1's partial on c
-    const double from_right = __shfl_down(dm[f], 1, WAVE); // cell c+1's partial on c
+    const double from_left = lane_up(dp[f]);    // cell c-1's partial on c
+    const double from_right = lane_down(dm[f]); // cell c+1's partial on c
     double acc = 0.0;
     if (c >= -1 && c <= n - 1) acc += up[f];
     if (c >= 0 && c <= n) acc += WRb[f];
