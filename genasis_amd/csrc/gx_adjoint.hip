@@ -130,9 +130,22 @@ __device__ inline double cell_width(const StageArgs<double> &a, int e, int c) {
 // bars in the segment that owns its cell c.
 constexpr int LINE_OUT = 58;  // complete cells per wave
 
+// a/b as div_rcp forms it (y = RN(1/b)) without its division fallback: for
+// normal a it is div_rcp's quotient, zero numerators keep their signed zero,
+// and tiny nonzero numerators (|a| < 2^-900) may differ from the exact
+// quotient in the last bits -- a limiter branch picked at 1e-270 apart.
+__device__ __forceinline__ double quot_rev(double a, double b, double y) {
+  double q = a * y;
+  double r = fma(-b, q, a);
+  q = fma(r, y, q);
+  r = fma(-b, q, a);
+  q = fma(r, y, q);
+  return a == 0.0 ? a : q;
+}
+
 // The limited slope of one cell and its reverse multipliers: s = slope(...)
-// exactly as the forward computes it (the three exactly rounded quotients
-// div_rcp gives, which also pick the limiter's branch), and kl, kr with
+// as the forward computes it (the quotients div_rcp gives, which also pick
+// the limiter's branch), and kl, kr with
 // ds = kl d(dVL) + kr d(dVR), dVL = v0 - vm, dVR = vp - v0.  Branch-free:
 // every case is formed and selected, so lanes on different limiter branches
 // do not serialise.  Multipliers use the geometry reciprocals (a gradient
@@ -145,13 +158,13 @@ __device__ __forceinline__ double slope_fwd_rev(double vm, double v0, double vp,
                                                 double rden, double theta, int use_limiter,
                                                 double &kl, double &kr) {
   const double dVL = v0 - vm, dVR = vp - v0;
-  const double c = div_rcp(dxr2 * dVL + dxl2 * dVR, den, rden);
+  const double c = quot_rev(dxr2 * dVL + dxl2 * dVR, den, rden);
   if (!use_limiter) {  // uniform over the launch
     kl = dxr2 * rden;
     kr = dxl2 * rden;
     return c;
   }
-  const double ql = div_rcp(theta * dVL, dxl, rdxl), qr = div_rcp(theta * dVR, dxr, rdxr);
+  const double ql = quot_rev(theta * dVL, dxl, rdxl), qr = quot_rev(theta * dVR, dxr, rdxr);
   const double tl = fabs(ql), tr = fabs(qr), ac = fabs(c);
   const double sf = copysign(0.5, dVL) + copysign(0.5, dVR);
   const double s = sf * fmin(fmin(tl, tr), ac);
@@ -869,10 +882,8 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
       rc = gx_step_rk2(c, time, dt);
     } else {
       const int j = n - last0;
-      if (j == 0)
-        HIP_TRY(hipMemcpyAsync(A->seg, c->U, s5, hipMemcpyDeviceToDevice, c->stream));
-      StepRecord rec;
-      rec.Uin = A->seg + (size_t)j * n5;
+      StepRecord rec;  // the first input is the segment's checkpoint
+      rec.Uin = j == 0 ? A->ckpt[last0 / checkpointEvery] : A->seg + (size_t)j * n5;
       rec.Y = A->segY + (size_t)j * n5;
       rec.W = A->segW + (size_t)j * n5;
       rec.Z = A->segZ + (size_t)j * n5;
@@ -912,14 +923,12 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   for (int seg0 = ((N - 1) / every) * every; seg0 >= 0; seg0 -= every) {
     const int len = std::min(every, N - seg0);
     // Recompute the segment from its checkpoint with the recorded flags,
-    // keeping per step the input U^n, Y, W and the state before the fix Z
-    // (unless the buffers hold it already).
-    if (A->seg_held != seg0)
-      HIP_TRY(hipMemcpyAsync(A->seg, A->ckpt[seg0 / every], s5, hipMemcpyDeviceToDevice,
-                             c->stream));
+    // keeping per step the input U^n (the checkpoint itself for the first),
+    // Y, W and the state before the fix Z (unless the buffers hold it).
+    const double *ck = A->ckpt[seg0 / every];
     for (int j = 0; j < len && A->seg_held != seg0; j++) {
       const int n = seg0 + j;
-      const double *Un = A->seg + (size_t)j * n5;
+      const double *Un = j == 0 ? ck : A->seg + (size_t)j * n5;
       double *Un1 = j + 1 < len ? A->seg + (size_t)(j + 1) * n5 : A->trash;
       const uint8_t *Fn = A->stepF + (size_t)n * c->ncell;
       double *Yn = A->segY + (size_t)j * n5, *Wn = A->segW + (size_t)j * n5;
@@ -931,7 +940,7 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
     A->seg_held = seg0;
     for (int j = len - 1; j >= 0; j--) {
       const int n = seg0 + j;
-      const double *Un = A->seg + (size_t)j * n5;
+      const double *Un = j == 0 ? ck : A->seg + (size_t)j * n5;
       const double *Yn = A->segY + (size_t)j * n5;
       const uint8_t *Fn = A->stepF + (size_t)n * c->ncell;
       const bool first = n == 0;

@@ -43,6 +43,36 @@ GX_HD double rcp_rev(double x) {
 #endif
 }
 
+// The forward values a face flux's reverse recomputes (sound speed, contact
+// speed, star and HLL denominators): on the host the forward's IEEE division
+// and square root (so the host evaluation repeats the forward bit for bit);
+// on the device a times rb = rcp_rev(b), and a square root from the hardware
+// reciprocal square root refined by Goldschmidt steps, within an ulp or two,
+// which moves a gradient by rounding only (identical inputs still give
+// identical values, so the ties of identical states stay ties).
+GX_HD double div_rev(double a, double b, double rb) {
+#ifdef __HIP_DEVICE_COMPILE__
+  (void)b;
+  return a * rb;
+#else
+  (void)rb;
+  return a / b;
+#endif
+}
+
+GX_HD double sqrt_rev(double x) {  // x > 0, normal
+#ifdef __HIP_DEVICE_COMPILE__
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  return fma(fma(-g, g, x), h, g);
+#else
+  return sqrt(x);
+#endif
+}
+
 // ---- limited slope (Gradient_Kernel.f90:166-190) --------------------------
 // s = (sign(1/2, dVL) + sign(1/2, dVR)) * min(|theta dVL/dxl|, |theta dVR/dxr|,
 // |(dxr^2 dVL + dxl^2 dVR)/den|); adds sb * ds/d(vm, v0, vp) to the bars.
@@ -102,8 +132,9 @@ GX_HD void fstate(const double W[5], double gamma, double mass, FState &s) {
   s.E = W[4];
   s.P = (gamma - 1.0) * s.E;
   s.ok = s.N > 0.0 && s.P > 0.0;
-  s.q = s.ok ? gamma * s.P / (mass * s.N) : 0.0;
-  s.CS = s.ok ? sqrt(s.q) : 0.0;
+  const double MN = s.ok ? mass * s.N : 1.0;
+  s.q = s.ok ? div_rev(gamma * s.P, MN, rcp_rev(MN)) : 0.0;
+  s.CS = s.ok ? sqrt_rev(s.q) : 0.0;
   s.Nc = s.N < 0.0 ? 0.0 : s.N;
   s.D = s.Nc;
   for (int k = 0; k < 3; k++) s.S[k] = mass * s.Nc * s.V[k];
@@ -196,7 +227,8 @@ GX_HD void face_flux_vjp(const double WL[5], const double WR[5], int d, bool df,
     const double SN = AP * SdR + AM * SdL - FR[1 + d] + FL[1 + d];
     const double nn = MD >= 0.0 ? MD : 0.0;
     const double dd = MD * MD >= kTiny ? MD * MD : kTiny;
-    const double MDinv = nn / dd;
+    const double rdd = rcp_rev(dd);
+    const double MDinv = div_rev(nn, dd, rdd);
     const double AC = 1.0 * SN * MDinv;
     double ACb = 0.0;
     // The star state on the side of the contact speed, written once for
@@ -215,7 +247,7 @@ GX_HD void face_flux_vjp(const double WL[5], const double WR[5], int d, bool df,
       const double AV = A + s * v, AAC = A + s * AC;
       const double aa = fabs(AAC);
       const double den = aa >= kSqrtTiny ? aa : kSqrtTiny;
-      const double inv = 1.0 / den;
+      const double inv = rcp_rev(den);
       const double Ds = XD * AV * inv;
       double Vs[3], Ss[3];
       for (int k = 0; k < 3; k++) {
@@ -281,7 +313,6 @@ GX_HD void face_flux_vjp(const double WL[5], const double WR[5], int d, bool df,
     // AC = (1*SN)*MDinv, MDinv = max(MD, 0)/max(MD^2, tiny)
     const double SNb = ACb * MDinv;
     const double MDinvb = ACb * SN;
-    const double rdd = rcp_rev(dd);
     const double nnb = MDinvb * rdd;
     const double ddb = (-MDinvb * MDinv) * rdd;
     double MDb = 0.0;
@@ -308,7 +339,7 @@ GX_HD void face_flux_vjp(const double WL[5], const double WR[5], int d, bool df,
     const double rden = rcp_rev(den);
     for (int f = 0; f < 5; f++) {
       const double dU = UR[f] - UL[f];
-      F[f] = (AP * FL[f] + AM * FR[f] - 1.0 * AP * AM * dU) / den;
+      F[f] = div_rev(AP * FL[f] + AM * FR[f] - 1.0 * AP * AM * dU, den, rden);
       const double numb = Fb[f] * rden;
       denb -= (Fb[f] * F[f]) * rden;
       APb += numb * (FL[f] - AM * dU);
