@@ -481,19 +481,18 @@ __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGe
         }
       }
     }
-    double y[5] = {0, 0, 0, 0, 0};
+    double y[5] = {0, 0, 0, 0, 0}, U[5];
+#pragma unroll
+    for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];  // the stage input
     if (a.src_is_prim) {
 #pragma unroll
       for (int f = 0; f < 5; f++) y[f] = V[f];
     } else {
-      double U[5];
-#pragma unroll
-      for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];
       adj::prim_vjp(U, a.ph.mass, a.ph.nmin, V, y);
     }
     if (idx) {
       double P[5];
-      load_prim(a, c3[0], c3[1], c3[2], P);
+      raw_to_prim(a, U, -1, P);
       const double tc = cfl_term(P, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, c3[0] + 2),
                                  geo(a, 1, G_WIDTH, c3[1] + 2), geo(a, 2, G_WIDTH, c3[2] + 2));
       if ((unsigned long long)__double_as_longlong(tc) == bits) atomicMin(idx, (int)t);
