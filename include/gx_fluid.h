@@ -212,7 +212,8 @@ int gx_get_boundary_fluence(gx_ctx *ctx, double *fluence);
  * previous total when computeChange.  An upload starts all tallies at zero,
  * so the first call (computeChange = 0) sets total = interior.  Sums are in
  * FP64 with a fixed order.  Any output may be NULL (GX_TALLY_N values each).
- * Steps taken by gx_adjoint_forward are not tallied.  Synchronises. */
+ * Steps taken by gx_adjoint_forward record the boundary fluence and add to
+ * the boundary tally as gx_step_rk2's do.  Synchronises. */
 int gx_compute_tally(gx_ctx *ctx, int computeChange, double *interior,
                      double *boundary, double *total, double *change);
 
