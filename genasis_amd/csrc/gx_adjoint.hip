@@ -50,18 +50,29 @@ __host__ __device__ inline long long plin(const PadGeom &g, int x, int y, int z)
   return (long long)(x + 2) + g.m[0] * ((long long)(y + 2) + (long long)g.m[1] * (z + 2));
 }
 
+// The states of one segment, per step (every steps): the input U^n (slot 0
+// unused: the segment's checkpoint is its first input), the stage values Y
+// and W, and the state before the fix Z.
+struct SegmentBuffers {
+  double *U = nullptr, *Y = nullptr, *W = nullptr, *Z = nullptr;
+  int held = -1;                 // first step of the segment held, or -1
+  hipEvent_t ready = nullptr;    // recorded when its recomputation is done
+};
+
 struct AdjointState {
   int nsteps = 0, every = 0;
   std::vector<double> dt, ts, invdt, ramp;
   std::vector<double *> ckpt;
-  // Per step of one segment (every steps): the input U^n, the stage
-  // values Y and W, and the state before the fix Z.
-  double *seg = nullptr, *segY = nullptr, *segW = nullptr, *segZ = nullptr;
+  // Segment s lives in sb[s % 2]: the reverse pass of one segment runs on the
+  // context's stream while the next (earlier) one is recomputed into the
+  // other set on `side`.
+  SegmentBuffers sb[2];
+  hipStream_t side = nullptr;
+  hipEvent_t mark = nullptr;
   uint8_t *stepF = nullptr;    // the shock flags each step used, nsteps ncell
   double *trash = nullptr;
   double *Vp = nullptr, *Vb = nullptr;  // primitives and their bars, 5 np
   double *Ub = nullptr, *K1b = nullptr, *K2b = nullptr, *P0b = nullptr;  // 5 ncell
-  int seg_held = -1;           // first step of the segment the buffers hold
   bool seed_ready = false;     // Ub holds dJ/dU_final (gx_adjoint_kinetic_energy)
   bool bar_ready = false;      // P0b holds the last reverse pass's result
   double *partials = nullptr;  // 2 per line block, both stages, three dimensions
@@ -69,7 +80,6 @@ struct AdjointState {
   double *partials2 = nullptr; // 2 per block of the first reduction level
   double *gb = nullptr;        // gamma bar
   int *argmax = nullptr;
-  uint8_t *flags = nullptr;
   double *band = nullptr;      // per y-row weight of the observable, n1
   double *lossp = nullptr;     // per-block loss partials
   PadGeom pg;
@@ -683,11 +693,17 @@ void release_adjoint(gx_ctx *c) {
   AdjointState *A = c->adj;
   if (!A) return;
   for (double *p : A->ckpt) (void)hipFree(p);
-  for (void *p : {(void *)A->seg, (void *)A->segY, (void *)A->segW, (void *)A->segZ,
-                  (void *)A->stepF, (void *)A->trash, (void *)A->Vp, (void *)A->Vb,
+  if (A->side) (void)hipStreamSynchronize(A->side);
+  for (SegmentBuffers &b : A->sb) {
+    for (void *p : {(void *)b.U, (void *)b.Y, (void *)b.W, (void *)b.Z}) (void)hipFree(p);
+    if (b.ready) (void)hipEventDestroy(b.ready);
+  }
+  if (A->mark) (void)hipEventDestroy(A->mark);
+  if (A->side) (void)hipStreamDestroy(A->side);
+  for (void *p : {(void *)A->stepF, (void *)A->trash, (void *)A->Vp, (void *)A->Vb,
                   (void *)A->Ub, (void *)A->K1b, (void *)A->K2b,
                   (void *)A->P0b, (void *)A->partials, (void *)A->partials2, (void *)A->gb, (void *)A->argmax,
-                  (void *)A->flags, (void *)A->band, (void *)A->lossp})
+                  (void *)A->band, (void *)A->lossp})
     (void)hipFree(p);
   delete A;
   c->adj = nullptr;
@@ -715,8 +731,9 @@ static void stage_geometry(const gx_ctx *c, StageArgs<double> &a) {
 }
 
 // One stage-1 launch: out1 = aux + K(src), out2 = aux + K(src)/2.
-static int launch_stage1(gx_ctx *c, const double *src, bool prim, const double *aux,
-                         double *out1, double *out2, double dt, const uint8_t *flags) {
+static int launch_stage1(gx_ctx *c, hipStream_t s, const double *src, bool prim,
+                         const double *aux, double *out1, double *out2, double dt,
+                         const uint8_t *flags) {
   StageArgs<double> a = base_args<double>(c);
   a.dt = dt;
   a.src_is_prim = prim ? 1 : 0;
@@ -726,13 +743,13 @@ static int launch_stage1(gx_ctx *c, const double *src, bool prim, const double *
   a.out2 = out2;
   a.flags = flags;
   stage_geometry(c, a);
-  stage_kernel<double, 1><<<a.nblk, NT, 0, c->stream>>>(a);
+  stage_kernel<double, 1><<<a.nblk, NT, 0, s>>>(a);
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }
 
-static int launch_stage2(gx_ctx *c, const double *Y, const double *W, double *out,
-                         double *Z, double dt, const uint8_t *flags) {
+static int launch_stage2(gx_ctx *c, hipStream_t s, const double *Y, const double *W,
+                         double *out, double *Z, double dt, const uint8_t *flags) {
   StageArgs<double> a = base_args<double>(c);
   a.dt = dt;
   a.src = Y;
@@ -741,8 +758,8 @@ static int launch_stage2(gx_ctx *c, const double *Y, const double *W, double *ou
   a.out2 = Z;
   a.flags = flags;
   stage_geometry(c, a);
-  HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
-  stage_kernel<double, 2><<<a.nblk, NT, 0, c->stream>>>(a);
+  HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), s));
+  stage_kernel<double, 2><<<a.nblk, NT, 0, s>>>(a);
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }
@@ -790,6 +807,33 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, 
   return GX_OK;
 }
 
+// Recomputes segment si into its buffers on the side stream, after the work
+// the context's stream holds so far (the reverse pass that last used them).
+static int recompute_segment(gx_ctx *c, int si) {
+  AdjointState *A = c->adj;
+  SegmentBuffers &b = A->sb[si & 1];
+  const int every = A->every, seg0 = si * every, len = std::min(every, A->nsteps - seg0);
+  const long long n5 = 5 * c->ncell;
+  const double *P0 = (const double *)c->P0;
+  HIP_TRY(hipEventRecord(A->mark, c->stream));
+  HIP_TRY(hipStreamWaitEvent(A->side, A->mark, 0));
+  const double *ck = A->ckpt[si];
+  for (int j = 0; j < len; j++) {
+    const int n = seg0 + j;
+    const double *Un = j == 0 ? ck : b.U + (size_t)j * n5;
+    double *Un1 = j + 1 < len ? b.U + (size_t)(j + 1) * n5 : A->trash;
+    const uint8_t *Fn = A->stepF + (size_t)n * c->ncell;
+    double *Yn = b.Y + (size_t)j * n5, *Wn = b.W + (size_t)j * n5;
+    int rc = launch_stage1(c, A->side, n == 0 ? P0 : Un, n == 0, Un, Yn, Wn, A->dt[n], Fn);
+    if (rc) return rc;
+    rc = launch_stage2(c, A->side, Yn, Wn, Un1, b.Z + (size_t)j * n5, A->dt[n], Fn);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(b.ready, A->side));
+  b.held = seg0;
+  return GX_OK;
+}
+
 static int ensure_state(gx_ctx *c, int nsteps, int every) {
   if (c->fp32) return fail(GX_E_ARG, "gx_adjoint: only Precision = GX_FP64");
   release_adjoint(c);
@@ -813,7 +857,12 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   const int nck = (nsteps + every - 1) / every + 1;
   A->ckpt.assign(nck, nullptr);
   for (int i = 0; i < nck; i++) HIP_TRY(hipMalloc(&A->ckpt[i], s5));
-  for (double **p : {&A->seg, &A->segY, &A->segW, &A->segZ}) HIP_TRY(hipMalloc(p, s5 * every));
+  for (SegmentBuffers &b : A->sb) {
+    for (double **p : {&b.U, &b.Y, &b.W, &b.Z}) HIP_TRY(hipMalloc(p, s5 * every));
+    HIP_TRY(hipEventCreateWithFlags(&b.ready, hipEventDisableTiming));
+  }
+  HIP_TRY(hipEventCreateWithFlags(&A->mark, hipEventDisableTiming));
+  HIP_TRY(hipStreamCreate(&A->side));
   HIP_TRY(hipMalloc(&A->stepF, (size_t)c->ncell * nsteps));
   for (double **p : {&A->trash, &A->Ub, &A->K1b, &A->K2b, &A->P0b})
     HIP_TRY(hipMalloc(p, s5));
@@ -830,7 +879,6 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
     const int none = 0x7fffffff;  // the dt kernel restores it after each use
     HIP_TRY(hipMemcpy(A->argmax, &none, sizeof(int), hipMemcpyHostToDevice));
   }
-  HIP_TRY(hipMalloc(&A->flags, (size_t)c->ncell));
   HIP_TRY(hipMalloc(&A->band, sizeof(double) * c->n[1]));
   HIP_TRY(hipMalloc(&A->lossp, sizeof(double) * blocks_for(c->ncell)));
   return GX_OK;
@@ -881,19 +929,22 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
       rc = gx_step_rk2(c, time, dt);
     } else {
       const int j = n - last0;
+      SegmentBuffers &b = A->sb[(last0 / checkpointEvery) & 1];
       StepRecord rec;  // the first input is the segment's checkpoint
-      rec.Uin = j == 0 ? A->ckpt[last0 / checkpointEvery] : A->seg + (size_t)j * n5;
-      rec.Y = A->segY + (size_t)j * n5;
-      rec.W = A->segW + (size_t)j * n5;
-      rec.Z = A->segZ + (size_t)j * n5;
-      rec.Uout = n + 1 < nSteps ? A->seg + (size_t)(j + 1) * n5 : (double *)c->U;
+      rec.Uin = j == 0 ? A->ckpt[last0 / checkpointEvery] : b.U + (size_t)j * n5;
+      rec.Y = b.Y + (size_t)j * n5;
+      rec.W = b.W + (size_t)j * n5;
+      rec.Z = b.Z + (size_t)j * n5;
+      rec.Uout = n + 1 < nSteps ? b.U + (size_t)(j + 1) * n5 : (double *)c->U;
       rc = step_rk2(c, dt, &rec);
     }
     if (rc) return rc;
     time = time + dt;
   }
+  HIP_TRY(hipEventRecord(A->sb[(last0 / checkpointEvery) & 1].ready, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  A->seg_held = last0;
+  A->sb[(last0 / checkpointEvery) & 1].held = last0;
+  A->sb[((last0 / checkpointEvery) & 1) ^ 1].held = -1;
   A->forward_done = true;
   return GX_OK;
 }
@@ -919,35 +970,33 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   const double *P0 = (const double *)c->P0;
   const int every = A->every, N = A->nsteps;
   bool k2_ready = false;  // K2b of the next step to reverse is formed already
-  for (int seg0 = ((N - 1) / every) * every; seg0 >= 0; seg0 -= every) {
-    const int len = std::min(every, N - seg0);
-    // Recompute the segment from its checkpoint with the recorded flags,
-    // keeping per step the input U^n (the checkpoint itself for the first),
-    // Y, W and the state before the fix Z (unless the buffers hold it).
-    const double *ck = A->ckpt[seg0 / every];
-    for (int j = 0; j < len && A->seg_held != seg0; j++) {
-      const int n = seg0 + j;
-      const double *Un = j == 0 ? ck : A->seg + (size_t)j * n5;
-      double *Un1 = j + 1 < len ? A->seg + (size_t)(j + 1) * n5 : A->trash;
-      const uint8_t *Fn = A->stepF + (size_t)n * c->ncell;
-      double *Yn = A->segY + (size_t)j * n5, *Wn = A->segW + (size_t)j * n5;
-      rc = launch_stage1(c, n == 0 ? P0 : Un, n == 0, Un, Yn, Wn, A->dt[n], Fn);
-      if (rc) return rc;
-      rc = launch_stage2(c, Yn, Wn, Un1, A->segZ + (size_t)j * n5, A->dt[n], Fn);
+  for (int si = (N - 1) / every; si >= 0; si--) {
+    const int seg0 = si * every, len = std::min(every, N - seg0);
+    // Segment si's states (recomputed from its checkpoint with the recorded
+    // flags, unless its buffers hold them), then the recomputation of
+    // segment si - 1 on the side stream, beside this segment's reverse pass.
+    SegmentBuffers &b = A->sb[si & 1];
+    if (b.held != seg0) {
+      rc = recompute_segment(c, si);
       if (rc) return rc;
     }
-    A->seg_held = seg0;
+    HIP_TRY(hipStreamWaitEvent(c->stream, b.ready, 0));
+    if (si >= 1 && A->sb[(si - 1) & 1].held != seg0 - every) {
+      rc = recompute_segment(c, si - 1);
+      if (rc) return rc;
+    }
+    const double *ck = A->ckpt[si];
     for (int j = len - 1; j >= 0; j--) {
       const int n = seg0 + j;
-      const double *Un = j == 0 ? ck : A->seg + (size_t)j * n5;
-      const double *Yn = A->segY + (size_t)j * n5;
+      const double *Un = j == 0 ? ck : b.U + (size_t)j * n5;
+      const double *Yn = b.Y + (size_t)j * n5;
       const uint8_t *Fn = A->stepF + (size_t)n * c->ncell;
       const bool first = n == 0;
       const double dt = A->dt[n];
       if (!k2_ready) {
         adj_fix_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
             c->ncell, c->params.BaryonMassReference, c->params.BaryonDensityMin,
-            A->segZ + (size_t)j * n5, A->Ub, A->K2b);
+            b.Z + (size_t)j * n5, A->Ub, A->K2b);
         HIP_TRY(hipGetLastError());
       }
       // stage 2's transpose, ending with Ub = Yb + Wb and K1b = Yb + Wb/2
@@ -959,7 +1008,7 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       // and, when the previous step's Z is in the segment, forms its K2b
       unsigned long long bits;
       memcpy(&bits, &A->invdt[n], sizeof(bits));
-      const double *Zp = j > 0 ? A->segZ + (size_t)(j - 1) * n5 : nullptr;
+      const double *Zp = j > 0 ? b.Z + (size_t)(j - 1) * n5 : nullptr;
       rc = stage_vjp(c, first ? P0 : Un, first, A->K1b, dt, Fn, out, A->partials, nullptr,
                      nullptr, bits, A->argmax, Zp, Zp ? A->K2b : nullptr);
       if (rc) return rc;
