@@ -103,7 +103,8 @@ def parabolic_check(n: int, steps: int, z_chunk: int):
     dts, runs = [], {}
     for recon in ("LINEAR", "PARABOLIC"):
         st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, ReconstructionType=recon))
-        st.set_tuning(z_chunk)
+        if z_chunk > 0:
+            st.set_tuning(z_chunk)
         st.set_primitive(*prim)
         st.set_profiling(True)
         st.sync()
@@ -145,7 +146,8 @@ def precision_check(n: int, steps: int, z_chunk: int):
     dts, runs, states = [], {}, {}
     for prec in ("FP64", "FP32"):
         st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, Precision=prec))
-        st.set_tuning(z_chunk)
+        if z_chunk > 0:
+            st.set_tuning(z_chunk)
         st.set_primitive(*prim)
         st.set_profiling(True)
         st.sync()
@@ -237,7 +239,8 @@ def batch_check(n_problems: int, steps: int, seed: int, z_chunk: int, sequential
         elapsed = 0.0
         for chart, prim, _ in made:
             st = Step_RK2_C_ASC(chart, params)
-            st.set_tuning(z_chunk)
+            if z_chunk > 0:
+                st.set_tuning(z_chunk)
             st.set_primitive(*prim)
             it = Integrator_C_PS(st, IntegratorParameters(FinishTime=1.0, nWrite=1))
             it._administer_checkpoint()
@@ -368,7 +371,8 @@ def main():
     ap.add_argument("--steps", type=int, default=98)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--z-chunk", type=int, default=16)
+    ap.add_argument("--z-chunk", type=int, default=0,
+                    help="stage z-chunk; 0 keeps the library's default for the chart")
     ap.add_argument("--cpu-baseline", choices=("auto", "off"), default="auto")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -407,7 +411,9 @@ def main():
     params = FluidParameters(CourantFactor=0.8, Precision=args.precision)
     esz = 8 if args.precision == "FP64" else 4
     stepper = Step_RK2_C_ASC(chart, params)
-    stepper.set_tuning(args.z_chunk)
+    if args.z_chunk > 0:
+        stepper.set_tuning(args.z_chunk)
+    z_chunk = stepper.get_tuning()
     stepper.set_primitive(*problems.kelvin_helmholtz(chart, seed=1234, padded=False))
     total0 = stepper.totals()
     tally0 = stepper.compute_tally(False)
@@ -482,7 +488,7 @@ def main():
         "data": "synthetic (seeded Kelvin-Helmholtz initial data)",
         "config": {"workload": f"Kelvin-Helmholtz {n}^3 periodic, PLM(MC theta=1.4)"
                                f"+HLLC, RK2, CFL 0.8, {args.precision}",
-                   "cells": chart.n_proper, "z_chunk": args.z_chunk},
+                   "cells": chart.n_proper, "z_chunk": z_chunk},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},

@@ -26,7 +26,7 @@ struct gx_ctx {
   int n[3];
   long long ncell;
   hipStream_t stream = nullptr;
-  int tz = 16;
+  int tz = 16;                // z-chunk of the stage kernels (stage_tz at creation)
   bool fp32 = false;          // state and sweeps in float (Precision = GX_FP32)
   size_t esz = sizeof(double);  // bytes per state value
   // device buffers, of the state's real type

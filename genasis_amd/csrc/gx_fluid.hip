@@ -182,6 +182,7 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
     c->n[d] = grid->nCells[d];
     c->ncell *= c->n[d];
   }
+  c->tz = stage_tz(((c->n[0] + TX - 1) / TX) * ((c->n[1] + TY - 1) / TY), c->n[2]);
 #define TRY_CLEAN(expr)                                                   \
   do {                                                                    \
     hipError_t e_ = (expr);                                               \
@@ -242,6 +243,12 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
 int gx_set_tuning(gx_ctx *c, int zChunk) {
   if (!c || zChunk < 1) return fail(GX_E_ARG, "gx_set_tuning: bad argument");
   c->tz = zChunk;
+  return GX_OK;
+}
+
+int gx_get_tuning(gx_ctx *c, int *zChunk) {
+  if (!c || !zChunk) return fail(GX_E_ARG, "gx_get_tuning: null argument");
+  *zChunk = c->tz;
   return GX_OK;
 }
 

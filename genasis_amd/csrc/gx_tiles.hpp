@@ -20,8 +20,8 @@
 
 namespace gx {
 
-constexpr int TX = 32;           // tile width in x (cells)
-constexpr int TY = 8;            // tile height in y
+constexpr int TX = 16;           // tile width in x (cells)
+constexpr int TY = 16;           // tile height in y
 constexpr int NT = TX * TY;      // threads per block
 constexpr int HX = TX + 4;       // plane tile with two ghost layers
 constexpr int HY = TY + 4;
@@ -714,6 +714,15 @@ __global__ __launch_bounds__(256) void cfl_kernel(StageArgs<R> a) {
 constexpr int QW = TX + 3, QH = TY + 3;  // primitives of x0-2 .. x0+TX, y0-2 .. y0+TY
 constexpr int BW = TX + 2, BH = TY + 2;  // tests of x0-1 .. x0+TX, y0-1 .. y0+TY
 constexpr int FLAG_TZ = 32;              // planes per block
+
+// Default planes per stage block (z-chunk): 32, halved (down to 4) while a
+// launch would hold fewer than 512 blocks -- two per CU of 256, what the FP64
+// stage kernel keeps resident -- so small charts still fill the chip.
+inline int stage_tz(int ntile_xy, int nz) {
+  int tz = 32;
+  while (tz > 4 && (long long)ntile_xy * ((nz + tz - 1) / tz) < 512) tz /= 2;
+  return tz;
+}
 
 // Planes per single-chart shock-flag block: FLAG_TZ, halved (down to 4) while
 // the launch would hold fewer than 1536 blocks -- six per CU, what its LDS

@@ -216,6 +216,12 @@ class Step_RK2_C_ASC:
     def set_tuning(self, z_chunk: int):
         _capi.check(self.lib.gx_set_tuning(self._h, int(z_chunk)), "gx_set_tuning")
 
+    def get_tuning(self) -> int:
+        """The stage kernels' z-chunk (planes per block)."""
+        v = ctypes.c_int()
+        _capi.check(self.lib.gx_get_tuning(self._h, ctypes.byref(v)), "gx_get_tuning")
+        return v.value
+
     def set_profiling(self, on: bool):
         _capi.check(self.lib.gx_set_profiling(self._h, int(bool(on))), "gx_set_profiling")
 

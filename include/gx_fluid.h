@@ -251,8 +251,11 @@ int gx_get_flags(gx_ctx *ctx, unsigned char *flags);
 int gx_set_flags(gx_ctx *ctx, const unsigned char *flags);
 
 /* Launch-shape control for measurement: z-chunk length of the fused stage
- * kernel (planes per thread block, default 16). */
+ * kernel (planes per thread block).  The default, set by gx_create, is 32
+ * planes, halved (down to 4) while a launch would hold fewer than 512 blocks
+ * of 16 x 16 cells; gx_get_tuning reads the current value. */
 int gx_set_tuning(gx_ctx *ctx, int zChunk);
+int gx_get_tuning(gx_ctx *ctx, int *zChunk);
 
 /* The HIP stream of the context (a hipStream_t), for event timing. */
 void *gx_stream(gx_ctx *ctx);
