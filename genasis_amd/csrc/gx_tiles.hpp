@@ -750,27 +750,44 @@ __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, 
   const R mass = R(a.ph.mass), nmin = R(a.ph.nmin);
   const R sqrt_tiny = Arith<R>::sqrt_tiny;
 
+  // Each thread converts the plane-tile cells t = tid + j NT (j < QJ); their
+  // raw values for plane k + 1 are loaded while plane k's tests run.
+  constexpr int QJ = (QH * QW + NT - 1) / NT;
+  R Un[QJ][5];
+  auto load_plane = [&](int k) {
+#pragma unroll
+    for (int j = 0; j < QJ; j++) {
+      const int t = tid + j * NT;
+      const int x = x0 - 2 + t % QW, y = y0 - 2 + t / QW;
+      const bool in = t < QH * QW && k >= 0 && k < nz && (unsigned)x < (unsigned)nx &&
+                      (unsigned)y < (unsigned)ny;
+      const unsigned i = in ? lin(a.n, x, y, k) : 0u;
+#pragma unroll
+      for (int f = 0; f < 5; f++) Un[j][f] = in ? ld(a.src + f * a.fs, i) : R(0);
+    }
+  };
+  load_plane(z0 - 2);
   for (int k = z0 - 2; k <= zend; ++k) {
     R(*Q)[QH * QW] = sQ[(k + 2) & 1];
     R(*Qm)[QH * QW] = sQ[(k + 3) & 1];
     const bool plane = k >= 0 && k < nz;
     if (plane) {
-      for (int t = tid; t < QH * QW; t += NT) {
-        const int x = x0 - 2 + t % QW, y = y0 - 2 + t / QW;
-        R V[5] = {R(0), R(0), R(0), R(0), R(0)};
-        if ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny) {
-          const unsigned i = lin(a.n, x, y, k);
-          R U[5];
 #pragma unroll
-          for (int f = 0; f < 5; f++) U[f] = ld(a.src + f * a.fs, i);
-          prim_from_cons(U, mass, nmin, V);
+      for (int j = 0; j < QJ; j++) {
+        const int t = tid + j * NT;
+        if (t < QH * QW) {
+          const int x = x0 - 2 + t % QW, y = y0 - 2 + t / QW;
+          R V[5] = {R(0), R(0), R(0), R(0), R(0)};
+          if ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny)
+            prim_from_cons(Un[j], mass, nmin, V);
+          Q[0][t] = gm1 * V[4];
+          Q[1][t] = V[1];
+          Q[2][t] = V[2];
+          Q[3][t] = V[3];
         }
-        Q[0][t] = gm1 * V[4];
-        Q[1][t] = V[1];
-        Q[2][t] = V[2];
-        Q[3][t] = V[3];
       }
     }
+    if (k < zend) load_plane(k + 1);
     __syncthreads();
     if (plane && k >= z0 - 1) {
       uint8_t *B = sB[(k + 3) % 3];
