@@ -898,9 +898,10 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
   AdjointState *A = c->adj;
   const size_t s5 = 5 * sizeof(double) * (size_t)c->ncell;
   const long long n5 = 5 * c->ncell;
-  // the last segment, whose states the reverse pass needs first, is kept
-  // in the segment buffers as it is run, not recomputed
-  const int last0 = ((nSteps - 1) / checkpointEvery) * checkpointEvery;
+  // the last two segments, whose states the reverse pass needs first, are
+  // kept in the two sets of segment buffers as they are run, not recomputed
+  const int nseg = (nSteps + checkpointEvery - 1) / checkpointEvery;
+  const int keep0 = std::max(0, nseg - 2) * checkpointEvery;
   double time = 0.0;
   for (int n = 0; n < nSteps; n++) {
     if (n % checkpointEvery == 0)
@@ -925,26 +926,31 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
     A->ts.push_back(ts);
     A->invdt.push_back(invdt);
     A->ramp.push_back(ramp);
-    if (n < last0) {
+    if (n < keep0) {
       rc = gx_step_rk2(c, time, dt);
     } else {
-      const int j = n - last0;
-      SegmentBuffers &b = A->sb[(last0 / checkpointEvery) & 1];
+      const int si = n / checkpointEvery, j = n - si * checkpointEvery;
+      SegmentBuffers &b = A->sb[si & 1];
       StepRecord rec;  // the first input is the segment's checkpoint
-      rec.Uin = j == 0 ? A->ckpt[last0 / checkpointEvery] : b.U + (size_t)j * n5;
+      rec.Uin = j == 0 ? A->ckpt[si] : b.U + (size_t)j * n5;
       rec.Y = b.Y + (size_t)j * n5;
       rec.W = b.W + (size_t)j * n5;
       rec.Z = b.Z + (size_t)j * n5;
-      rec.Uout = n + 1 < nSteps ? b.U + (size_t)(j + 1) * n5 : (double *)c->U;
+      // the segment's last step writes the next checkpoint's state (or the
+      // final state) in the context
+      rec.Uout = j + 1 < checkpointEvery && n + 1 < nSteps ? b.U + (size_t)(j + 1) * n5
+                                                            : (double *)c->U;
       rc = step_rk2(c, dt, &rec);
     }
     if (rc) return rc;
     time = time + dt;
   }
-  HIP_TRY(hipEventRecord(A->sb[(last0 / checkpointEvery) & 1].ready, c->stream));
+  for (SegmentBuffers &b : A->sb) {
+    HIP_TRY(hipEventRecord(b.ready, c->stream));
+    b.held = -1;
+  }
+  for (int si = keep0 / checkpointEvery; si < nseg; si++) A->sb[si & 1].held = si * checkpointEvery;
   HIP_TRY(hipStreamSynchronize(c->stream));
-  A->sb[(last0 / checkpointEvery) & 1].held = last0;
-  A->sb[((last0 / checkpointEvery) & 1) ^ 1].held = -1;
   A->forward_done = true;
   return GX_OK;
 }
