@@ -152,3 +152,18 @@ def test_device_resident_seed_and_result():
     with pytest.raises(GxError):
         adj.backward(None)
     st.close()
+
+
+@pytest.mark.parametrize("every", [1, 4])
+def test_segment_lengths(every):
+    """One step per segment (every reverse step starts with the fix kernel,
+    buffer sets alternate per step) and one segment holding every step (the
+    forward run keeps it; nothing is recomputed), against autograd."""
+    chart = Chart((10, 12, 8))
+    prim = problems.kelvin_helmholtz(chart, seed=21, padded=False)
+    n_steps = 4
+    loss, dN, dg = kinetic_energy_gradient(chart, prim, n_steps, every)
+    J, gP, gg, _ = _torch_grad(chart, prim, n_steps)
+    assert abs(loss - J) <= 1e-14 * abs(J)
+    np.testing.assert_allclose(dN, gP[0], rtol=1e-9, atol=1e-12 * np.abs(gP[0]).max())
+    assert abs(dg - gg) <= 1e-9 * abs(gg), (dg, gg)
