@@ -1,11 +1,13 @@
 // gx_adjoint.hip -- discrete adjoint of the FP64 step of one chart
 // (gx_adjoint_* of include/gx_fluid.h).
 //
-// The forward run records the time steps and the state every `every` steps.
-// The reverse pass walks the steps backwards; for each segment it recomputes
-// the states from the checkpoint, and for each step it recomputes the two
-// stage inputs (Y and W = U + K1/2) and Z = W + K2/2 with the forward stage
-// kernel, then applies the transposed step:
+// The forward run records the time steps, the shock flags of every step and
+// the state every `every` steps (a segment), and keeps the last two
+// segments' per-step states.  The reverse pass walks the segments backwards,
+// each segment's states (the step inputs U, the stage values Y and
+// W = U + K1/2, and Z = W + K2/2) recomputed from its checkpoint with the
+// forward stage kernels on a second stream while the reverse pass of the
+// segment after it runs, and applies per step the transposed step:
 //
 //   U' = fix(Z), Z = W + K2/2, K2 = dt R(Y), Y = U + K1, W = U + K1/2,
 //   K1 = dt R(U or the uploaded primitives), dt = dt(CFL maximum of U),
@@ -13,11 +15,11 @@
 // where R(S) = -div F / Vol of the stage input S.  The transpose of R is a
 // primitives pass onto the padded layout (n + 4 per dimension, ghosts by the
 // ghost rule), one line pass per dimension (slopes, the face flux's reverse
-// sweep and the limiter's reverse along lines, neighbours by lane shuffles),
-// a fold of the ghost bars onto the cells they copy, and the conserved ->
-// primitive reverse.  Shock flags are recomputed from each step's input and
-// replayed as constants; dt and gamma bars are reduced per block, then in
-// two levels, in a fixed order.
+// sweep and the limiter's reverse along lines, neighbours by DPP lane
+// shifts), a fold of the ghost bars onto the cells they copy, and the
+// conserved -> primitive reverse.  The recorded shock flags are replayed as
+// constants; dt and gamma bars are reduced per block, then in two levels, in
+// a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
