@@ -45,7 +45,8 @@ EXPORTS = (
     "gx_upload_features", "gx_download_features", "gx_compute_time_step",
     "gx_step_rk2", "gx_sync", "gx_totals", "gx_set_state", "gx_set_primitive",
     "gx_get_conserved", "gx_get_primitive", "gx_get_flags", "gx_set_flags",
-    "gx_set_tuning", "gx_get_tuning", "gx_stream", "gx_set_profiling", "gx_kernel_times",
+    "gx_set_tuning", "gx_get_tuning", "gx_set_record_sources", "gx_get_sources",
+    "gx_stream", "gx_set_profiling", "gx_kernel_times",
     "gx_self_test_division", "gx_last_error",
     "gx_get_boundary_fluence", "gx_compute_tally",
     "gx_write_checkpoint", "gx_read_checkpoint",
@@ -126,6 +127,8 @@ def load(path: str = LIB_PATH):
         "gx_set_flags": (i, [vp, P(ctypes.c_ubyte)]),
         "gx_set_tuning": (i, [vp, i]),
         "gx_get_tuning": (i, [vp, P(ctypes.c_int)]),
+        "gx_set_record_sources": (i, [vp, i]),
+        "gx_get_sources": (i, [vp, pd]),
         "gx_stream": (vp, [vp]),
         "gx_set_profiling": (i, [vp, i]),
         "gx_kernel_times": (i, [vp, pd, P(ctypes.c_longlong)]),

@@ -94,6 +94,8 @@ __host__ __device__ inline StageArgs<R> problem_args(const BatchArgs<R> &B,
   a.geo = B.geo + P.geo_off;
   a.flags = B.flags + P.off;
   a.invdt_bits = B.invdt + p;
+  a.sources = nullptr;  // batches record no divergence sources
+  a.rdt = 0.0;
   return a;
 }
 

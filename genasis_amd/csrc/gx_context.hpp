@@ -36,6 +36,8 @@ struct gx_ctx {
   void *U = nullptr, *Y = nullptr, *W = nullptr, *P0 = nullptr;
   uint8_t *flags = nullptr;
   unsigned long long *invdt = nullptr;
+  void *sources = nullptr;        // Current % Sources of the divergence, 5 fields
+  bool record_sources = false;    // gx_set_record_sources
   double *partial = nullptr;
   int n_partial = 0;
   double *scratch = nullptr;  // download staging: GX_N_FIELDS x ncell

@@ -84,10 +84,17 @@ thread_local std::string gx::g_last_error;
 // ReconstructionType.
 template <typename R, int STAGE>
 static void launch_stage(const gx_ctx *c, int nblk, hipStream_t st, const StageArgs<R> &a) {
-  if (c->params.ReconstructionType == GX_PARABOLIC)
+  const bool para = c->params.ReconstructionType == GX_PARABOLIC;
+  if (a.sources) {
+    if (para)
+      stage_kernel_parabolic<R, STAGE, true><<<nblk, NT, 0, st>>>(a);
+    else
+      stage_kernel<R, STAGE, true><<<nblk, NT, 0, st>>>(a);
+  } else if (para) {
     stage_kernel_parabolic<R, STAGE><<<nblk, NT, 0, st>>>(a);
-  else
+  } else {
     stage_kernel<R, STAGE><<<nblk, NT, 0, st>>>(a);
+  }
 }
 
 template <typename R, int STAGE>
@@ -151,6 +158,7 @@ int gx_destroy(gx_ctx *c) {
   (void)hipFree(c->W);
   (void)hipFree(c->P0);
   (void)hipFree(c->flags);
+  (void)hipFree(c->sources);
   (void)hipFree(c->invdt);
   (void)hipFree(c->partial);
   (void)hipFree(c->scratch);
@@ -244,6 +252,25 @@ int gx_set_tuning(gx_ctx *c, int zChunk) {
   if (!c || zChunk < 1) return fail(GX_E_ARG, "gx_set_tuning: bad argument");
   c->tz = zChunk;
   return GX_OK;
+}
+
+int gx_set_record_sources(gx_ctx *c, int on) {
+  if (!c) return fail(GX_E_ARG, "gx_set_record_sources: null context");
+  if (on) {
+    const size_t bytes = 5 * c->esz * (size_t)c->ncell;
+    if (!c->sources) HIP_TRY(hipMalloc(&c->sources, bytes));
+    HIP_TRY(hipMemsetAsync(c->sources, 0, bytes, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  c->record_sources = on != 0;
+  return GX_OK;
+}
+
+int gx_get_sources(gx_ctx *c, double *sources) {
+  if (!c || !sources) return fail(GX_E_ARG, "gx_get_sources: null argument");
+  if (!c->sources)
+    return fail(GX_E_STATE, "gx_get_sources: recording was never enabled (gx_set_record_sources)");
+  return get_values(c->stream, c->fp32, sources, c->sources, 5 * (size_t)c->ncell);
 }
 
 int gx_get_tuning(gx_ctx *c, int *zChunk) {
@@ -556,6 +583,11 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
     a1.nbx = nbx;
     a1.nby = nby;
     a1.nblk = nblk;
+    if (c->record_sources) {
+      // Weight_RK * Increment / TimeStep with RN(1/dt) (div_rcp's range)
+      a1.sources = (R *)c->sources;
+      a1.rdt = (dt >= 0x1p-500 && dt <= 0x1p122) ? 1.0 / dt : 0.0;
+    }
     launch_stage<R, 1>(c, nblk, st, a1);
     HIP_TRY(hipGetLastError());
     if (prof) HIP_TRY(hipEventRecord(c->ev[1], st));
@@ -575,6 +607,8 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
     a2.nbx = nbx;
     a2.nby = nby;
     a2.nblk = nblk;
+    a2.sources = a1.sources;
+    a2.rdt = a1.rdt;
     launch_stage<R, 2>(c, nblk, st, a2);
     HIP_TRY(hipGetLastError());
     if (prof) HIP_TRY(hipEventRecord(c->ev[3], st));

@@ -55,6 +55,8 @@ struct StageArgs {
   const R *geo;
   const uint8_t *flags; // DiffusiveFlux bits of the step
   unsigned long long *invdt_bits;  // stage 2: max CFL term (as FP64 bits)
+  R *sources;           // Current % Sources of the divergence, if recorded
+  double rdt;           // RN(1/dt) for the recording, 0: divide
 };
 
 // Waves per SIMD the stage kernel is compiled for: the FP32 tile uses half
@@ -299,7 +301,8 @@ template <typename R> struct EdgeShared {
   R sEy[5][TX];
 };
 
-template <typename R, int STAGE, int RECON = RECON_LINEAR>
+// REC: record the divergence sources (a.sources).
+template <typename R, int STAGE, int RECON = RECON_LINEAR, bool REC = false>
 __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0, int z0,
                                            StageShared<R> &L, EdgeShared<R> *E = nullptr) {
   constexpr bool PARA = RECON == RECON_PARABOLIC;
@@ -583,6 +586,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         K = K + Arith<R>::rdiv(dt * (A3 * Flo[f] - A3 * Fhi[f]), Vol, rVol);
         Kc[f] = K;
       }
+      if constexpr (REC) {
+        // RecordDivergenceKernel (EV/Steps/Step_RK_C_ASC__Kernel.f90:12-46):
+        // Sources = Sources + Weight_RK * Increment / TimeStep, Weight_RK =
+        // 1/2, Sources cleared in stage 1 (Step_RK_C_ASC__Template.f90:
+        // 1572-1589)
+        const R hdt = R(a.dt), rdt = R(a.rdt);
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+          R *sp = a.sources + f * a.fs;
+          const R q = rdt != R(0) ? Arith<R>::rdiv(R(0.5) * Kc[f], hdt, rdt)
+                                  : R(0.5) * Kc[f] / hdt;
+          st(sp, ic, (STAGE == 1 ? R(0) : ld(sp, ic)) + q);
+        }
+      }
       if (STAGE == 1) {
         // Y = Solution + 1*K1; Solution + 1/2 K1 kept for the final combination
         // (Step_RK__Template.f90:240-278, MultiplyAdd_Command.f90:56-94).
@@ -642,24 +659,26 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
   }
 }
 
-template <typename R, int STAGE>
+template <typename R, int STAGE, bool REC = false>
 __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel(
     StageArgs<R> a) {
   __shared__ StageShared<R> L;
   const int tile = tile_of_block(blockIdx.x, a.nblk);
-  stage_tile<R, STAGE>(a, (tile % a.nbx) * TX, ((tile / a.nbx) % a.nby) * TY,
-                       (tile / (a.nbx * a.nby)) * a.tz, L);
+  stage_tile<R, STAGE, RECON_LINEAR, REC>(a, (tile % a.nbx) * TX,
+                                          ((tile / a.nbx) % a.nby) * TY,
+                                          (tile / (a.nbx * a.nby)) * a.tz, L);
 }
 
 // The same stage with PARABOLIC reconstruction.
-template <typename R, int STAGE>
+template <typename R, int STAGE, bool REC = false>
 __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel_parabolic(
     StageArgs<R> a) {
   __shared__ StageShared<R> L;
   __shared__ EdgeShared<R> E;
   const int tile = tile_of_block(blockIdx.x, a.nblk);
-  stage_tile<R, STAGE, RECON_PARABOLIC>(a, (tile % a.nbx) * TX, ((tile / a.nbx) % a.nby) * TY,
-                                        (tile / (a.nbx * a.nby)) * a.tz, L, &E);
+  stage_tile<R, STAGE, RECON_PARABOLIC, REC>(a, (tile % a.nbx) * TX,
+                                             ((tile / a.nbx) % a.nby) * TY,
+                                             (tile / (a.nbx * a.nby)) * a.tz, L, &E);
 }
 
 // Max CFL term over the current state (the first step after an upload, or

@@ -216,6 +216,18 @@ class Step_RK2_C_ASC:
     def set_tuning(self, z_chunk: int):
         _capi.check(self.lib.gx_set_tuning(self._h, int(z_chunk)), "gx_set_tuning")
 
+    def record_sources(self, on: bool = True):
+        """Record Current % Sources of the divergence in every step
+        (RecordDivergence); turning it on clears them."""
+        _capi.check(self.lib.gx_set_record_sources(self._h, int(bool(on))),
+                    "gx_set_record_sources")
+
+    def sources(self) -> np.ndarray:
+        """Sources of the divergence of the last step: D, S_1..3, G."""
+        out = np.empty((5,) + self.shape)
+        _capi.check(self.lib.gx_get_sources(self._h, _pd(out)), "gx_get_sources")
+        return out
+
     def get_tuning(self) -> int:
         """The stage kernels' z-chunk (planes per block)."""
         v = ctypes.c_int()

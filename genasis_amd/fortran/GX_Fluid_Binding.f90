@@ -40,7 +40,8 @@ module GX_Fluid_Binding
     gx_default_params, gx_create, gx_destroy, gx_upload, gx_download, &
     gx_compute_time_step, gx_step_rk2, gx_sync, gx_totals, gx_last_error, &
     gx_compute_tally, gx_get_boundary_fluence, gx_write_checkpoint, &
-    gx_read_checkpoint, LastErrorString
+    gx_read_checkpoint, gx_set_record_sources, gx_get_sources, &
+    LastErrorString
 
   !-- gx_read_checkpoint modes: the reference's RestartFrom, or exact
   integer ( c_int ), parameter, public :: &
@@ -138,6 +139,22 @@ module GX_Fluid_Binding
       type ( c_ptr ), value :: ctx
       real ( c_double ), intent ( out ) :: Fluence ( * )
     end function gx_get_boundary_fluence
+
+    !-- Current % Sources (RecordDivergence), off until switched on; D,
+    !   S_1..3, G of the proper cells, x fastest
+    integer ( c_int ) function gx_set_record_sources ( ctx, On ) &
+                        bind ( C, name = 'gx_set_record_sources' )
+      import :: c_int, c_ptr
+      type ( c_ptr ), value :: ctx
+      integer ( c_int ), value :: On
+    end function gx_set_record_sources
+
+    integer ( c_int ) function gx_get_sources ( ctx, Sources ) &
+                        bind ( C, name = 'gx_get_sources' )
+      import :: c_int, c_ptr, c_double
+      type ( c_ptr ), value :: ctx
+      real ( c_double ), intent ( out ) :: Sources ( * )
+    end function gx_get_sources
 
     !-- Checkpoint file at AdministerCheckpoint; Path is a NUL-terminated
     !   character array (trim ( Path ) // c_null_char)

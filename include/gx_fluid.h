@@ -257,6 +257,17 @@ int gx_set_flags(gx_ctx *ctx, const unsigned char *flags);
 int gx_set_tuning(gx_ctx *ctx, int zChunk);
 int gx_get_tuning(gx_ctx *ctx, int *zChunk);
 
+/* Current % Sources of the divergence (ApplyDivergence_C's recording,
+ * Step_RK_C_ASC__Template.f90:1548-1595 with RecordDivergenceKernel,
+ * Step_RK_C_ASC__Kernel.f90:12-46): with recording on, every gx_step_rk2
+ * sets Sources = sum over its stages of Weight_RK * Increment / TimeStep for
+ * D, S_1..3 and G (the conserved entropy is not evolved).  Off by default
+ * (a diagnostic; it adds a read and a write of five fields per stage);
+ * turning it on clears Sources.  gx_get_sources copies the five proper-cell
+ * fields (FP64 values of the context's precision).  Both synchronise. */
+int gx_set_record_sources(gx_ctx *ctx, int on);
+int gx_get_sources(gx_ctx *ctx, double *sources);
+
 /* The HIP stream of the context (a hipStream_t), for event timing. */
 void *gx_stream(gx_ctx *ctx);
 

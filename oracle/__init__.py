@@ -54,6 +54,7 @@ def _load():
         lib.gxo_time_step_candidate.argtypes = [ctypes.c_void_p, _PD]
         lib.gxo_step.argtypes = [ctypes.c_void_p, _D]
         lib.gxo_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD]
+        lib.gxo_get_sources.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD]
         lib.gxo_get_flags.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD]
         lib.gxo_set_flags.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD]
         lib.gxo_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD, _PD, _PD]
@@ -152,6 +153,14 @@ class OracleFluid:
     def field(self, name) -> np.ndarray:
         out = np.empty(self.shape)
         self._lib.gxo_get_field(self._h, FIELD[name], _ptr(out))
+        return out
+
+    def sources(self) -> np.ndarray:
+        """Current % Sources of the divergence after the last step (padded,
+        D, S_1..3, G, DS)."""
+        out = np.empty((6,) + self.shape)
+        for f in range(6):
+            self._lib.gxo_get_sources(self._h, f, _ptr(out[f]))
         return out
 
     def flags(self, d) -> np.ndarray:

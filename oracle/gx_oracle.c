@@ -52,6 +52,7 @@ struct gxo_ctx {
   double *cur[GXO_N_FIELDS];
   double *df[3];
   double *sol[6], *y[6], *k1[6], *k2[6];
+  double *sources[6]; /* Current % Sources, the divergence part (RecordDivergence) */
   /* IncrementDivergence scratch (EV/Increments/StorageDivergence_Form.f90) */
   double *il[GXO_N_FIELDS], *ir[GXO_N_FIELDS];
   double *icl[GXO_N_FIELDS], *icr[GXO_N_FIELDS];
@@ -159,6 +160,7 @@ gxo_ctx *gxo_create(const int n[3], const double lo[3], const double hi[3],
     c->y[f] = alloc0(c->nv);
     c->k1[f] = alloc0(c->nv);
     c->k2[f] = alloc0(c->nv);
+    c->sources[f] = alloc0(c->nv);
     c->fil[f] = alloc0(c->nv);
     c->fir[f] = alloc0(c->nv);
     c->fi[f] = alloc0(c->nv);
@@ -195,7 +197,7 @@ void gxo_destroy(gxo_ctx *c) {
     free(c->icr[f]);
   }
   for (int f = 0; f < 6; f++) {
-    free(c->sol[f]); free(c->y[f]); free(c->k1[f]); free(c->k2[f]);
+    free(c->sol[f]); free(c->y[f]); free(c->k1[f]); free(c->k2[f]); free(c->sources[f]);
     free(c->fil[f]); free(c->fir[f]); free(c->fi[f]);
   }
   free(c->ap); free(c->am); free(c->ac); free(c->dxi); free(c->xc);
@@ -998,6 +1000,23 @@ double gxo_time_step_candidate(gxo_ctx *c, double *invdt_out) {
   return c->courant * ts;
 }
 
+/* ApplyDivergence_C's recording (EV/Steps/Step_RK_C_ASC__Template.f90:
+ * 1572-1589) with RecordDivergenceKernel (EV/Steps/Step_RK_C_ASC__Kernel.f90:
+ * 12-46): Sources is cleared in stage 1, then Sources = Sources + Weight_RK *
+ * Increment / TimeStep over all values, after the increment and before the
+ * ghost exchange of K. */
+static void record_divergence(gxo_ctx *c, double **K, double dt, int stage) {
+  for (int f = 0; f < 6; f++) {
+    double *s = c->sources[f];
+    if (stage == 1) memset(s, 0, sizeof(double) * c->nv);
+    for (long v = 0; v < c->nv; v++) s[v] = s[v] + c->weight_rk * K[f][v] / dt;
+  }
+}
+
+void gxo_get_sources(const gxo_ctx *c, int f, double *out) {
+  memcpy(out, c->sources[f], sizeof(double) * c->nv);
+}
+
 /* Step_RK_C_ASC % Compute -> Step_RK % ComputeTemplate (EV/Steps/Step_RK__
  * Template.f90:207-288; EV/Steps/Step_RK_C_ASC__Template.f90:504-542,
  * 692-816, 833-914, 1112-1281) with the Heun tableau of Step_RK2_C_ASC
@@ -1013,6 +1032,7 @@ void gxo_step(gxo_ctx *c, double dt) {
   clear6(c, c->k1);
   c->weight_rk = 0.5; /* B(1) */
   increment_divergence(c, c->k1, dt);
+  record_divergence(c, c->k1, dt, 1);
   exchange_ghosts(c, c->k1, 6);
   /* Stage 2: Y = Solution + 1*K1; Current <- Y; K2 = L(Current)*dt */
   for (int f = 0; f < 6; f++) memcpy(c->y[f], c->sol[f], sizeof(double) * c->nv);
@@ -1022,6 +1042,7 @@ void gxo_step(gxo_ctx *c, double dt) {
   clear6(c, c->k2);
   c->weight_rk = 0.5; /* B(2) */
   increment_divergence(c, c->k2, dt);
+  record_divergence(c, c->k2, dt, 2);
   exchange_ghosts(c, c->k2, 6);
   /* IncrementSolution, K1 first */
   multiply_add(c, c->sol, c->k1, 0.5);

@@ -96,6 +96,10 @@ void gxo_step(gxo_ctx *c, double dt);
 /* Copy Current field f (padded layout) out. */
 void gxo_get_field(const gxo_ctx *c, int f, double *out);
 
+/* Current % Sources of the divergence (RecordDivergence) after the last
+ * step, conserved field f (D, S_1..3, G, DS order), all nv values. */
+void gxo_get_sources(const gxo_ctx *c, int f, double *out);
+
 /* Copy DiffusiveFlux_I_d (d = 0,1,2) out, padded layout. */
 void gxo_get_flags(const gxo_ctx *c, int d, double *out);
 
