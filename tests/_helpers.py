@@ -21,6 +21,13 @@ class OracleStepper:
         self.o.step(dt)
 
 
+def same_bits(a, b) -> bool:
+    """FP64 arrays with identical bit patterns (so +0 and -0 differ)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
 def oracle_conserved(o, chart: Chart) -> np.ndarray:
     return np.stack([chart.proper(a) for a in o.conserved()])
 

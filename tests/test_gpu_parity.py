@@ -1,5 +1,5 @@
 """HIP step vs the oracle (CPU restatement of the reference step) on the same
-seeded inputs: FP64 results must be identical (np.array_equal), time-step
+seeded inputs: FP64 results must be identical bit for bit, time-step
 candidates bit-identical, shock flags identical."""
 import numpy as np
 import pytest
@@ -8,8 +8,9 @@ from genasis_amd.chart import Chart
 from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
 from genasis_amd import problems
 
-from ._helpers import (make_oracle, oracle_conserved, oracle_flag_bits,
-                       oracle_primitive)
+from . import exact_riemann
+from ._helpers import (OracleStepper, evolve, make_oracle, oracle_conserved,
+                       oracle_flag_bits, oracle_primitive, same_bits)
 
 pytestmark = pytest.mark.gpu
 
@@ -42,9 +43,10 @@ def _pair(chart, prim, courant=0.8, **kw):
 
 
 def _compare(o, g, chart, tag):
+    """Identical bit patterns (signed zeros included; any NaN matches NaN)."""
     uo = oracle_conserved(o, chart)
     ug = g.conserved()
-    bad = ~((uo == ug) | (np.isnan(uo) & np.isnan(ug)))
+    bad = ~((uo.view(np.uint64) == ug.view(np.uint64)) | (np.isnan(uo) & np.isnan(ug)))
     if bad.any():
         idx = np.argwhere(bad)[:5]
         rel = np.max(np.abs(uo - ug) / np.maximum(np.abs(uo), 1e-300))
@@ -104,6 +106,39 @@ def test_hll_and_unlimited_bitwise():
     _run_lockstep(o, g, chart, 3, "unlimited")
 
 
+def test_sod_workload_to_finish_time_bitwise():
+    """The Sod shock tube at 256 x 16 x 16 (outflow, CFL 0.8) through the
+    integrator to FinishTime 0.2 -- 100 ramp cycles, the last step clipped
+    onto FinishTime: the same cycles and time steps, the final state and
+    shock flags bit for bit, and the density within the exact solution's
+    L1 bound."""
+    chart = Chart((256, 16, 16), bc=OUT)
+    o, g = _pair(chart, problems.sod(chart))
+    it_o = evolve(OracleStepper(o), 10**9, finish_time=0.2)
+    it_g = evolve(g, 10**9, finish_time=0.2)
+    assert it_g.s.Time == 0.2 and it_g.s.iCycle > 100
+    assert it_o.s.history == it_g.s.history
+    _compare(o, g, chart, "sod 256x16x16 at t = 0.2")
+    rho = g.primitive()[0]
+    assert np.all(rho == rho[:1, :1, :])  # every (y, z) column the same
+    exact = exact_riemann.sample(chart.center[0][2:-2], 0.2, 0.5, (1.0, 0.0, 1.0),
+                                 (0.125, 0.0, 0.1))[0]
+    assert np.mean(np.abs(rho[0, 0] - exact)) < 3.0e-3
+
+
+def test_kelvin_helmholtz_hundred_cycles_bitwise():
+    """100 cycles of the periodic Kelvin-Helmholtz problem (64 x 64 x 32,
+    several z-chunks per column) through the integrator: time steps and the
+    final state bit for bit."""
+    chart = Chart((64, 64, 32), bc=PER)
+    o, g = _pair(chart, problems.kelvin_helmholtz(chart, seed=13))
+    it_o = evolve(OracleStepper(o), 100, finish_time=10.0)
+    it_g = evolve(g, 100, finish_time=10.0)
+    assert it_g.s.iCycle == 100
+    assert it_o.s.history == it_g.s.history
+    _compare(o, g, chart, "kh 64x64x32 after 100 cycles")
+
+
 def test_primitive_and_totals():
     chart = Chart((16, 16, 16), bc=PER)
     o, g = _pair(chart, problems.kelvin_helmholtz(chart, seed=11))
@@ -111,7 +146,7 @@ def test_primitive_and_totals():
         c = o.time_step_candidate()[0]
         o.step(0.01 * c)
         g.step(0.0, 0.01 * c)
-    assert np.array_equal(oracle_primitive(o, chart), g.primitive())
+    assert same_bits(oracle_primitive(o, chart), g.primitive())
     u = oracle_conserved(o, chart)
     vol = np.multiply.outer(np.multiply.outer(
         chart.width_left[2][2:-2] + chart.width_right[2][2:-2],
