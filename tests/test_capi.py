@@ -61,3 +61,5 @@ def test_default_params_and_argument_errors():
         rc = lib.gx_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
         assert rc == -1 and b"spacings" in lib.gx_last_error(), big
     assert lib.gx_step_rk2(None, 0.0, 1.0) == -1
+    assert lib.gx_set_record_sources(None, 1) == -1
+    assert lib.gx_get_sources(None, None) == -1 and b"gx_get_sources" in lib.gx_last_error()
