@@ -132,6 +132,24 @@ __device__ __forceinline__ R slope(R vm, R v0, R vp, R dxl, R dxr, R dxl2, R dxr
               fabs(central));
 }
 
+// The same slope with the left quotient |theta dVL / dxl| given (*q) and the
+// right one returned in *q.  The right quotient of a cell is the left one of
+// its upper neighbour bit for bit (dV and the centre spacing are the same
+// differences of the same values), so a march along one dimension computes
+// each once.
+template <typename R>
+__device__ __forceinline__ R slope_carry(R vm, R v0, R vp, R dxr, R dxl2, R dxr2, R den,
+                                         R rdxr, R rden, R theta, int use_limiter, R *q) {
+  const R dVL = v0 - vm;
+  const R dVR = vp - v0;
+  const R central = Arith<R>::rdiv(dxr2 * dVL + dxl2 * dVR, den, rden);
+  if (!use_limiter) return central;
+  const R qL = *q;
+  const R qR = fabs(Arith<R>::rdiv(theta * dVR, dxr, rdxr));
+  *q = qR;
+  return (copysign(R(0.5), dVL) + copysign(R(0.5), dVR)) * fmin(fmin(qL, qR), fabs(central));
+}
+
 // ReconstructionType (IncrementDivergence_FV__Form.f90:239-241, 637-655).
 enum Recon { RECON_LINEAR = 0, RECON_PARABOLIC = 1 };
 

@@ -4,7 +4,8 @@ Usage:
   python -m genasis_amd.profiling trace  <kt_results.db>
   python -m genasis_amd.profiling counter <results.db> <COUNTER>
 
-``trace`` prints per-kernel calls, total and average duration (ns) from a
+``trace`` prints per-kernel calls, total and average duration (microseconds,
+as the rocpd ``top_kernels`` view gives them) from a
 ``rocprofv3 --kernel-trace --stats`` run; ``counter`` prints the per-dispatch
 mean of one PMC counter per kernel from a ``rocprofv3 --pmc`` run.
 """
@@ -20,9 +21,10 @@ def kernel_trace(db_path: str, top: int = 12) -> str:
     rows = db.execute(
         "select name, total_calls, total_duration, average, percentage "
         "from top_kernels").fetchall()
-    out = [f"{'kernel':64s} {'calls':>6s} {'total_ns':>14s} {'avg_ns':>12s} {'pct':>6s}"]
+    # top_kernels divides the nanosecond timestamps by 1000
+    out = [f"{'kernel':64s} {'calls':>6s} {'total_us':>14s} {'avg_us':>12s} {'pct':>6s}"]
     for name, calls, total, avg, pct in rows[:top]:
-        out.append(f"{name[:64]:64s} {calls:6d} {total:14.0f} {avg:12.1f} {pct:6.2f}")
+        out.append(f"{name[:64]:64s} {calls:6d} {total:14.1f} {avg:12.1f} {pct:6.2f}")
     return "\n".join(out)
 
 
