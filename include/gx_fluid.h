@@ -32,6 +32,10 @@
  *                      (…/Evolutions/Steps/Step_RK_C_ASC__Template.f90:
  *                      526-537, 1370-1422; …/Increments/IncrementDivergence_
  *                      FV__Form.f90:1002-1069)
+ *   gx_set_record_sources / gx_get_sources  ApplyDivergence_C's
+ *                      RecordDivergenceKernel into Current % Sources
+ *                      (…/Evolutions/Steps/Step_RK_C_ASC__Template.f90:
+ *                      1572-1589, Step_RK_C_ASC__Kernel.f90:12-46)
  *   gx_write_checkpoint / gx_read_checkpoint  Integrator_Template % Write at a
  *                      checkpoint and the RestartFrom path of PrepareInitial
  *                      (…/Evolutions/Integrators/Integrator_Template.f90:
