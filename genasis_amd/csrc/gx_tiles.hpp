@@ -466,11 +466,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    if (tid < nxs + nys) {
+    // LINEAR FP64: every wave takes the edge cells, wave w field w (wave 0
+    // also field 4), so no wave adds more than two slopes per edge task to
+    // its own ten.  Otherwise wave 0 takes all five fields (FP32 keeps its
+    // 128 registers).
+    constexpr bool SPREAD = !PARA && sizeof(R) == sizeof(double);
+    const int ut = SPREAD ? tid % WAVE : tid;
+    if (ut < nxs + nys) {
       int base, stride, l, out, gd;
-      bool isx = tid < nxs;
+      bool isx = ut < nxs;
       if (isx) {
-        const int j = tid >> 1, side = tid & 1;
+        const int j = ut >> 1, side = ut & 1;
         const int i = side ? tw + 1 : 0;  // slope index of cell x0-1+i
         base = (j + 2) * HX + (i + 1);
         stride = 1;
@@ -478,7 +484,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         gd = 0;
         out = j * SXW + i;
       } else {
-        const int u = tid - nxs;
+        const int u = ut - nxs;
         const int i = u >> 1, side = u & 1;
         const int j = side ? th + 1 : 0;  // slope index of cell y0-1+j
         base = (j + 1) * HX + (i + 2);
@@ -507,6 +513,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
             else
               E->sEy[f][e] = vI;
           }
+        }
+      } else if constexpr (SPREAD) {
+        const int wv = __builtin_amdgcn_readfirstlane(tid / WAVE);
+        for (int f = wv; f < 5; f += NT / WAVE) {
+          const R sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
+                                 sP[f][base + stride], theta, ph.use_limiter);
+          if (isx)
+            sSx[f][out] = sv;
+          else
+            sSy[f][out] = sv;
         }
       } else {
 #pragma unroll
