@@ -466,11 +466,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    // LINEAR FP64: every wave takes the edge cells, wave w field w (wave 0
-    // also field 4), so no wave adds more than two slopes per edge task to
-    // its own ten.  Otherwise wave 0 takes all five fields (FP32 keeps its
-    // 128 registers).
-    constexpr bool SPREAD = !PARA && sizeof(R) == sizeof(double);
+    // FP64: every wave takes the edge cells, wave w field w (wave 0 also
+    // field 4), so no wave adds more than two fields per edge task to its
+    // own work.  FP32 has wave 0 take all five fields: with four blocks per
+    // CU the others fill its barrier waits, and sharing measured slower.
+    constexpr bool SPREAD = sizeof(R) == sizeof(double);
     const int ut = SPREAD ? tid % WAVE : tid;
     if (ut < nxs + nys) {
       int base, stride, l, out, gd;
@@ -495,10 +495,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
       }
       if constexpr (PARA) {
         // left/bottom edge cell: its outer-face value; right/top: its inner
-        const bool hi = isx ? (tid & 1) : ((tid - nxs) & 1);
-        const int e = isx ? (tid >> 1) : ((tid - nxs) >> 1);
-#pragma unroll
-        for (int f = 0; f < 5; f++) {
+        const bool hi = isx ? (ut & 1) : ((ut - nxs) & 1);
+        const int e = isx ? (ut >> 1) : ((ut - nxs) >> 1);
+        const int f0 = SPREAD ? __builtin_amdgcn_readfirstlane(tid / WAVE) : 0;
+        const int fs = SPREAD ? NT / WAVE : 1;
+        for (int f = f0; f < 5; f += fs) {
           R vI, vO;
           parabolic_lds(sG, gd, l, sP[f][base - stride], sP[f][base], sP[f][base + stride],
                         &vI, &vO);
