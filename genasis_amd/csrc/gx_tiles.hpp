@@ -397,11 +397,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
     }
     face_flux(WL, WR, 2, (flk >> 2) & 1, ph, Flo);
   }
-  if (tid < nhalo) {
+  // Halo cells of the plane tile.  FP64: waves 2 and 3 convert the next
+  // plane's halo while waves 0 and 1 compute the tile-edge faces, so the
+  // plane's load phase only stores them.
+  constexpr bool LATE = sizeof(R) == sizeof(double);
+  const int hid = LATE ? tid - 2 * WAVE : tid;
+  const bool htask = hid >= 0 && hid < nhalo;
+  R hv[5];
+  if (htask) {
     int hx, hy;
-    halo_cell(tid, x0, y0, tw, th, &hx, &hy);
+    halo_cell(hid, x0, y0, tw, th, &hx, &hy);
     hpos = (hy - y0 + 2) * HX + (hx - x0 + 2);
     load_raw(a, hx, hy, z0, hr, &hflip);
+    if constexpr (LATE) raw_to_prim(a, hr, hflip, hv);
   }
 
   for (int k = z0; k < zend; ++k) {
@@ -413,14 +421,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
 #pragma unroll
       for (int f = 0; f < 5; f++) sP[f][ctr] = Vk[f];
     }
-    if (tid < nhalo) {
-      R V[5];
-      raw_to_prim(a, hr, hflip, V);
+    if (htask) {
+      if constexpr (!LATE) raw_to_prim(a, hr, hflip, hv);
 #pragma unroll
-      for (int f = 0; f < 5; f++) sP[f][hpos] = V[f];
+      for (int f = 0; f < 5; f++) sP[f][hpos] = hv[f];
       if (more) {
         int hx, hy;
-        halo_cell(tid, x0, y0, tw, th, &hx, &hy);
+        halo_cell(hid, x0, y0, tw, th, &hx, &hy);
         load_raw(a, hx, hy, k + 1, hr, &hflip);
       }
     }
@@ -584,6 +591,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
       if (tid < th) x_face(ph, sP, sSx, sFx, sG, tw, tid, flx);
       __builtin_amdgcn_sched_barrier(0);
       if (tid >= WAVE && tid - WAVE < tw) y_face(ph, sP, sSy, sFy, sG, tid - WAVE, th, flx);
+    }
+    if constexpr (LATE) {
+      if (htask && more) raw_to_prim(a, hr, hflip, hv);  // the next plane's halo
     }
     __syncthreads();
 
