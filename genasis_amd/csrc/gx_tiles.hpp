@@ -401,6 +401,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
   // plane's halo while waves 0 and 1 compute the tile-edge faces, so the
   // plane's load phase only stores them.
   constexpr bool LATE = sizeof(R) == sizeof(double);
+  const int wave = __builtin_amdgcn_readfirstlane(tid / WAVE);
   const int hid = LATE ? tid - 2 * WAVE : tid;
   const bool htask = hid >= 0 && hid < nhalo;
   R hv[5];
@@ -415,6 +416,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
   for (int k = z0; k < zend; ++k) {
     const bool more = k + 1 < zend;
     R ar[5], zn[5];
+    R Vk2[5];  // primitives of the own column at k+2 (FP64: converted early)
     int znflip = -1;
     unsigned fln = 0, flx = 0;  // own flags at k+1; flag of this thread's edge face
     if (active) {
@@ -473,7 +475,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    // FP64: every wave takes the edge cells, wave w field w (wave 0 also
+    // FP64: every wave takes the edge cells, wave w field w (wave 3 also
     // field 4), so no wave adds more than two fields per edge task to its
     // own work.  FP32 has wave 0 take all five fields: with four blocks per
     // CU the others fill its barrier waits, and sharing measured slower.
@@ -504,8 +506,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         // left/bottom edge cell: its outer-face value; right/top: its inner
         const bool hi = isx ? (ut & 1) : ((ut - nxs) & 1);
         const int e = isx ? (ut >> 1) : ((ut - nxs) >> 1);
-        const int f0 = SPREAD ? __builtin_amdgcn_readfirstlane(tid / WAVE) : 0;
-        const int fs = SPREAD ? NT / WAVE : 1;
+        const int f0 = SPREAD ? wave : 0;
+        const int fs = SPREAD ? (wave == 3 ? 1 : 5) : 1;
         for (int f = f0; f < 5; f += fs) {
           R vI, vO;
           parabolic_lds(sG, gd, l, sP[f][base - stride], sP[f][base], sP[f][base + stride],
@@ -523,8 +525,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
           }
         }
       } else if constexpr (SPREAD) {
-        const int wv = __builtin_amdgcn_readfirstlane(tid / WAVE);
-        for (int f = wv; f < 5; f += NT / WAVE) {
+        for (int f = wave; f < 5; f += wave == 3 ? 1 : 5) {
           const R sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
                                  sP[f][base + stride], theta, ph.use_limiter);
           if (isx)
@@ -543,6 +544,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
             sSy[f][out] = sv;
         }
       }
+    }
+    // FP64: waves 0 and 1, whose slope work ends a field early, convert the
+    // own column's next z-stencil value here; waves 2 and 3 do it beside the
+    // halo conversion in the face step
+    if constexpr (LATE) {
+      if (active && wave < 2) raw_to_prim(a, zr, zflip, Vk2);
     }
     __syncthreads();
 
@@ -594,15 +601,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
     }
     if constexpr (LATE) {
       if (htask && more) raw_to_prim(a, hr, hflip, hv);  // the next plane's halo
+      if (active && wave >= 2) raw_to_prim(a, zr, zflip, Vk2);
     }
     __syncthreads();
 
     if (active) {
       // z stencil of plane k: slope at k+1, flux through the face k+1/2.
-      R Vk2[5], sk1[5], Fhi[5];
+      R sk1[5], Fhi[5];
       {
         R WL[5], WR[5];
-        raw_to_prim(a, zr, zflip, Vk2);
+        if constexpr (!LATE) raw_to_prim(a, zr, zflip, Vk2);
         if constexpr (PARA) {
 #pragma unroll
           for (int f = 0; f < 5; f++) {
