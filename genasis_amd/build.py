@@ -1,12 +1,14 @@
 """Compile the HIP library for gfx950 in-tree (genasis_amd/lib/libgx_fluid.so).
 
 -ffp-contract=off keeps the reference's rounding (no fused multiply-add);
-packed FP32 instructions are disabled for all HIP code.
+packed FP32 instructions are disabled for all HIP code.  The translation
+units compile in parallel into _build/ and are linked into one library.
 """
 from __future__ import annotations
 
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -14,22 +16,34 @@ SRCS = [os.path.join(HERE, "csrc", f) for f in ("gx_fluid.hip", "gx_batch.hip",
                                                         "gx_adjoint.hip", "gx_checkpoint.hip")]
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libgx_fluid.so")
+OBJ_DIR = os.path.join(ROOT, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
 
+def _compile(src: str) -> str:
+    obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
+    r = subprocess.run([HIPCC, *FLAGS, "-c", src, "-o", obj], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src}:\n" + r.stdout + r.stderr)
+    return obj
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=len(SRCS)) as ex:
+        objs = list(ex.map(_compile, SRCS))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *SRCS, "-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    r = subprocess.run([HIPCC, *FLAGS, "-shared", *objs, "-o", tmp],
+                       capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+        raise RuntimeError("hipcc link failed:\n" + r.stdout + r.stderr)
     if verbose and r.stderr:
         print(r.stderr)
     os.replace(tmp, LIB)

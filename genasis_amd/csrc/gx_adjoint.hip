@@ -594,8 +594,10 @@ __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const d
   if (threadIdx.x != 0) return;
   double gb = gsum;
   const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
-  const int t = *idx;
-  if (invdt > DBL_MIN && ts > 0.0 && t >= 0 && t < ncell) {
+  const int t = idx[0];
+  if (invdt > DBL_MIN && ts > 0.0 && !(t >= 0 && t < ncell)) {
+    idx[1] += 1;  // no cell reproduced the maximum: gx_adjoint_backward fails
+  } else if (invdt > DBL_MIN && ts > 0.0) {
     const double invb = dtb * (-ts / invdt);
     const int x = t % a.n[0], y = (t / a.n[0]) % a.n[1], z = t / (a.n[0] * a.n[1]);
     double V[5];
@@ -623,7 +625,7 @@ __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const d
     }
   }
   *gbar += gb;
-  *idx = 0x7fffffff;  // ready for the next step's fold
+  idx[0] = 0x7fffffff;  // ready for the next step's fold
 }
 
 // ComputeFromInitial's reverse: D = N, S_k = (M N) V_k, G = E + S.V/2 with
@@ -876,10 +878,12 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   HIP_TRY(hipMemset(A->partials, 0, sizeof(double) * 2 * 6 * (size_t)A->part_stride));
   HIP_TRY(hipMalloc(&A->partials2, sizeof(double) * 2 * PART_BLOCKS));
   HIP_TRY(hipMalloc(&A->gb, sizeof(double)));
-  HIP_TRY(hipMalloc(&A->argmax, sizeof(int)));
+  // argmax[0]: cell of the step's CFL maximum (the dt kernel restores
+  // INT_MAX after each use); argmax[1]: steps whose maximum no cell matched
+  HIP_TRY(hipMalloc(&A->argmax, 2 * sizeof(int)));
   {
-    const int none = 0x7fffffff;  // the dt kernel restores it after each use
-    HIP_TRY(hipMemcpy(A->argmax, &none, sizeof(int), hipMemcpyHostToDevice));
+    const int init[2] = {0x7fffffff, 0};
+    HIP_TRY(hipMemcpy(A->argmax, init, sizeof(init), hipMemcpyHostToDevice));
   }
   HIP_TRY(hipMalloc(&A->band, sizeof(double) * c->n[1]));
   HIP_TRY(hipMalloc(&A->lossp, sizeof(double) * blocks_for(c->ncell)));
@@ -889,6 +893,7 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
 extern "C" {
 
 int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycles) {
+  GX_API_BEGIN
   if (!c || nSteps < 1 || checkpointEvery < 1 || nRampCycles < 1)
     return fail(GX_E_ARG, "gx_adjoint_forward: bad argument");
   if (!c->uploaded || !c->prims_valid)
@@ -955,9 +960,11 @@ int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycl
   HIP_TRY(hipStreamSynchronize(c->stream));
   A->forward_done = true;
   return GX_OK;
+  GX_API_END("gx_adjoint_forward")
 }
 
 int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *gammaBar) {
+  GX_API_BEGIN
   if (!c || !gammaBar) return fail(GX_E_ARG, "gx_adjoint_backward: null argument");
   AdjointState *A = c->adj;
   if (!A || !A->forward_done)
@@ -1040,13 +1047,24 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   HIP_TRY(hipGetLastError());
   if (P0bar) HIP_TRY(hipMemcpyAsync(P0bar, A->P0b, s5, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(gammaBar, A->gb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  int missed = 0;
+  HIP_TRY(hipMemcpyAsync(&missed, A->argmax + 1, sizeof(int), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipMemsetAsync(A->argmax + 1, 0, sizeof(int), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  A->bar_ready = true;
   c->invdt_valid = false;  // the recomputation reused the CFL accumulator
+  if (missed)
+    return fail(GX_E_STATE,
+                "gx_adjoint_backward: the recomputed CFL terms did not reproduce the "
+                "recorded maximum in " + std::to_string(missed) +
+                " step(s); the time-step part of the gradient is missing");
+  A->bar_ready = true;
   return GX_OK;
+  GX_API_END("gx_adjoint_backward")
 }
 
 int gx_adjoint_get_bar(gx_ctx *c, double *P0bar) {
+  GX_API_BEGIN
   if (!c || !P0bar) return fail(GX_E_ARG, "gx_adjoint_get_bar: null argument");
   AdjointState *A = c->adj;
   if (!A || !A->bar_ready)
@@ -1055,10 +1073,12 @@ int gx_adjoint_get_bar(gx_ctx *c, double *P0bar) {
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
+  GX_API_END("gx_adjoint_get_bar")
 }
 
 int gx_adjoint_kinetic_energy(gx_ctx *c, int component, double lo, double hi,
                               double *loss, double *Ubar) {
+  GX_API_BEGIN
   if (!c || !loss || component < 0 || component > 2)
     return fail(GX_E_ARG, "gx_adjoint_kinetic_energy: bad argument");
   if (!c->adj) return fail(GX_E_STATE, "gx_adjoint_kinetic_energy: no forward run recorded");
@@ -1095,6 +1115,7 @@ int gx_adjoint_kinetic_energy(gx_ctx *c, int component, double lo, double hi,
   for (double v : part) s += v;
   *loss = s;
   return GX_OK;
+  GX_API_END("gx_adjoint_kinetic_energy")
 }
 
 // Host evaluations of the device code's reverse sweeps, for checking.

@@ -391,6 +391,7 @@ static int check_problem(gx_batch *b, int p, const char *who) {
 extern "C" {
 
 int gx_batch_destroy(gx_batch *b) {
+  GX_API_BEGIN
   if (!b) return GX_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
@@ -402,10 +403,12 @@ int gx_batch_destroy(gx_batch *b) {
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return GX_OK;
+  GX_API_END("gx_batch_destroy")
 }
 
 int gx_batch_create(int nProblems, const gx_grid *grids, const gx_params *params,
                     gx_batch **out) {
+  GX_API_BEGIN
   if (!out || !grids || nProblems < 1)
     return fail(GX_E_ARG, "gx_batch_create: bad argument");
   *out = nullptr;
@@ -512,9 +515,11 @@ int gx_batch_create(int nProblems, const gx_grid *grids, const gx_params *params
   }
   *out = b;
   return GX_OK;
+  GX_API_END("gx_batch_create")
 }
 
 int gx_batch_set_primitive(gx_batch *b, int iProblem, const double *primitive) {
+  GX_API_BEGIN
   int rc = check_problem(b, iProblem, "gx_batch_set_primitive");
   if (rc) return rc;
   if (!primitive) return fail(GX_E_ARG, "gx_batch_set_primitive: null argument");
@@ -543,9 +548,11 @@ int gx_batch_set_primitive(gx_batch *b, int iProblem, const double *primitive) {
   b->uploaded[iProblem] = 1;
   b->cfl_valid[iProblem] = 0;
   return GX_OK;
+  GX_API_END("gx_batch_set_primitive")
 }
 
 int gx_batch_set_integrator(gx_batch *b, double FinishTime, int FinishCycle, int nRampCycles) {
+  GX_API_BEGIN
   if (!b || nRampCycles < 1 || FinishCycle < 0)
     return fail(GX_E_ARG, "gx_batch_set_integrator: bad argument");
   b->finish_time = FinishTime;
@@ -553,9 +560,11 @@ int gx_batch_set_integrator(gx_batch *b, double FinishTime, int FinishCycle, int
   b->n_ramp = nRampCycles;
   drop_graph(b);
   return GX_OK;
+  GX_API_END("gx_batch_set_integrator")
 }
 
 int gx_batch_step(gx_batch *b, int nSteps) {
+  GX_API_BEGIN
   if (!b || nSteps < 0) return fail(GX_E_ARG, "gx_batch_step: bad argument");
   for (int p = 0; p < b->np; p++)
     if (!b->uploaded[p]) return fail(GX_E_STATE, "gx_batch_step: a problem has no state");
@@ -591,15 +600,19 @@ int gx_batch_step(gx_batch *b, int nSteps) {
   }
   HIP_TRY(hipGraphLaunch(b->graph, b->stream));
   return GX_OK;
+  GX_API_END("gx_batch_step")
 }
 
 int gx_batch_sync(gx_batch *b) {
+  GX_API_BEGIN
   if (!b) return fail(GX_E_ARG, "gx_batch_sync: null batch");
   HIP_TRY(hipStreamSynchronize(b->stream));
   return GX_OK;
+  GX_API_END("gx_batch_sync")
 }
 
 int gx_batch_cfl(gx_batch *b, double *candidates) {
+  GX_API_BEGIN
   if (!b || !candidates) return fail(GX_E_ARG, "gx_batch_cfl: null argument");
   for (int p = 0; p < b->np; p++)
     if (!b->uploaded[p]) return fail(GX_E_STATE, "gx_batch_cfl: a problem has no state");
@@ -615,9 +628,11 @@ int gx_batch_cfl(gx_batch *b, double *candidates) {
     candidates[p] = b->params.CourantFactor * fmin(DBL_MAX, 1.0 / fmax(DBL_MIN, invdt));
   }
   return GX_OK;
+  GX_API_END("gx_batch_cfl")
 }
 
 int gx_batch_times(gx_batch *b, double *time, double *lastTimeStep, int *cycle) {
+  GX_API_BEGIN
   if (!b || !time) return fail(GX_E_ARG, "gx_batch_times: null argument");
   HIP_TRY(hipMemcpyAsync(time, b->time, sizeof(double) * b->np, hipMemcpyDeviceToHost,
                          b->stream));
@@ -629,18 +644,22 @@ int gx_batch_times(gx_batch *b, double *time, double *lastTimeStep, int *cycle) 
                            b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return GX_OK;
+  GX_API_END("gx_batch_times")
 }
 
 int gx_batch_get_conserved(gx_batch *b, int iProblem, double *conserved) {
+  GX_API_BEGIN
   int rc = check_problem(b, iProblem, "gx_batch_get_conserved");
   if (rc) return rc;
   if (!conserved) return fail(GX_E_ARG, "gx_batch_get_conserved: null argument");
   const BatchProblem &P = b->prob[iProblem];
   return get_values(b->stream, b->fp32, conserved,
                     (const char *)b->U + 5 * (size_t)P.off * b->esz, 5 * (size_t)P.ncell);
+  GX_API_END("gx_batch_get_conserved")
 }
 
 int gx_batch_get_flags(gx_batch *b, int iProblem, unsigned char *flags) {
+  GX_API_BEGIN
   int rc = check_problem(b, iProblem, "gx_batch_get_flags");
   if (rc) return rc;
   if (!flags) return fail(GX_E_ARG, "gx_batch_get_flags: null argument");
@@ -649,10 +668,12 @@ int gx_batch_get_flags(gx_batch *b, int iProblem, unsigned char *flags) {
                          b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return GX_OK;
+  GX_API_END("gx_batch_get_flags")
 }
 
 int gx_batch_set_tuning(gx_batch *b, int nClasses, const int *sizeClass, const int *zChunk,
                         int useGraph) {
+  GX_API_BEGIN
   if (!b || nClasses < 0 || (nClasses > 0 && (!sizeClass || !zChunk)))
     return fail(GX_E_ARG, "gx_batch_set_tuning: bad argument");
   for (int i = 0; i < nClasses; i++) {
@@ -661,10 +682,12 @@ int gx_batch_set_tuning(gx_batch *b, int nClasses, const int *sizeClass, const i
   }
   b->use_graph = useGraph != 0;
   return build_tiles(b, {});
+  GX_API_END("gx_batch_set_tuning")
 }
 
 int gx_batch_get_tuning(gx_batch *b, int capacity, int *sizeClass, int *zChunk,
                         int *nClasses) {
+  GX_API_BEGIN
   if (!b || !nClasses) return fail(GX_E_ARG, "gx_batch_get_tuning: null argument");
   int i = 0;
   for (const auto &kv : b->class_tz) {
@@ -676,23 +699,34 @@ int gx_batch_get_tuning(gx_batch *b, int capacity, int *sizeClass, int *zChunk,
   }
   *nClasses = i;
   return GX_OK;
+  GX_API_END("gx_batch_get_tuning")
 }
 
 int gx_batch_autotune(gx_batch *b, int nRepeat) {
+  GX_API_BEGIN
   if (!b || nRepeat < 1) return fail(GX_E_ARG, "gx_batch_autotune: bad argument");
   for (int p = 0; p < b->np; p++)
     if (!b->uploaded[p]) return fail(GX_E_STATE, "gx_batch_autotune: a problem has no state");
   // Stage 1 writes only Y and W, so timing it leaves the state untouched.
-  // Every problem is made active with its current dt; the flags stay.
+  // Every problem is made active with its current dt; the flags stay.  On
+  // any failure the z-chunks in force before the call and the full tile
+  // list are restored before returning.
+  const std::map<int, int> before = b->class_tz;
   std::vector<int> ones(b->np, 1);
   HIP_TRY(hipMemcpy(b->active(), ones.data(), sizeof(int) * b->np, hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(b->src_prim(), 0, sizeof(int) * b->np));
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = GX_OK;
+  auto hip_ok = [&](hipError_t e, const char *what) {
+    if (e != hipSuccess && rc == GX_OK)
+      rc = fail(GX_E_HIP, std::string("gx_batch_autotune: ") + what + ": " + hipGetErrorString(e));
+    return rc == GX_OK;
+  };
+  hip_ok(hipEventCreate(&e0), "hipEventCreate");
+  hip_ok(hipEventCreate(&e1), "hipEventCreate");
   std::map<int, int> best;
-  for (const auto &kv : std::map<int, int>(b->class_tz)) {
+  for (const auto &kv : before) {
+    if (rc) break;
     const int cls = kv.first;
     std::vector<char> keep(b->np, 0);
     int nmax = 1;
@@ -704,14 +738,16 @@ int gx_batch_autotune(gx_batch *b, int nRepeat) {
     float best_ms = 1e30f;
     int best_tz = kv.second;
     for (int tz : {4, 8, 16, 32, 64}) {
+      if (rc) break;
       if (tz > nmax && tz != 4) continue;
       b->class_tz[cls] = tz;
       rc = build_tiles(b, keep);
-      if (rc) break;
       float ms_min = 1e30f;
       for (int r = 0; r < nRepeat + 1 && rc == GX_OK; r++) {
-        HIP_TRY(hipMemsetAsync(b->counters(), 0, sizeof(int) * NXCD, b->stream));
-        HIP_TRY(hipEventRecord(e0, b->stream));
+        if (!hip_ok(hipMemsetAsync(b->counters(), 0, sizeof(int) * NXCD, b->stream),
+                    "hipMemsetAsync") ||
+            !hip_ok(hipEventRecord(e0, b->stream), "hipEventRecord"))
+          break;
         rc = by_precision(b->fp32, [&](auto tag) {
           using R = decltype(tag);
           batch_stage_kernel<R, 1><<<b->stage_blocks, NT, 0, b->stream>>>(
@@ -719,27 +755,28 @@ int gx_batch_autotune(gx_batch *b, int nRepeat) {
           HIP_TRY(hipGetLastError());
           return GX_OK;
         });
-        HIP_TRY(hipEventRecord(e1, b->stream));
-        HIP_TRY(hipEventSynchronize(e1));
         float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (rc || !hip_ok(hipEventRecord(e1, b->stream), "hipEventRecord") ||
+            !hip_ok(hipEventSynchronize(e1), "hipEventSynchronize") ||
+            !hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime"))
+          break;
         if (r > 0) ms_min = std::min(ms_min, ms);  // the first launch warms up
       }
-      if (rc) break;
-      if (ms_min < best_ms) {
+      if (rc == GX_OK && ms_min < best_ms) {
         best_ms = ms_min;
         best_tz = tz;
       }
     }
-    if (rc) break;
-    b->class_tz[cls] = best_tz;
     best[cls] = best_tz;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  for (const auto &kv : best) b->class_tz[kv.first] = kv.second;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  b->class_tz = before;
+  if (rc == GX_OK)
+    for (const auto &kv : best) b->class_tz[kv.first] = kv.second;
   const int rc2 = build_tiles(b, {});
   return rc ? rc : rc2;
+  GX_API_END("gx_batch_autotune")
 }
 
 long long gx_batch_cells(gx_batch *b) { return b ? b->total_cells : 0; }

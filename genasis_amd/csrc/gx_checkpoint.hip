@@ -106,6 +106,7 @@ extern "C" {
 
 int gx_write_checkpoint(gx_ctx *c, const char *path, double time, long long cycle,
                         int iCheckpoint) {
+  GX_API_BEGIN
   const char *who = "gx_write_checkpoint";
   if (!c || !path) return fail(GX_E_ARG, "gx_write_checkpoint: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_write_checkpoint: no state uploaded");
@@ -178,10 +179,12 @@ int gx_write_checkpoint(gx_ctx *c, const char *path, double time, long long cycl
     return io_fail(who, path, "cannot rename the written file into place");
   }
   return GX_OK;
+  GX_API_END("gx_write_checkpoint")
 }
 
 int gx_read_checkpoint(gx_ctx *c, const char *path, int mode, double *time,
                        long long *cycle, int *iCheckpoint) {
+  GX_API_BEGIN
   const char *who = "gx_read_checkpoint";
   if (!c || !path) return fail(GX_E_ARG, "gx_read_checkpoint: null argument");
   if (mode != GX_RESTART_PRIMITIVE && mode != GX_RESTART_EXACT)
@@ -235,6 +238,7 @@ int gx_read_checkpoint(gx_ctx *c, const char *path, int mode, double *time,
   if (cycle) *cycle = h.cycle;
   if (iCheckpoint) *iCheckpoint = h.iCheckpoint;
   return GX_OK;
+  GX_API_END("gx_read_checkpoint")
 }
 
 }  // extern "C"

@@ -126,6 +126,7 @@ const char *gx_last_error(void) { return g_last_error.c_str(); }
 
 int gx_self_test_division(long long n, unsigned long long seed,
                           long long *mismatches) {
+  GX_API_BEGIN
   if (n < 0 || !mismatches) return fail(GX_E_ARG, "gx_self_test_division: bad argument");
   unsigned long long *d = nullptr;
   HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
@@ -139,9 +140,11 @@ int gx_self_test_division(long long n, unsigned long long seed,
     return fail(GX_E_HIP, std::string("gx_self_test_division: ") + hipGetErrorString(e));
   *mismatches = (long long)h;
   return GX_OK;
+  GX_API_END("gx_self_test_division")
 }
 
 int gx_destroy(gx_ctx *c) {
+  GX_API_BEGIN
   if (!c) return GX_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   gx::release_adjoint(c);
@@ -165,9 +168,11 @@ int gx_destroy(gx_ctx *c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GX_OK;
+  GX_API_END("gx_destroy")
 }
 
 int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
+  GX_API_BEGIN
   if (!grid || !out) return fail(GX_E_ARG, "gx_create: null argument");
   *out = nullptr;
   gx_params p;
@@ -246,15 +251,19 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
 #undef TRY_CLEAN
   *out = c;
   return GX_OK;
+  GX_API_END("gx_create")
 }
 
 int gx_set_tuning(gx_ctx *c, int zChunk) {
+  GX_API_BEGIN
   if (!c || zChunk < 1) return fail(GX_E_ARG, "gx_set_tuning: bad argument");
-  c->tz = zChunk;
+  c->tz = std::min(zChunk, c->n[2]);  // a longer chunk is the whole chart
   return GX_OK;
+  GX_API_END("gx_set_tuning")
 }
 
 int gx_set_record_sources(gx_ctx *c, int on) {
+  GX_API_BEGIN
   if (!c) return fail(GX_E_ARG, "gx_set_record_sources: null context");
   if (on) {
     const size_t bytes = 5 * c->esz * (size_t)c->ncell;
@@ -264,24 +273,30 @@ int gx_set_record_sources(gx_ctx *c, int on) {
   }
   c->record_sources = on != 0;
   return GX_OK;
+  GX_API_END("gx_set_record_sources")
 }
 
 int gx_get_sources(gx_ctx *c, double *sources) {
+  GX_API_BEGIN
   if (!c || !sources) return fail(GX_E_ARG, "gx_get_sources: null argument");
   if (!c->sources)
     return fail(GX_E_STATE, "gx_get_sources: recording was never enabled (gx_set_record_sources)");
   return get_values(c->stream, c->fp32, sources, c->sources, 5 * (size_t)c->ncell);
+  GX_API_END("gx_get_sources")
 }
 
 int gx_get_tuning(gx_ctx *c, int *zChunk) {
+  GX_API_BEGIN
   if (!c || !zChunk) return fail(GX_E_ARG, "gx_get_tuning: null argument");
   *zChunk = c->tz;
   return GX_OK;
+  GX_API_END("gx_get_tuning")
 }
 
 void *gx_stream(gx_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 int gx_set_profiling(gx_ctx *c, int on) {
+  GX_API_BEGIN
   if (!c) return fail(GX_E_ARG, "gx_set_profiling: null context");
   if (on && !c->ev[0])
     for (int e = 0; e < 6; e++) HIP_TRY(hipEventCreate(&c->ev[e]));
@@ -291,15 +306,18 @@ int gx_set_profiling(gx_ctx *c, int on) {
     c->kernel_n[k] = 0;
   }
   return GX_OK;
+  GX_API_END("gx_set_profiling")
 }
 
 int gx_kernel_times(gx_ctx *c, double ms[4], long long count[4]) {
+  GX_API_BEGIN
   if (!c || !ms || !count) return fail(GX_E_ARG, "gx_kernel_times: null argument");
   for (int k = 0; k < 4; k++) {
     ms[k] = c->kernel_ms[k];
     count[k] = c->kernel_n[k];
   }
   return GX_OK;
+  GX_API_END("gx_kernel_times")
 }
 
 // A new state starts new tallies: fluence, boundary tallies, total and
@@ -314,6 +332,7 @@ static int reset_tallies(gx_ctx *c) {
 }
 
 int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
+  GX_API_BEGIN
   if (!c || !primitive || !conserved) return fail(GX_E_ARG, "gx_set_state: null argument");
   const size_t n = 5 * (size_t)c->ncell;
   int rc = put_values(c->stream, c->fp32, c->P0, primitive, n);
@@ -328,9 +347,11 @@ int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
   c->prims_valid = true;
   c->invdt_valid = false;
   return GX_OK;
+  GX_API_END("gx_set_state")
 }
 
 int gx_set_primitive(gx_ctx *c, const double *primitive) {
+  GX_API_BEGIN
   if (!c || !primitive) return fail(GX_E_ARG, "gx_set_primitive: null argument");
   const int rc = put_values(c->stream, c->fp32, c->P0, primitive, 5 * (size_t)c->ncell);
   if (rc) return rc;
@@ -349,6 +370,7 @@ int gx_set_primitive(gx_ctx *c, const double *primitive) {
   c->prims_valid = true;
   c->invdt_valid = false;
   return GX_OK;
+  GX_API_END("gx_set_primitive")
 }
 
 static const int PRIM_FIELDS[5] = {GX_COMOVING_BARYON_DENSITY, GX_VELOCITY_U_1,
@@ -361,32 +383,36 @@ static const int CONS_FIELDS[5] = {GX_CONSERVED_BARYON_DENSITY,
                                    GX_CONSERVED_ENERGY};
 
 int gx_upload(gx_ctx *c, const double *value, long long ld, const int *map) {
+  GX_API_BEGIN
   if (!c || !value || !map) return fail(GX_E_ARG, "gx_upload: null argument");
   const int m0 = c->n[0] + 4, m1 = c->n[1] + 4, m2 = c->n[2] + 4;
   if (ld < (long long)m0 * m1 * m2) return fail(GX_E_ARG, "gx_upload: ldValue < nValues");
   for (int f = 0; f < 5; f++)
-    if (map[PRIM_FIELDS[f]] <= 0 || map[CONS_FIELDS[f]] <= 0)
-      return fail(GX_E_ARG, "gx_upload: primitive and conserved fields are required");
-  std::vector<double> prim(5 * (size_t)c->ncell), cons(5 * (size_t)c->ncell);
+    if (map[PRIM_FIELDS[f]] <= 0)
+      return fail(GX_E_ARG, "gx_upload: the primitive fields N, V_1..3, E are required");
+  // PrepareEvolution: the Current's proper-cell primitives; the conserved
+  // fields follow from them on the device (ComputeFromInitial).
+  std::vector<double> prim(5 * (size_t)c->ncell);
   for (int f = 0; f < 5; f++) {
     const double *vp = value + (size_t)(map[PRIM_FIELDS[f]] - 1) * ld;
-    const double *vc = value + (size_t)(map[CONS_FIELDS[f]] - 1) * ld;
     for (int z = 0; z < c->n[2]; z++)
       for (int y = 0; y < c->n[1]; y++)
         for (int x = 0; x < c->n[0]; x++) {
           const size_t src = (size_t)(x + 2) + (size_t)m0 * ((y + 2) + (size_t)m1 * (z + 2));
           const size_t dst = (size_t)f * c->ncell + x + (size_t)c->n[0] * (y + (size_t)c->n[1] * z);
           prim[dst] = vp[src];
-          cons[dst] = vc[src];
         }
   }
-  return gx_set_state(c, prim.data(), cons.data());
+  return gx_set_primitive(c, prim.data());
+  GX_API_END("gx_upload")
 }
 
 int gx_get_conserved(gx_ctx *c, double *conserved) {
+  GX_API_BEGIN
   if (!c || !conserved) return fail(GX_E_ARG, "gx_get_conserved: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_get_conserved: no state uploaded");
   return get_values(c->stream, c->fp32, conserved, c->U, 5 * (size_t)c->ncell);
+  GX_API_END("gx_get_conserved")
 }
 
 static int run_current(gx_ctx *c) {
@@ -398,7 +424,8 @@ static int run_current(gx_ctx *c) {
     using R = decltype(tag);
     StageArgs<R> a = base_args<R>(c);
     a.aux = (const R *)c->U;
-    current_kernel<R><<<grid_blocks(c->ncell), 256, 0, c->stream>>>(a, c_v, c->scratch);
+    current_kernel<R><<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
+        a, c_v, c->prims_valid ? (const R *)c->P0 : nullptr, c->scratch);
     return 0;
   });
   HIP_TRY(hipGetLastError());
@@ -406,6 +433,7 @@ static int run_current(gx_ctx *c) {
 }
 
 int gx_get_primitive(gx_ctx *c, double *primitive) {
+  GX_API_BEGIN
   if (!c || !primitive) return fail(GX_E_ARG, "gx_get_primitive: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_get_primitive: no state uploaded");
   const size_t bytes = sizeof(double) * (size_t)c->ncell;
@@ -421,9 +449,11 @@ int gx_get_primitive(gx_ctx *c, double *primitive) {
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
+  GX_API_END("gx_get_primitive")
 }
 
 int gx_download(gx_ctx *c, double *value, long long ld, const int *map) {
+  GX_API_BEGIN
   if (!c || !value || !map) return fail(GX_E_ARG, "gx_download: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_download: no state uploaded");
   const int m0 = c->n[0] + 4, m1 = c->n[1] + 4, m2 = c->n[2] + 4;
@@ -434,15 +464,6 @@ int gx_download(gx_ctx *c, double *value, long long ld, const int *map) {
   HIP_TRY(hipMemcpyAsync(host.data(), c->scratch, sizeof(double) * host.size(),
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  if (c->prims_valid) {
-    // The Current still holds the uploaded primitives.
-    std::vector<double> p(5 * (size_t)c->ncell);
-    rc = get_values(c->stream, c->fp32, p.data(), c->P0, p.size());
-    if (rc) return rc;
-    for (int f = 0; f < 5; f++)
-      memcpy(host.data() + (size_t)PRIM_FIELDS[f] * c->ncell,
-             p.data() + (size_t)f * c->ncell, sizeof(double) * c->ncell);
-  }
   const int g = 2;
   const int n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
   for (int f = 0; f < GX_N_FIELDS; f++) {
@@ -467,23 +488,29 @@ int gx_download(gx_ctx *c, double *value, long long ld, const int *map) {
     (void)n2;
   }
   return GX_OK;
+  GX_API_END("gx_download")
 }
 
 int gx_get_flags(gx_ctx *c, unsigned char *flags) {
+  GX_API_BEGIN
   if (!c || !flags) return fail(GX_E_ARG, "gx_get_flags: null argument");
   HIP_TRY(hipMemcpyAsync(flags, c->flags, (size_t)c->ncell, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
+  GX_API_END("gx_get_flags")
 }
 
 int gx_set_flags(gx_ctx *c, const unsigned char *flags) {
+  GX_API_BEGIN
   if (!c || !flags) return fail(GX_E_ARG, "gx_set_flags: null argument");
   HIP_TRY(hipMemcpyAsync(c->flags, flags, (size_t)c->ncell, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
+  GX_API_END("gx_set_flags")
 }
 
 int gx_upload_features(gx_ctx *c, const double *df, long long ld) {
+  GX_API_BEGIN
   if (!c || !df) return fail(GX_E_ARG, "gx_upload_features: null argument");
   const int m0 = c->n[0] + 4, m1 = c->n[1] + 4;
   std::vector<unsigned char> b((size_t)c->ncell, 0);
@@ -496,9 +523,11 @@ int gx_upload_features(gx_ctx *c, const double *df, long long ld) {
             b[x + (size_t)c->n[0] * (y + (size_t)c->n[1] * z)] |= (unsigned char)(1u << d);
         }
   return gx_set_flags(c, b.data());
+  GX_API_END("gx_upload_features")
 }
 
 int gx_download_features(gx_ctx *c, double *df, long long ld) {
+  GX_API_BEGIN
   if (!c || !df) return fail(GX_E_ARG, "gx_download_features: null argument");
   const int m0 = c->n[0] + 4, m1 = c->n[1] + 4;
   std::vector<unsigned char> b((size_t)c->ncell);
@@ -512,9 +541,11 @@ int gx_download_features(gx_ctx *c, double *df, long long ld) {
           df[dst] = ((b[x + (size_t)c->n[0] * (y + (size_t)c->n[1] * z)] >> d) & 1) ? 1.0 : 0.0;
         }
   return GX_OK;
+  GX_API_END("gx_download_features")
 }
 
 int gx_compute_time_step(gx_ctx *c, double *candidate) {
+  GX_API_BEGIN
   if (!c || !candidate) return fail(GX_E_ARG, "gx_compute_time_step: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_compute_time_step: no state uploaded");
   if (!c->invdt_valid) {
@@ -540,13 +571,16 @@ int gx_compute_time_step(gx_ctx *c, double *candidate) {
   const double ts = fmin(DBL_MAX, 1.0 / tsi);
   *candidate = c->params.CourantFactor * ts;
   return GX_OK;
+  GX_API_END("gx_compute_time_step")
 }
 
 int gx_step_rk2(gx_ctx *c, double time, double dt) {
+  GX_API_BEGIN
   (void)time;
   if (!c) return fail(GX_E_ARG, "gx_step_rk2: null context");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_step_rk2: no state uploaded");
   return gx::step_rk2(c, dt, nullptr);
+  GX_API_END("gx_step_rk2")
 }
 
 }  // extern "C"
@@ -655,22 +689,27 @@ extern "C" {
 
 
 int gx_sync(gx_ctx *c) {
+  GX_API_BEGIN
   if (!c) return fail(GX_E_ARG, "gx_sync: null context");
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
+  GX_API_END("gx_sync")
 }
 
 int gx_get_boundary_fluence(gx_ctx *c, double *fluence) {
+  GX_API_BEGIN
   if (!c || !fluence) return fail(GX_E_ARG, "gx_get_boundary_fluence: null argument");
   const size_t n = (size_t)fluence_count(c->n);
   HIP_TRY(hipMemcpyAsync(fluence, c->fluence, sizeof(double) * n, hipMemcpyDeviceToHost,
                          c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
+  GX_API_END("gx_get_boundary_fluence")
 }
 
 int gx_compute_tally(gx_ctx *c, int computeChange, double *interior, double *boundary,
                      double *total, double *change) {
+  GX_API_BEGIN
   if (!c) return fail(GX_E_ARG, "gx_compute_tally: null context");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_compute_tally: no state uploaded");
   by_precision(c->fp32, [&](auto tag) {
@@ -709,9 +748,11 @@ int gx_compute_tally(gx_ctx *c, int computeChange, double *interior, double *bou
   if (total) memcpy(total, c->tally_total, sizeof(in));
   if (change) memcpy(change, c->tally_change, sizeof(in));
   return GX_OK;
+  GX_API_END("gx_compute_tally")
 }
 
 int gx_totals(gx_ctx *c, double out[5]) {
+  GX_API_BEGIN
   if (!c || !out) return fail(GX_E_ARG, "gx_totals: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_totals: no state uploaded");
   by_precision(c->fp32, [&](auto tag) {
@@ -732,6 +773,7 @@ int gx_totals(gx_ctx *c, double out[5]) {
     out[f] = s;
   }
   return GX_OK;
+  GX_API_END("gx_totals")
 }
 
 }  // extern "C"

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -18,10 +19,30 @@ namespace gx {
 // Message of the last failed call, per host thread (gx_last_error).
 extern thread_local std::string g_last_error;
 
-inline int fail(int code, const std::string &msg) {
-  g_last_error = msg;
+inline int fail(int code, const char *msg) noexcept {
+  try {
+    g_last_error = msg;
+  } catch (...) {
+    g_last_error.clear();
+  }
   return code;
 }
+
+inline int fail(int code, const std::string &msg) noexcept { return fail(code, msg.c_str()); }
+
+// Every extern "C" body that returns a status runs between GX_API_BEGIN and
+// GX_API_END: a C++ exception (a host allocation in std::vector or
+// std::string, or anything else) becomes a status code and never crosses
+// the C-ABI into the caller.
+#define GX_API_BEGIN try {
+#define GX_API_END(who)                                                      \
+  }                                                                          \
+  catch (const std::bad_alloc &) {                                           \
+    return gx::fail(GX_E_NOMEM, who ": host allocation failed");             \
+  }                                                                          \
+  catch (...) {                                                              \
+    return gx::fail(GX_E_STATE, who ": internal error (C++ exception)");     \
+  }
 
 #define HIP_TRY(expr)                                                       \
   do {                                                                      \

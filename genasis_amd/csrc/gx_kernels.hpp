@@ -87,32 +87,73 @@ template <> struct Arith<float> {
 };
 
 // Fluid_P_I % ComputeFromConservedCommon, primitive part (F/Fluid_P__Kernel.
-// f90:94-192): N, V, E from D, S, G with the density floor and E >= 0.
+// f90:94-192): N, V, E from D, S, G with the density floor and E >= 0.  The
+// floor N_Min is compared in FP64 whatever the state's precision.
 template <typename R>
-__device__ __forceinline__ void prim_from_cons(const R U[5], R mass, R nmin, R V[5]) {
-  const R D = U[0];
+__device__ __forceinline__ void prim_from_cons(const R U[5], R mass, double nmin, R V[5]);
+
+template <>
+__device__ __forceinline__ void prim_from_cons<double>(const double U[5], double mass,
+                                                       double nmin, double V[5]) {
+  const double D = U[0];
   if (D > nmin) {
-    const R MD = mass * D;
+    const double MD = mass * D;
     V[0] = D;
-    if (MD > R(0x1p-100) && MD < R(0x1p100)) {
-      const R y = Arith<R>::recip(MD);
-      V[1] = Arith<R>::rdiv(U[1], MD, y);
-      V[2] = Arith<R>::rdiv(U[2], MD, y);
-      V[3] = Arith<R>::rdiv(U[3], MD, y);
+    if (MD > 0x1p-100 && MD < 0x1p100) {
+      const double y = 1.0 / MD;
+      V[1] = div_rcp(U[1], MD, y);
+      V[2] = div_rcp(U[2], MD, y);
+      V[3] = div_rcp(U[3], MD, y);
     } else {
       V[1] = U[1] / MD;
       V[2] = U[2] / MD;
       V[3] = U[3] / MD;
     }
-    V[4] = U[4] - R(0.5) * ((U[1] * V[1] + U[2] * V[2]) + U[3] * V[3]);
+    V[4] = U[4] - 0.5 * ((U[1] * V[1] + U[2] * V[2]) + U[3] * V[3]);
   } else {
     V[0] = nmin;
-    V[1] = R(0);
-    V[2] = R(0);
-    V[3] = R(0);
-    V[4] = R(0);
+    V[1] = 0.0;
+    V[2] = 0.0;
+    V[3] = 0.0;
+    V[4] = 0.0;
   }
-  if (V[4] < R(0)) V[4] = R(0);
+  if (V[4] < 0.0) V[4] = 0.0;
+}
+
+// FP32 state: the positivity decisions are FP64 ones.  D > N_Min is compared
+// in double; E = G - S.V/2 is formed in float, and where it lies within the
+// float evaluation's error bound of zero -- 2^-18 (|G| + S.V/2), some 60
+// times the bound of the three roundings of S/(M D) and the five of the sum
+// -- it is formed again in double, whose sign then decides the E < 0 repair.
+// Outside that band the float sign is the exact sign, so every cell takes
+// the branch FP64 arithmetic takes on the same float state.
+template <>
+__device__ __forceinline__ void prim_from_cons<float>(const float U[5], float mass,
+                                                      double nmin, float V[5]) {
+  const float D = U[0];
+  if ((double)D > nmin) {
+    const float y = __builtin_amdgcn_rcpf(mass * D);
+    V[0] = D;
+    V[1] = U[1] * y;
+    V[2] = U[2] * y;
+    V[3] = U[3] * y;
+    const float ke = 0.5f * ((U[1] * V[1] + U[2] * V[2]) + U[3] * V[3]);
+    float E = U[4] - ke;
+    if (!(fabsf(E) > 0x1p-18f * (fabsf(U[4]) + ke))) {
+      const double MD = (double)mass * (double)D;
+      const double v1 = (double)U[1] / MD, v2 = (double)U[2] / MD, v3 = (double)U[3] / MD;
+      const double e =
+          (double)U[4] - 0.5 * (((double)U[1] * v1 + (double)U[2] * v2) + (double)U[3] * v3);
+      E = e < 0.0 ? -1.0f : (float)e;
+    }
+    V[4] = E < 0.0f ? 0.0f : E;
+  } else {
+    V[0] = (float)nmin;
+    V[1] = 0.0f;
+    V[2] = 0.0f;
+    V[3] = 0.0f;
+    V[4] = 0.0f;
+  }
 }
 
 // Gradient_Kernel limited slope (OP/Calculus/Derivatives/Gradient_Kernel.f90:

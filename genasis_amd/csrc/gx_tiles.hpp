@@ -147,7 +147,7 @@ __device__ __forceinline__ void raw_to_prim(const StageArgs<R> &a, const R raw[5
 #pragma unroll
     for (int f = 0; f < 5; f++) V[f] = raw[f];
   } else {
-    prim_from_cons(raw, R(a.ph.mass), R(a.ph.nmin), V);
+    prim_from_cons(raw, R(a.ph.mass), a.ph.nmin, V);
   }
   V[1] = negate_if(V[1], flip == 0);
   V[2] = negate_if(V[2], flip == 1);
@@ -834,7 +834,8 @@ __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, 
   const int zend = min(z0 + a.tz, nz);
   const int tid = threadIdx.x;
   const R gm1 = R(a.ph.gamma) - R(1), st = R(a.ph.st);
-  const R mass = R(a.ph.mass), nmin = R(a.ph.nmin);
+  const R mass = R(a.ph.mass);
+  const double nmin = a.ph.nmin;
   const R sqrt_tiny = Arith<R>::sqrt_tiny;
 
   // Each thread converts the plane-tile cells t = tid + j NT (j < QJ); their
@@ -935,11 +936,14 @@ __global__ __launch_bounds__(NT) void shock_flags_kernel(StageArgs<R> a, uint8_t
                    (tile / (a.nbx * a.nby)) * a.tz, flags, L);
 }
 
-// Current fields of ComputeFromConservedCommon for download
-// (F/Fluid_P_I__Form.f90:604-722), proper cells, on the stored state, in FP64.
+// The 23 Current fields for download, proper cells, in FP64: those of
+// ComputeFromConservedCommon (F/Fluid_P_I__Form.f90:604-722) on the stored
+// conserved state, or, while the Current still holds uploaded primitives
+// (prim != null), those of ComputeFromPrimitiveCommon (:509-601) with the
+// conserved fields the upload derived from them.
 template <typename R>
 __global__ __launch_bounds__(256) void current_kernel(StageArgs<R> a, double c_v,
-                                                      double *out) {
+                                                      const R *prim, double *out) {
   const long long ntot = (long long)a.n[0] * a.n[1] * a.n[2];
   const double sqrt_huge = 1.3407807929942596e+154;  // sqrt(DBL_MAX)
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
@@ -947,7 +951,12 @@ __global__ __launch_bounds__(256) void current_kernel(StageArgs<R> a, double c_v
     double U[5], V[5];
 #pragma unroll
     for (int f = 0; f < 5; f++) U[f] = (double)a.aux[f * a.fs + i];
-    fix_conserved(U, a.ph.mass, a.ph.nmin, V);
+    if (prim) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) V[f] = (double)prim[f * a.fs + i];
+    } else {
+      fix_conserved(U, a.ph.mass, a.ph.nmin, V);
+    }
     double *o = out;
     const double g = a.ph.gamma, M = a.ph.mass;
     const double N = V[0], E = V[4];

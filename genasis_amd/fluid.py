@@ -200,6 +200,18 @@ class Step_RK2_C_ASC:
                                          fm.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
                     "gx_download")
 
+    def upload_features(self, df: np.ndarray):
+        """DiffusiveFlux_I_1..3 in the reference layout: a (3, ldValue)
+        C-contiguous array (column d is df[d])."""
+        v = np.ascontiguousarray(df, dtype=np.float64)
+        _capi.check(self.lib.gx_upload_features(self._h, _pd(v), v.shape[1]),
+                    "gx_upload_features")
+
+    def download_features(self, df: np.ndarray):
+        assert df.flags.c_contiguous and df.dtype == np.float64 and df.shape[0] == 3
+        _capi.check(self.lib.gx_download_features(self._h, _pd(df), df.shape[1]),
+                    "gx_download_features")
+
     # --- the two boundary calls of the integrator --------------------------
     def time_step_candidate(self) -> float:
         c = ctypes.c_double()

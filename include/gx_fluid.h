@@ -143,10 +143,12 @@ int gx_destroy(gx_ctx *ctx);
  * column-major: field f of value iv at value[(fieldMap[f]-1)*ldValue + iv],
  * nValues = prod(nCells+4) with x fastest.  fieldMap has GX_N_FIELDS entries,
  * 1-based column numbers; an entry <= 0 marks an absent field.  The five
- * primitive fields (N, V_1..3, E) and the five conserved fields (D, S_1..3,
- * G) must be present.  The first step after an upload reconstructs from the
- * uploaded primitives, as the reference's first stage does.  Shock flags are
- * cleared. */
+ * primitive fields (N, V_1..3, E) must be present; the conserved fields and
+ * the other Current fields follow from them on the device, as
+ * PrepareEvolution's ComputeFromInitial does (the columns the host holds for
+ * them are not read).  The first step after an upload reconstructs from the
+ * uploaded primitives, as the reference's first stage does.  Shock flags,
+ * boundary fluence and tallies are cleared. */
 int gx_upload(gx_ctx *ctx, const double *value, long long ldValue,
               const int *fieldMap);
 
@@ -255,9 +257,10 @@ int gx_get_flags(gx_ctx *ctx, unsigned char *flags);
 int gx_set_flags(gx_ctx *ctx, const unsigned char *flags);
 
 /* Launch-shape control for measurement: z-chunk length of the fused stage
- * kernel (planes per thread block).  The default, set by gx_create, is 32
- * planes, halved (down to 4) while a launch would hold fewer than 512 blocks
- * of 16 x 16 cells; gx_get_tuning reads the current value. */
+ * kernel (planes per thread block), >= 1; a value above nCells[2] is taken
+ * as nCells[2].  The default, set by gx_create, is 32 planes, halved (down
+ * to 4) while a launch would hold fewer than 512 blocks of 16 x 16 cells;
+ * gx_get_tuning reads the current value. */
 int gx_set_tuning(gx_ctx *ctx, int zChunk);
 int gx_get_tuning(gx_ctx *ctx, int *zChunk);
 

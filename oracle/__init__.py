@@ -56,6 +56,8 @@ def _load():
         lib.gxo_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD]
         lib.gxo_get_sources.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD]
         lib.gxo_get_flags.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD]
+        lib.gxo_fallback_counts.argtypes = [ctypes.c_void_p,
+                                            ctypes.POINTER(ctypes.c_long), ctypes.c_int]
         lib.gxo_set_flags.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD]
         lib.gxo_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int, _PD, _PD, _PD]
         lib.gxo_limited_slope.restype = _D
@@ -162,6 +164,14 @@ class OracleFluid:
         for f in range(6):
             self._lib.gxo_get_sources(self._h, f, _ptr(out[f]))
         return out
+
+    def fallback_counts(self, reset=False):
+        """Proper cells ComputeFromConserved repaired since the last reset:
+        {"stage": (vacuum, energy), "store": (vacuum, energy)} for the stage-2
+        input Y and for StoreSolution; vacuum = D <= N_Min, energy = E < 0."""
+        out = (ctypes.c_long * 4)()
+        self._lib.gxo_fallback_counts(self._h, out, int(bool(reset)))
+        return {"stage": (int(out[0]), int(out[1])), "store": (int(out[2]), int(out[3]))}
 
     def flags(self, d) -> np.ndarray:
         out = np.empty(self.shape)

@@ -69,6 +69,11 @@ struct gxo_ctx {
    * 63-64): TallyBoundaryLocal, TallyBoundaryGlobal, TallyTotal, TallyChange */
   double tally_local[GXO_TALLY_N], tally_global[GXO_TALLY_N];
   double tally_total[GXO_TALLY_N], tally_change[GXO_TALLY_N];
+  /* Proper cells that ComputeFromConserved repaired since the last
+   * gxo_fallback_counts reset, per call site (0: the stage-2 input Y,
+   * 1: StoreSolution and gxo_set_conserved) and branch (0: D <= N_Min,
+   * vacuum; 1: E < 0). */
+  long n_repaired[2][2];
 };
 
 static inline long IDX(const gxo_ctx *c, int p0, int p1, int p2) {
@@ -283,7 +288,8 @@ static inline void from_primitive_value(const gxo_ctx *c, double **F, long iv) {
 /* Fluid_P_I % ComputeFromConservedCommon (F/Fluid_P_I__Form.f90:604-722) with
  * UseEntropy = .false.: Compute_N_V_E_G_Kernel (F/Fluid_P__Kernel.f90:94-192),
  * Apply_EOS_I_E, Compute_DS_G, Compute_FE_P_G. */
-static inline void from_conserved_value(const gxo_ctx *c, double **F, long iv) {
+static inline int from_conserved_value(const gxo_ctx *c, double **F, long iv) {
+  int repaired = 0;
   F[GXO_M][iv] = c->mass;
   double M = F[GXO_M][iv];
   double D = F[GXO_D][iv];
@@ -297,6 +303,7 @@ static inline void from_conserved_value(const gxo_ctx *c, double **F, long iv) {
                            F[GXO_S_2][iv] * F[GXO_V_2][iv]) +
                           F[GXO_S_3][iv] * F[GXO_V_3][iv]);
   } else {
+    repaired |= 1;
     F[GXO_N][iv] = c->n_min;
     F[GXO_V_1][iv] = 0.0; F[GXO_V_2][iv] = 0.0; F[GXO_V_3][iv] = 0.0;
     F[GXO_E][iv] = 0.0;
@@ -305,6 +312,7 @@ static inline void from_conserved_value(const gxo_ctx *c, double **F, long iv) {
     F[GXO_G][iv] = 0.0;
   }
   if (F[GXO_E][iv] < 0.0) {
+    repaired |= 2;
     F[GXO_E][iv] = 0.0;
     F[GXO_G][iv] = F[GXO_E][iv] +
                    0.5 * ((F[GXO_S_1][iv] * F[GXO_V_1][iv] +
@@ -314,6 +322,7 @@ static inline void from_conserved_value(const gxo_ctx *c, double **F, long iv) {
   eos_e(c, F, iv);
   F[GXO_DS][iv] = F[GXO_SB][iv] * F[GXO_N][iv];
   fe_p_g(F, iv);
+  return repaired;
 }
 
 static void from_primitive_all(const gxo_ctx *c, double **F) {
@@ -322,10 +331,26 @@ static void from_primitive_all(const gxo_ctx *c, double **F) {
   for (long iv = 0; iv < nv; iv++) from_primitive_value(c, F, iv);
 }
 
-static void from_conserved_all(const gxo_ctx *c, double **F) {
-  long nv = c->nv;
-#pragma omp parallel for schedule(static)
-  for (long iv = 0; iv < nv; iv++) from_conserved_value(c, F, iv);
+static inline int is_proper(const gxo_ctx *c, long iv) {
+  const int p0 = (int)(iv % c->m[0]);
+  const int p1 = (int)((iv / c->m[0]) % c->m[1]);
+  const int p2 = (int)(iv / ((long)c->m[0] * c->m[1]));
+  return p0 >= NG && p0 < c->m[0] - NG && p1 >= NG && p1 < c->m[1] - NG &&
+         p2 >= NG && p2 < c->m[2] - NG;
+}
+
+static void from_conserved_all(gxo_ctx *c, double **F, int site) {
+  long nv = c->nv, nvac = 0, neng = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nvac, neng)
+  for (long iv = 0; iv < nv; iv++) {
+    const int r = from_conserved_value(c, F, iv);
+    if (r && is_proper(c, iv)) {
+      nvac += r & 1;
+      neng += (r >> 1) & 1;
+    }
+  }
+  c->n_repaired[site][0] += nvac;
+  c->n_repaired[site][1] += neng;
 }
 
 /* ---- Chart_SLD % ExchangeGhostData on one rank (CH/SingleLevelCharts/
@@ -970,7 +995,7 @@ void gxo_set_conserved(gxo_ctx *c, const double *D, const double *S1,
   memcpy(c->cur[GXO_S_2], S2, b);
   memcpy(c->cur[GXO_S_3], S3, b);
   memcpy(c->cur[GXO_G], G, b);
-  from_conserved_all(c, c->cur);
+  from_conserved_all(c, c->cur, 1);
 }
 
 /* Integrator_C_PS % ComputeTimeStep_C_ASC (EV/Integrators/Integrator_C_PS__
@@ -1038,7 +1063,7 @@ void gxo_step(gxo_ctx *c, double dt) {
   for (int f = 0; f < 6; f++) memcpy(c->y[f], c->sol[f], sizeof(double) * c->nv);
   multiply_add(c, c->y, c->k1, 1.0);
   copy_conserved_from(c, c->y);
-  from_conserved_all(c, c->cur);
+  from_conserved_all(c, c->cur, 0);
   clear6(c, c->k2);
   c->weight_rk = 0.5; /* B(2) */
   increment_divergence(c, c->k2, dt);
@@ -1049,7 +1074,7 @@ void gxo_step(gxo_ctx *c, double dt) {
   multiply_add(c, c->sol, c->k2, 0.5);
   /* StoreSolution with DetectFeatures = .true. */
   copy_conserved_from(c, c->sol);
-  from_conserved_all(c, c->cur);
+  from_conserved_all(c, c->cur, 1);
   detect_features(c);
   /* AccumulateBoundaryTally (Step_RK_C_ASC__Template.f90:533-537;
    * FLD/Current_ASC__Template.f90:203-219; FLD/Tally_C__Form.f90:215-279):
@@ -1092,6 +1117,11 @@ void gxo_compute_tally(gxo_ctx *c, int compute_change, double *interior,
 
 void gxo_get_field(const gxo_ctx *c, int f, double *out) {
   memcpy(out, c->cur[f], sizeof(double) * c->nv);
+}
+
+void gxo_fallback_counts(gxo_ctx *c, long out[4], int reset) {
+  for (int k = 0; k < 4; k++) out[k] = c->n_repaired[k / 2][k % 2];
+  if (reset) memset(c->n_repaired, 0, sizeof(c->n_repaired));
 }
 
 void gxo_get_flags(const gxo_ctx *c, int d, double *out) {

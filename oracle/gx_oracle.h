@@ -100,6 +100,13 @@ void gxo_get_field(const gxo_ctx *c, int f, double *out);
  * step, conserved field f (D, S_1..3, G, DS order), all nv values. */
 void gxo_get_sources(const gxo_ctx *c, int f, double *out);
 
+/* Proper cells that ComputeFromConserved (F/Fluid_P__Kernel.f90:94-192)
+ * repaired since the last reset: out[0], out[1] in the stage-2 input Y,
+ * out[2], out[3] in StoreSolution (and gxo_set_conserved); out[0], out[2]
+ * with D <= N_Min (set to vacuum), out[1], out[3] with E < 0 (E = 0, G
+ * rebuilt).  reset != 0 clears the counts. */
+void gxo_fallback_counts(gxo_ctx *c, long out[4], int reset);
+
 /* Copy DiffusiveFlux_I_d (d = 0,1,2) out, padded layout. */
 void gxo_get_flags(const gxo_ctx *c, int d, double *out);
 

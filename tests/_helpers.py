@@ -54,3 +54,19 @@ def evolve(stepper, n_cycles, finish_time=1.0e30, n_write=1):
         FinishTime=finish_time, FinishCycle=n_cycles, nWrite=n_write))
     it.evolve()
     return it
+
+
+def cold_rarefaction(n=(24, 6, 6), v=2.0, p=1.0e-6):
+    """A cold double rarefaction (N = 1, P = 1e-6, v_x = -v left of x = 0.5
+    and +v right of it, a small transverse wobble v_y) on an outflow chart:
+    with BaryonDensityMin = 0.05 its centre is repaired in both branches of
+    ComputeFromConserved, at the stage-2 input and at StoreSolution, within
+    a dozen full CFL steps."""
+    chart = Chart(n, bc=(("OUTFLOW", "OUTFLOW"),) * 3)
+    x, y, z = chart.padded_centers()
+    shape = chart.m[::-1]
+    N = np.ones(shape)
+    E = np.full(shape, p / 0.4)
+    V1 = np.where(x < 0.5, -v, v)
+    V2 = 0.05 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * z)
+    return chart, (N, V1, V2, np.zeros(shape), E)

@@ -99,3 +99,61 @@ def test_batch_matches_single_charts(precision):
     t, _, _ = batch.times()
     assert (t == finish).any() and (t < finish).any(), t
     assert cyc.max() == sum(n for n, _ in plan)
+
+
+def test_ensemble_size_classes_vs_oracle():
+    """The ensemble workload at its size classes: eight cubic Riemann problems
+    of riemann_ensemble_problem with edges 32 .. 128 (five size classes),
+    autotuned, then advanced by graph replays.  After 5 cycles every member
+    equals the oracle driven by the host integrator (Integrator_C_PS %
+    ComputeCycle_ASC, Integrator_C_PS__Form.f90:328-371) bit for bit --
+    state, flags, Time -- and after 50 cycles it equals a single-chart
+    context driven the same way."""
+    import oracle
+    from ._helpers import OracleStepper
+
+    edges = [32, 40, 50, 60, 70, 90, 100, 128]
+    probs = [problems.riemann_ensemble_problem(k, n, seed=77, padded=True)
+             for k, n in enumerate(edges)]
+    classes = {int(np.floor(2 * np.log2(n))) for n in edges}
+    assert classes >= {10, 11, 12, 13}, classes
+    params = FluidParameters(CourantFactor=0.8)
+    charts = [c for c, _, _ in probs]
+    batch = Batch_RK2_C_ASC(charts, params)
+    for k, (chart, prim, _) in enumerate(probs):
+        batch.set_primitive(k, *prim)
+    finish = 1.0
+    batch.set_integrator(FinishTime=finish)
+    tuned = batch.autotune(n_repeat=1)
+    assert set(tuned) == classes, tuned
+    batch.step(5)
+    batch.sync()
+    t, _, cyc = batch.times()
+    assert (cyc == 5).all()
+    for k, (chart, prim, _) in enumerate(probs):
+        o = oracle.OracleFluid(chart, courant=0.8)
+        o.set_primitive(*prim)
+        it = Integrator_C_PS(OracleStepper(o), IntegratorParameters(FinishTime=finish, nWrite=1))
+        it._administer_checkpoint()
+        for _ in range(5):
+            it.compute_cycle()
+        assert t[k] == it.s.Time, (k, t[k], it.s.Time)
+        uo = np.stack([chart.proper(a) for a in o.conserved()])
+        ub = batch.conserved(k)
+        assert same_bits(ub, uo), (k, edges[k], int((ub != uo).sum()))
+        fo = np.zeros(chart.n[::-1], np.uint8)
+        for d in range(3):
+            fo |= (chart.proper(o.flags(d)) > 0).astype(np.uint8) << d
+        assert np.array_equal(batch.flags(k), fo), k
+        o.close()
+    singles = [_Single(c, p, params, finish) for c, p, _ in probs]
+    for ref in singles:
+        ref.cycles(5)
+    # three more replays of one captured 15-cycle graph, with host work between
+    for i in range(3):
+        batch.step(15)
+        batch.sync()
+        for ref in singles:
+            ref.cycles(15)
+        cyc = _compare(batch, singles, f"replay {i}")
+    assert (cyc == 50).all()
