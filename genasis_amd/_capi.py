@@ -43,6 +43,7 @@ GX_TALLY_N = len(TALLY_NAMES)
 EXPORTS = (
     "gx_default_params", "gx_create", "gx_destroy", "gx_upload", "gx_download",
     "gx_upload_features", "gx_download_features", "gx_compute_time_step",
+    "gx_compute_time_step_noalloc",
     "gx_step_rk2", "gx_sync", "gx_totals", "gx_set_state", "gx_set_primitive",
     "gx_get_conserved", "gx_get_primitive", "gx_get_flags", "gx_set_flags",
     "gx_set_tuning", "gx_get_tuning", "gx_set_record_sources", "gx_get_sources",
@@ -111,6 +112,7 @@ def load(path: str = LIB_PATH):
         "gx_upload_features": (i, [vp, pd, ll]),
         "gx_download_features": (i, [vp, pd, ll]),
         "gx_compute_time_step": (i, [vp, pd]),
+        "gx_compute_time_step_noalloc": (i, [P(GxGrid), P(GxParams), vp, vp, vp]),
         "gx_step_rk2": (i, [vp, d, d]),
         "gx_sync": (i, [vp]),
         "gx_totals": (i, [vp, pd]),

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -72,12 +73,62 @@ __global__ __launch_bounds__(256) void division_selftest_kernel(
 }  // namespace gx
 
 // ---------------------------------------------------------------------------
+// Scratch of the allocation-free time-step candidate
+// (gx_compute_time_step_noalloc): the CFL accumulator in the first 256 bytes,
+// then a chart geometry image of 3 x N_GEO rows of NOALLOC_GS values (only
+// the WidthLeft + WidthRight rows are written).  1.18 MB per GPU, held until
+// the process ends; every call on a GPU shares it.
+// ---------------------------------------------------------------------------
+namespace gx {
+constexpr int NOALLOC_MAX_EDGE = 4096;
+constexpr int NOALLOC_GS = NOALLOC_MAX_EDGE + 4;
+constexpr size_t NOALLOC_GEO_OFFSET = 32;  // doubles
+constexpr size_t NOALLOC_DOUBLES = NOALLOC_GEO_OFFSET + 3 * (size_t)N_GEO * NOALLOC_GS;
+}  // namespace gx
+
+static __device__ __align__(256) double scratch[gx::NOALLOC_DOUBLES];
+
+namespace gx {
+
+// Clears the accumulator and writes WidthLeft + WidthRight of the proper
+// cells of each dimension (one thread per dimension), by the construction
+// of geometry_1d: edges by repeated addition of (Max - Min)/n, centre
+// 0.5 (E(i) + E(i+1)), widths centre - E(i) and E(i+1) - centre.
+template <typename R>
+__global__ __launch_bounds__(64) void noalloc_setup_kernel(gx_grid grid, double *scr) {
+  const int d = threadIdx.x;
+  if (d == 0) *(unsigned long long *)scr = 0ull;
+  if (d >= 3) return;
+  R *geo = (R *)(scr + NOALLOC_GEO_OFFSET) + (size_t)(d * N_GEO + G_WIDTH) * NOALLOC_GS;
+  const int n = grid.nCells[d];
+  const double width = (grid.MaxCoordinate[d] - grid.MinCoordinate[d]) / n;
+  double e0 = grid.MinCoordinate[d];
+  for (int i = 0; i < n; i++) {
+    const double e1 = e0 + width;
+    const double c = 0.5 * (e0 + e1);
+    geo[i + 2] = (R)((c - e0) + (e1 - c));
+    e0 = e1;
+  }
+}
+
+// ComputeTimeStep_C_ASC's tail on the device: CourantFactor * min(huge,
+// 1/max(tiny, invdt)) from the accumulated maximum.
+__global__ void noalloc_candidate_kernel(const double *scr, double courant, double *candidate) {
+  const double invdt = __longlong_as_double(*(const long long *)scr);
+  const double tsi = fmax(DBL_MIN, invdt);
+  const double ts = fmin(DBL_MAX, 1.0 / tsi);
+  *candidate = courant * ts;
+}
+
+}  // namespace gx
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 
 using namespace gx;
 
-thread_local std::string gx::g_last_error;
+thread_local char gx::g_last_error[gx::ERROR_CAP];
 
 
 // The stage kernel and the boundary-line kernel of the context's
@@ -122,7 +173,7 @@ void gx_default_params(gx_params *p) {
   p->ReconstructionType = GX_LINEAR;
 }
 
-const char *gx_last_error(void) { return g_last_error.c_str(); }
+const char *gx_last_error(void) { return g_last_error; }
 
 int gx_self_test_division(long long n, unsigned long long seed,
                           long long *mismatches) {
@@ -572,6 +623,83 @@ int gx_compute_time_step(gx_ctx *c, double *candidate) {
   *candidate = c->params.CourantFactor * ts;
   return GX_OK;
   GX_API_END("gx_compute_time_step")
+}
+
+// The address of this GPU's scratch (hipGetSymbolAddress once per device,
+// with the device current); the array's name is not a device address on the
+// host.
+static int noalloc_scratch(double **out) {
+  static std::mutex mu;
+  static double *ptr[64] = {};
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    return fail(GX_E_HIP, "gx_compute_time_step_noalloc: no current HIP device");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(GX_E_HIP, "gx_compute_time_step_noalloc: hipGetSymbolAddress of the "
+                            "scratch failed");
+    ptr[dev] = (double *)p;
+  }
+  *out = ptr[dev];
+  return GX_OK;
+}
+
+int gx_compute_time_step_noalloc(const gx_grid *grid, const gx_params *params,
+                                 const void *conserved, double *candidate, void *stream) {
+  GX_API_BEGIN
+  if (!grid || !params || !conserved || !candidate)
+    return fail(GX_E_ARG, "gx_compute_time_step_noalloc: null argument");
+  if (params->Precision != GX_FP64 && params->Precision != GX_FP32)
+    return fail(GX_E_ARG, "gx_compute_time_step_noalloc: Precision must be GX_FP64 or GX_FP32");
+  long long ncell = 1;
+  for (int d = 0; d < 3; d++) {
+    if (grid->nCells[d] < 1 || grid->nCells[d] > NOALLOC_MAX_EDGE)
+      return fail(GX_E_ARG, "gx_compute_time_step_noalloc: nCells must lie in [1, 4096] "
+                            "in every dimension (the size of its static scratch)");
+    if (!(grid->MaxCoordinate[d] > grid->MinCoordinate[d]))
+      return fail(GX_E_ARG, "gx_compute_time_step_noalloc: MaxCoordinate must exceed "
+                            "MinCoordinate");
+    ncell *= grid->nCells[d];
+  }
+  if (ncell >= (1ll << 29))
+    return fail(GX_E_ARG, "gx_compute_time_step_noalloc: at most 2^29 - 1 cells");
+  double *scr = nullptr;
+  int rc = noalloc_scratch(&scr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const bool fp32 = params->Precision == GX_FP32;
+  by_precision(fp32, [&](auto tag) {
+    using R = decltype(tag);
+    StageArgs<R> a;
+    memset(&a, 0, sizeof(a));
+    for (int d = 0; d < 3; d++) a.n[d] = grid->nCells[d];
+    for (int i = 0; i < 6; i++) a.bc[i] = grid->BoundaryCondition[i];
+    a.gs = NOALLOC_GS;
+    a.fs = ncell;
+    a.ph.gamma = params->AdiabaticIndex;
+    a.ph.mass = params->BaryonMassReference;
+    a.ph.nmin = params->BaryonDensityMin;
+    a.src = (const R *)conserved;
+    a.src_is_prim = 0;
+    a.geo = (const R *)(scr + NOALLOC_GEO_OFFSET);
+    a.invdt_bits = (unsigned long long *)scr;
+    noalloc_setup_kernel<R><<<1, 64, 0, st>>>(*grid, scr);
+    cfl_kernel<R><<<grid_blocks(ncell), 256, 0, st>>>(a);
+    return 0;
+  });
+  noalloc_candidate_kernel<<<1, 1, 0, st>>>(scr, params->CourantFactor, candidate);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    char msg[256];
+    snprintf(msg, sizeof(msg), "gx_compute_time_step_noalloc: launch failed: %s",
+             hipGetErrorString(e));
+    return fail(GX_E_HIP, msg);
+  }
+  return GX_OK;
+  GX_API_END("gx_compute_time_step_noalloc")
 }
 
 int gx_step_rk2(gx_ctx *c, double time, double dt) {

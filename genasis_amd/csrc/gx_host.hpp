@@ -16,15 +16,15 @@
 
 namespace gx {
 
-// Message of the last failed call, per host thread (gx_last_error).
-extern thread_local std::string g_last_error;
+// Message of the last failed call, per host thread (gx_last_error): a fixed
+// buffer, so recording an error allocates nothing.
+constexpr size_t ERROR_CAP = 512;
+extern thread_local char g_last_error[ERROR_CAP];
 
 inline int fail(int code, const char *msg) noexcept {
-  try {
-    g_last_error = msg;
-  } catch (...) {
-    g_last_error.clear();
-  }
+  size_t i = 0;
+  for (; msg && msg[i] && i + 1 < ERROR_CAP; i++) g_last_error[i] = msg[i];
+  g_last_error[i] = '\0';
   return code;
 }
 

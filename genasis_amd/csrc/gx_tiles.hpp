@@ -750,7 +750,9 @@ __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel_par
 }
 
 // Max CFL term over the current state (the first step after an upload, or
-// whenever the stage-2 value is not available), in FP64.
+// whenever the stage-2 value is not available), in FP64: the stored values
+// (of either precision) are widened and converted to primitives in double,
+// as the stage-2 epilogue does with the values it stores.
 template <typename R>
 __global__ __launch_bounds__(256) void cfl_kernel(StageArgs<R> a) {
   __shared__ double sRed[256 / WAVE];
@@ -761,11 +763,18 @@ __global__ __launch_bounds__(256) void cfl_kernel(StageArgs<R> a) {
     const int x = (int)(i % a.n[0]);
     const int y = (int)((i / a.n[0]) % a.n[1]);
     const int z = (int)(i / ((long long)a.n[0] * a.n[1]));
-    R Vr[5];
-    load_prim(a, x, y, z, Vr);
-    double V[5];
+    R raw[5];
+    int flip;
+    load_raw(a, x, y, z, raw, &flip);  // a proper cell: no ghost rule
+    double U[5], V[5];
 #pragma unroll
-    for (int f = 0; f < 5; f++) V[f] = (double)Vr[f];
+    for (int f = 0; f < 5; f++) U[f] = (double)raw[f];
+    if (a.src_is_prim) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) V[f] = U[f];
+    } else {
+      prim_from_cons<double>(U, a.ph.mass, a.ph.nmin, V);
+    }
     const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, (double)geo(a, 0, G_WIDTH, x + 2),
                                (double)geo(a, 1, G_WIDTH, y + 2),
                                (double)geo(a, 2, G_WIDTH, z + 2));

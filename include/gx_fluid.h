@@ -168,6 +168,23 @@ int gx_download_features(gx_ctx *ctx, double *df, long long ldValue);
  * sum_d max(FEP_d, -FEM_d)/(WidthLeft_d + WidthRight_d).  Synchronises. */
 int gx_compute_time_step(gx_ctx *ctx, double *candidate);
 
+/* gx_compute_time_step without a context and without allocating.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).
+ * conserved: device pointer to the five proper-cell fields D, S_1..3, G
+ * (x fastest), double for Precision GX_FP64, float for GX_FP32; candidate:
+ * device pointer to one double that receives CourantFactor * min(huge,
+ * 1/max(tiny, invdt)) as gx_compute_time_step computes it; stream: a
+ * hipStream_t (NULL: the null stream).  Stream-ordered and asynchronous:
+ * nothing is allocated and nothing synchronises, so the call can be captured
+ * in a HIP graph.  Its scratch (the CFL maximum and the chart's cell widths,
+ * 1.18 MB) is a static device array per GPU, held until the process ends:
+ * at most 4096 cells per dimension and 2^29 - 1 cells in all (GX_E_ARG
+ * beyond).  Every call on a GPU shares that array: calls that overlap on
+ * one GPU (other streams, other host threads) race on it. */
+int gx_compute_time_step_noalloc(const gx_grid *grid, const gx_params *params,
+                                 const void *conserved, double *candidate, void *stream);
+
 /* One Heun RK2 step of size dt (stream-ordered, asynchronous).  time is the
  * reference's Time argument and does not enter the arithmetic. */
 int gx_step_rk2(gx_ctx *ctx, double time, double dt);
