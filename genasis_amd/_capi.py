@@ -41,7 +41,7 @@ GX_TALLY_N = len(TALLY_NAMES)
 
 # Every symbol include/gx_fluid.h declares.
 EXPORTS = (
-    "gx_default_params", "gx_create", "gx_destroy", "gx_upload", "gx_download",
+    "gx_default_params", "gx_create", "gx_destroy", "gx_create_bricks", "gx_get_bricks", "gx_upload", "gx_download",
     "gx_upload_features", "gx_download_features", "gx_compute_time_step",
     "gx_compute_time_step_noalloc",
     "gx_step_rk2", "gx_sync", "gx_totals", "gx_set_state", "gx_set_primitive",
@@ -107,6 +107,8 @@ def load(path: str = LIB_PATH):
         "gx_default_params": (None, [P(GxParams)]),
         "gx_create": (i, [P(GxGrid), P(GxParams), P(vp)]),
         "gx_destroy": (i, [vp]),
+        "gx_create_bricks": (i, [P(GxGrid), P(GxParams), P(ctypes.c_int), P(vp)]),
+        "gx_get_bricks": (i, [vp, P(ctypes.c_int)]),
         "gx_upload": (i, [vp, pd, ll, P(ctypes.c_int)]),
         "gx_download": (i, [vp, pd, ll, P(ctypes.c_int)]),
         "gx_upload_features": (i, [vp, pd, ll]),

@@ -13,7 +13,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRCS = [os.path.join(HERE, "csrc", f) for f in ("gx_fluid.hip", "gx_batch.hip",
-                                                        "gx_adjoint.hip", "gx_checkpoint.hip")]
+                                                        "gx_adjoint.hip", "gx_checkpoint.hip",
+                                                        "gx_bricks.hip")]
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libgx_fluid.so")
 OBJ_DIR = os.path.join(ROOT, "_build")

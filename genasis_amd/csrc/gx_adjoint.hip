@@ -894,6 +894,8 @@ extern "C" {
 
 int gx_adjoint_forward(gx_ctx *c, int nSteps, int checkpointEvery, int nRampCycles) {
   GX_API_BEGIN
+  if (c && c->bricks)
+    return fail(GX_E_STATE, "gx_adjoint_forward: not available on a chart split into bricks");
   if (!c || nSteps < 1 || checkpointEvery < 1 || nRampCycles < 1)
     return fail(GX_E_ARG, "gx_adjoint_forward: bad argument");
   if (!c->uploaded || !c->prims_valid)

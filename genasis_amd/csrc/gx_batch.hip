@@ -530,7 +530,7 @@ int gx_batch_set_primitive(gx_batch *b, int iProblem, const double *primitive) {
   by_precision(b->fp32, [&](auto tag) {
     using R = decltype(tag);
     from_primitive_kernel<R><<<grid_blocks(P.ncell), 256, 0, b->stream>>>(
-        P.ncell, b->params.BaryonMassReference, (R *)((char *)b->P0 + o),
+        P.ncell, P.ncell, b->params.BaryonMassReference, (R *)((char *)b->P0 + o),
         (R *)((char *)b->U + o));
     return 0;
   });

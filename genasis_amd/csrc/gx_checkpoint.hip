@@ -107,6 +107,8 @@ extern "C" {
 int gx_write_checkpoint(gx_ctx *c, const char *path, double time, long long cycle,
                         int iCheckpoint) {
   GX_API_BEGIN
+  if (c && c->bricks)
+    return fail(GX_E_STATE, "gx_write_checkpoint: not available on a chart split into bricks");
   const char *who = "gx_write_checkpoint";
   if (!c || !path) return fail(GX_E_ARG, "gx_write_checkpoint: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_write_checkpoint: no state uploaded");
@@ -185,6 +187,8 @@ int gx_write_checkpoint(gx_ctx *c, const char *path, double time, long long cycl
 int gx_read_checkpoint(gx_ctx *c, const char *path, int mode, double *time,
                        long long *cycle, int *iCheckpoint) {
   GX_API_BEGIN
+  if (c && c->bricks)
+    return fail(GX_E_STATE, "gx_read_checkpoint: not available on a chart split into bricks");
   const char *who = "gx_read_checkpoint";
   if (!c || !path) return fail(GX_E_ARG, "gx_read_checkpoint: null argument");
   if (mode != GX_RESTART_PRIMITIVE && mode != GX_RESTART_EXACT)

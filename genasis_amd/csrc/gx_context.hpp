@@ -15,6 +15,27 @@ namespace gx {
 struct AdjointState;
 // Frees the adjoint state of a context (defined with the adjoint).
 void release_adjoint(gx_ctx *c);
+
+// A chart split into bricks (gx_create_bricks, defined in gx_bricks.hip):
+// the entry points below stand in for the single-brick code paths.
+struct BrickSet;
+void release_bricks(gx_ctx *c);
+// Primitives (and conserved fields, or NULL: from the primitives) of the
+// chart's proper cells into the bricks, with their halos and cleared flags.
+int bricks_set_state(gx_ctx *c, const double *primitive, const double *conserved);
+// Five proper-cell fields in chart order: which = 0 the conserved state,
+// 1 the uploaded primitives.
+int bricks_get(gx_ctx *c, int which, double *out);
+// The 23 Current fields of the proper cells in chart order (host).
+int bricks_current(gx_ctx *c, double *host);
+int bricks_get_flags(gx_ctx *c, unsigned char *flags);
+int bricks_set_flags(gx_ctx *c, const unsigned char *flags);
+// CFL maximum of the current state into c->invdt (stream-ordered).
+int bricks_cfl(gx_ctx *c);
+int bricks_step(gx_ctx *c, double dt);
+int bricks_totals(gx_ctx *c, double out[5]);
+// Interior tally partials of every brick into c->tpartial (stream-ordered).
+int bricks_interior_partials(gx_ctx *c);
 }  // namespace gx
 
 using gx::ChartGeometry;
@@ -62,6 +83,7 @@ struct gx_ctx {
   long long kernel_n[4] = {0, 0, 0, 0};
   ChartGeometry geom;
   gx::AdjointState *adj = nullptr;  // discrete adjoint (gx_adjoint_*), on demand
+  gx::BrickSet *bricks = nullptr;   // a chart split into bricks (gx_create_bricks)
 };
 
 namespace gx {

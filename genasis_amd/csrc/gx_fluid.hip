@@ -199,6 +199,7 @@ int gx_destroy(gx_ctx *c) {
   if (!c) return GX_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   gx::release_adjoint(c);
+  gx::release_bricks(c);
   for (int e = 0; e < 6; e++)
     if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
   (void)hipFree(c->fluence);
@@ -316,6 +317,7 @@ int gx_set_tuning(gx_ctx *c, int zChunk) {
 int gx_set_record_sources(gx_ctx *c, int on) {
   GX_API_BEGIN
   if (!c) return fail(GX_E_ARG, "gx_set_record_sources: null context");
+  if (c->bricks) return fail(GX_E_STATE, "gx_set_record_sources: not available on a chart split into bricks");
   if (on) {
     const size_t bytes = 5 * c->esz * (size_t)c->ncell;
     if (!c->sources) HIP_TRY(hipMalloc(&c->sources, bytes));
@@ -330,6 +332,7 @@ int gx_set_record_sources(gx_ctx *c, int on) {
 int gx_get_sources(gx_ctx *c, double *sources) {
   GX_API_BEGIN
   if (!c || !sources) return fail(GX_E_ARG, "gx_get_sources: null argument");
+  if (c->bricks) return fail(GX_E_STATE, "gx_get_sources: not available on a chart split into bricks");
   if (!c->sources)
     return fail(GX_E_STATE, "gx_get_sources: recording was never enabled (gx_set_record_sources)");
   return get_values(c->stream, c->fp32, sources, c->sources, 5 * (size_t)c->ncell);
@@ -349,6 +352,7 @@ void *gx_stream(gx_ctx *c) { return c ? (void *)c->stream : nullptr; }
 int gx_set_profiling(gx_ctx *c, int on) {
   GX_API_BEGIN
   if (!c) return fail(GX_E_ARG, "gx_set_profiling: null context");
+  if (c->bricks) return fail(GX_E_STATE, "gx_set_profiling: not available on a chart split into bricks");
   if (on && !c->ev[0])
     for (int e = 0; e < 6; e++) HIP_TRY(hipEventCreate(&c->ev[e]));
   c->profiling = on != 0;
@@ -386,13 +390,19 @@ int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
   GX_API_BEGIN
   if (!c || !primitive || !conserved) return fail(GX_E_ARG, "gx_set_state: null argument");
   const size_t n = 5 * (size_t)c->ncell;
-  int rc = put_values(c->stream, c->fp32, c->P0, primitive, n);
-  if (rc) return rc;
-  rc = put_values(c->stream, c->fp32, c->U, conserved, n);
-  if (rc) return rc;
+  int rc;
+  if (c->bricks) {
+    rc = gx::bricks_set_state(c, primitive, conserved);
+    if (rc) return rc;
+  } else {
+    rc = put_values(c->stream, c->fp32, c->P0, primitive, n);
+    if (rc) return rc;
+    rc = put_values(c->stream, c->fp32, c->U, conserved, n);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
+  }
   rc = reset_tallies(c);
   if (rc) return rc;
-  HIP_TRY(hipMemsetAsync(c->flags, 0, (size_t)c->ncell, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->uploaded = true;
   c->prims_valid = true;
@@ -404,12 +414,23 @@ int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
 int gx_set_primitive(gx_ctx *c, const double *primitive) {
   GX_API_BEGIN
   if (!c || !primitive) return fail(GX_E_ARG, "gx_set_primitive: null argument");
+  if (c->bricks) {
+    int rb = gx::bricks_set_state(c, primitive, nullptr);
+    if (rb) return rb;
+    rb = reset_tallies(c);
+    if (rb) return rb;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->uploaded = true;
+    c->prims_valid = true;
+    c->invdt_valid = false;
+    return GX_OK;
+  }
   const int rc = put_values(c->stream, c->fp32, c->P0, primitive, 5 * (size_t)c->ncell);
   if (rc) return rc;
   by_precision(c->fp32, [&](auto tag) {
     using R = decltype(tag);
     from_primitive_kernel<R><<<grid_blocks(c->ncell), 256, 0, c->stream>>>(
-        c->ncell, c->params.BaryonMassReference, (R *)c->P0, (R *)c->U);
+        c->ncell, c->ncell, c->params.BaryonMassReference, (R *)c->P0, (R *)c->U);
     return 0;
   });
   HIP_TRY(hipGetLastError());
@@ -462,9 +483,14 @@ int gx_get_conserved(gx_ctx *c, double *conserved) {
   GX_API_BEGIN
   if (!c || !conserved) return fail(GX_E_ARG, "gx_get_conserved: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_get_conserved: no state uploaded");
+  if (c->bricks) return gx::bricks_get(c, 0, conserved);
   return get_values(c->stream, c->fp32, conserved, c->U, 5 * (size_t)c->ncell);
   GX_API_END("gx_get_conserved")
 }
+
+// The 23 Current fields of the proper cells in chart order into host
+// (GX_N_FIELDS x ncell doubles).
+static int current_to_host(gx_ctx *c, double *host);
 
 static int run_current(gx_ctx *c) {
   if (!c->scratch)
@@ -483,11 +509,31 @@ static int run_current(gx_ctx *c) {
   return GX_OK;
 }
 
+static int current_to_host(gx_ctx *c, double *host) {
+  if (c->bricks) return gx::bricks_current(c, host);
+  int rc = run_current(c);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(host, c->scratch, sizeof(double) * GX_N_FIELDS * (size_t)c->ncell,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GX_OK;
+}
+
 int gx_get_primitive(gx_ctx *c, double *primitive) {
   GX_API_BEGIN
   if (!c || !primitive) return fail(GX_E_ARG, "gx_get_primitive: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_get_primitive: no state uploaded");
   const size_t bytes = sizeof(double) * (size_t)c->ncell;
+  if (c->bricks) {
+    if (c->prims_valid) return gx::bricks_get(c, 1, primitive);
+    std::vector<double> host(GX_N_FIELDS * (size_t)c->ncell);
+    const int rc = current_to_host(c, host.data());
+    if (rc) return rc;
+    for (int f = 0; f < 5; f++)
+      memcpy(primitive + (size_t)f * c->ncell, host.data() + (size_t)PRIM_FIELDS[f] * c->ncell,
+             bytes);
+    return GX_OK;
+  }
   if (c->prims_valid) {
     return get_values(c->stream, c->fp32, primitive, c->P0, 5 * (size_t)c->ncell);
   } else {
@@ -509,12 +555,9 @@ int gx_download(gx_ctx *c, double *value, long long ld, const int *map) {
   if (!c->uploaded) return fail(GX_E_STATE, "gx_download: no state uploaded");
   const int m0 = c->n[0] + 4, m1 = c->n[1] + 4, m2 = c->n[2] + 4;
   if (ld < (long long)m0 * m1 * m2) return fail(GX_E_ARG, "gx_download: ldValue < nValues");
-  int rc = run_current(c);
-  if (rc) return rc;
   std::vector<double> host(GX_N_FIELDS * (size_t)c->ncell);
-  HIP_TRY(hipMemcpyAsync(host.data(), c->scratch, sizeof(double) * host.size(),
-                         hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  int rc = current_to_host(c, host.data());
+  if (rc) return rc;
   const int g = 2;
   const int n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
   for (int f = 0; f < GX_N_FIELDS; f++) {
@@ -545,6 +588,7 @@ int gx_download(gx_ctx *c, double *value, long long ld, const int *map) {
 int gx_get_flags(gx_ctx *c, unsigned char *flags) {
   GX_API_BEGIN
   if (!c || !flags) return fail(GX_E_ARG, "gx_get_flags: null argument");
+  if (c->bricks) return gx::bricks_get_flags(c, flags);
   HIP_TRY(hipMemcpyAsync(flags, c->flags, (size_t)c->ncell, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
@@ -554,6 +598,7 @@ int gx_get_flags(gx_ctx *c, unsigned char *flags) {
 int gx_set_flags(gx_ctx *c, const unsigned char *flags) {
   GX_API_BEGIN
   if (!c || !flags) return fail(GX_E_ARG, "gx_set_flags: null argument");
+  if (c->bricks) return gx::bricks_set_flags(c, flags);
   HIP_TRY(hipMemcpyAsync(c->flags, flags, (size_t)c->ncell, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return GX_OK;
@@ -599,7 +644,10 @@ int gx_compute_time_step(gx_ctx *c, double *candidate) {
   GX_API_BEGIN
   if (!c || !candidate) return fail(GX_E_ARG, "gx_compute_time_step: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_compute_time_step: no state uploaded");
-  if (!c->invdt_valid) {
+  if (!c->invdt_valid && c->bricks) {
+    const int rb = gx::bricks_cfl(c);
+    if (rb) return rb;
+  } else if (!c->invdt_valid) {
     HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), c->stream));
     by_precision(c->fp32, [&](auto tag) {
       using R = decltype(tag);
@@ -707,6 +755,7 @@ int gx_step_rk2(gx_ctx *c, double time, double dt) {
   (void)time;
   if (!c) return fail(GX_E_ARG, "gx_step_rk2: null context");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_step_rk2: no state uploaded");
+  if (c->bricks) return gx::bricks_step(c, dt);
   return gx::step_rk2(c, dt, nullptr);
   GX_API_END("gx_step_rk2")
 }
@@ -840,15 +889,20 @@ int gx_compute_tally(gx_ctx *c, int computeChange, double *interior, double *bou
   GX_API_BEGIN
   if (!c) return fail(GX_E_ARG, "gx_compute_tally: null context");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_compute_tally: no state uploaded");
-  by_precision(c->fp32, [&](auto tag) {
-    using R = decltype(tag);
-    StageArgs<R> a = base_args<R>(c);
-    a.aux = (const R *)c->U;
-    interior_tally_kernel<R><<<c->n_partial, TALLY_BLOCK, 0, c->stream>>>(
-        a, c->prims_valid ? (const R *)c->P0 : nullptr, c->tgeo, c->tpartial);
-    return 0;
-  });
-  HIP_TRY(hipGetLastError());
+  if (c->bricks) {
+    const int rb = gx::bricks_interior_partials(c);
+    if (rb) return rb;
+  } else {
+    by_precision(c->fp32, [&](auto tag) {
+      using R = decltype(tag);
+      StageArgs<R> a = base_args<R>(c);
+      a.aux = (const R *)c->U;
+      interior_tally_kernel<R><<<c->n_partial, TALLY_BLOCK, 0, c->stream>>>(
+          a, c->prims_valid ? (const R *)c->P0 : nullptr, c->tgeo, c->tpartial);
+      return 0;
+    });
+    HIP_TRY(hipGetLastError());
+  }
   std::vector<double> h((size_t)NI_SUM * c->n_partial);
   double local[TALLY_N];
   HIP_TRY(hipMemcpyAsync(h.data(), c->tpartial, sizeof(double) * h.size(),
@@ -883,6 +937,7 @@ int gx_totals(gx_ctx *c, double out[5]) {
   GX_API_BEGIN
   if (!c || !out) return fail(GX_E_ARG, "gx_totals: null argument");
   if (!c->uploaded) return fail(GX_E_STATE, "gx_totals: no state uploaded");
+  if (c->bricks) return gx::bricks_totals(c, out);
   by_precision(c->fp32, [&](auto tag) {
     using R = decltype(tag);
     StageArgs<R> a = base_args<R>(c);

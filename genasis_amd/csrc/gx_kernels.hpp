@@ -17,6 +17,10 @@ namespace gx {
 constexpr int BC_PERIODIC = 0;
 constexpr int BC_OUTFLOW = 1;
 constexpr int BC_REFLECTING = 2;
+// A brick face shared with another brick of the chart (or with the chart's
+// other end across a periodic face split into bricks): its ghost values are
+// the halo the brick exchange fills (Chart_SLD % ExchangeGhostData).
+constexpr int BC_NEIGHBOR = 3;
 
 // Per-index 1-D geometry of one dimension, padded index p = c + 2 for the
 // interior-based cell index c in [-2, n+2).

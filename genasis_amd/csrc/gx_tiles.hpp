@@ -57,6 +57,14 @@ struct StageArgs {
   unsigned long long *invdt_bits;  // stage 2: max CFL term (as FP64 bits)
   R *sources;           // Current % Sources of the divergence, if recorded
   double rdt;           // RN(1/dt) for the recording, 0: divide
+  // Bricks only.  hoff[b]: first value of the halo block of face b (bc
+  // BC_NEIGHBOR) in every field slab; fhoff[d]: first flag of the ghost layer
+  // beyond the outer face of dimension d when that face is inside the chart
+  // (-1: none, the flags there are 0); chart_face: bit b set when face b is
+  // a face of the chart (ClearBoundary clears there, periodic or not).
+  int hoff[6];
+  int fhoff[3];
+  int chart_face;
 };
 
 // Waves per SIMD the stage kernel is compiled for: the FP32 tile uses half
@@ -102,16 +110,36 @@ __device__ __forceinline__ int map_coord(int c, int n, int bc_in, int bc_out,
   return c;
 }
 
+// Halo value of the ghost cell c of brick face b = 2d + s (BC_NEIGHBOR): a
+// block of two layers (layer 0 next to the brick) by the two transverse
+// extents with two ghost cells on each side (edges included), the layer
+// fastest, then the lower, then the higher transverse dimension.
+template <typename A>
+__device__ __forceinline__ unsigned halo_index(const A &a, int b, int cx, int cy, int cz) {
+  const int d = b >> 1;
+  const int cd = d == 0 ? cx : (d == 1 ? cy : cz);
+  const int cj = d == 0 ? cy : cx, ck = d == 2 ? cy : cz;
+  const int nj = d == 0 ? a.n[1] : a.n[0];
+  const int layer = (b & 1) ? cd - a.n[d] : -1 - cd;
+  return (unsigned)a.hoff[b] + (unsigned)layer +
+         2u * ((unsigned)(cj + 2) + (unsigned)(nj + 4) * (unsigned)(ck + 2));
+}
+
 // Linear index of cell (x, y, z) with at most one coordinate outside the
 // chart by at most two cells, after the ghost rule of that face; *flip = the
-// dimension whose velocity a reflecting face negates, or -1.
-template <typename A>
+// dimension whose velocity a reflecting face negates, or -1.  A brick reads
+// the ghosts of its BC_NEIGHBOR faces from their halos.
+template <bool BRICK = false, typename A>
 __device__ __forceinline__ unsigned cell_index(const A &a, int x, int y, int z,
                                                int *flip) {
   *flip = -1;
   if ((unsigned)x < (unsigned)a.n[0] && (unsigned)y < (unsigned)a.n[1] &&
       (unsigned)z < (unsigned)a.n[2])
     return lin(a.n, x, y, z);
+  if constexpr (BRICK) {
+    const int b = x < 0 ? 0 : (x >= a.n[0] ? 1 : (y < 0 ? 2 : (y >= a.n[1] ? 3 : (z < 0 ? 4 : 5))));
+    if (a.bc[b] == BC_NEIGHBOR) return halo_index(a, b, x, y, z);
+  }
   bool fx = false, fy = false, fz = false;
   const int cx = map_coord(x, a.n[0], a.bc[0], a.bc[1], &fx);
   const int cy = map_coord(y, a.n[1], a.bc[2], a.bc[3], &fy);
@@ -121,10 +149,10 @@ __device__ __forceinline__ unsigned cell_index(const A &a, int x, int y, int z,
 }
 
 // Raw stage input (conserved, or primitive after an upload) of one cell.
-template <typename R>
+template <bool BRICK = false, typename R>
 __device__ __forceinline__ void load_raw(const StageArgs<R> &a, int x, int y, int z,
                                          R raw[5], int *flip) {
-  const unsigned i = cell_index(a, x, y, z, flip);
+  const unsigned i = cell_index<BRICK>(a, x, y, z, flip);
 #pragma unroll
   for (int f = 0; f < 5; f++) raw[f] = ld(a.src + f * a.fs, i);
 }
@@ -154,12 +182,12 @@ __device__ __forceinline__ void raw_to_prim(const StageArgs<R> &a, const R raw[5
   V[3] = negate_if(V[3], flip == 2);
 }
 
-template <typename R>
+template <bool BRICK = false, typename R>
 __device__ __forceinline__ void load_prim(const StageArgs<R> &a, int x, int y,
                                           int z, R V[5]) {
   R raw[5];
   int flip;
-  load_raw(a, x, y, z, raw, &flip);
+  load_raw<BRICK>(a, x, y, z, raw, &flip);
   raw_to_prim(a, raw, flip, V);
 }
 
@@ -320,8 +348,11 @@ template <typename R> struct EdgeShared {
   R sEy[5][TX];
 };
 
-// REC: record the divergence sources (a.sources).
-template <typename R, int STAGE, int RECON = RECON_LINEAR, bool REC = false>
+// REC: record the divergence sources (a.sources).  BRICK: a brick of a
+// chart split into bricks (ghosts of BC_NEIGHBOR faces from the halos, and
+// the flags of the faces beyond the brick from the flag halo).
+template <typename R, int STAGE, int RECON = RECON_LINEAR, bool REC = false,
+          bool BRICK = false>
 __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0, int z0,
                                            StageShared<R> &L, EdgeShared<R> *E = nullptr) {
   constexpr bool PARA = RECON == RECON_PARABOLIC;
@@ -364,11 +395,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
 
   if (active) {
     R Vm2[5], Vm1[5], sm1[5], WL[5], WR[5];
-    load_prim(a, x, y, z0 - 2, Vm2);
-    load_prim(a, x, y, z0 - 1, Vm1);
-    load_prim(a, x, y, z0, Vk);
-    load_prim(a, x, y, z0 + 1, Vk1);
-    load_raw(a, x, y, z0 + 2, zr, &zflip);
+    load_prim<BRICK>(a, x, y, z0 - 2, Vm2);
+    load_prim<BRICK>(a, x, y, z0 - 1, Vm1);
+    load_prim<BRICK>(a, x, y, z0, Vk);
+    load_prim<BRICK>(a, x, y, z0 + 1, Vk1);
+    load_raw<BRICK>(a, x, y, z0 + 2, zr, &zflip);
     flk = ld(a.flags, lin(a.n, x, y, z0));
     if constexpr (PARA) {
       // sk holds the outer-face value of plane k
@@ -409,7 +440,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
     int hx, hy;
     halo_cell(hid, x0, y0, tw, th, &hx, &hy);
     hpos = (hy - y0 + 2) * HX + (hx - x0 + 2);
-    load_raw(a, hx, hy, z0, hr, &hflip);
+    load_raw<BRICK>(a, hx, hy, z0, hr, &hflip);
     if constexpr (LATE) raw_to_prim(a, hr, hflip, hv);
   }
 
@@ -430,20 +461,31 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
       if (more) {
         int hx, hy;
         halo_cell(hid, x0, y0, tw, th, &hx, &hy);
-        load_raw(a, hx, hy, k + 1, hr, &hflip);
+        load_raw<BRICK>(a, hx, hy, k + 1, hr, &hflip);
       }
     }
     if (active) {
       const unsigned ic = lin(a.n, x, y, k);
 #pragma unroll
       for (int f = 0; f < 5; f++) ar[f] = ld(a.aux + f * a.fs, ic);
-      if (more) load_raw(a, x, y, k + 3, zn, &znflip);
-      if (k + 1 < nz) fln = ld(a.flags, lin(a.n, x, y, k + 1));
+      if (more) load_raw<BRICK>(a, x, y, k + 3, zn, &znflip);
+      if (k + 1 < nz)
+        fln = ld(a.flags, lin(a.n, x, y, k + 1));
+      else if (BRICK && a.fhoff[2] >= 0)
+        fln = ld(a.flags, (unsigned)a.fhoff[2] + (unsigned)x + (unsigned)nx * (unsigned)y);
     }
     if (tid < th) {
-      if (x0 + tw < nx) flx = ld(a.flags, lin(a.n, x0 + tw, y0 + tid, k)) & 1u;
+      if (x0 + tw < nx)
+        flx = ld(a.flags, lin(a.n, x0 + tw, y0 + tid, k)) & 1u;
+      else if (BRICK && a.fhoff[0] >= 0)
+        flx = ld(a.flags, (unsigned)a.fhoff[0] + (unsigned)(y0 + tid) + (unsigned)ny * (unsigned)k) &
+              1u;
     } else if (tid >= WAVE && tid - WAVE < tw) {
-      if (y0 + th < ny) flx = (ld(a.flags, lin(a.n, x0 + tid - WAVE, y0 + th, k)) >> 1) & 1u;
+      if (y0 + th < ny)
+        flx = (ld(a.flags, lin(a.n, x0 + tid - WAVE, y0 + th, k)) >> 1) & 1u;
+      else if (BRICK && a.fhoff[1] >= 0)
+        flx = (ld(a.flags, (unsigned)a.fhoff[1] + (unsigned)(x0 + tid - WAVE) +
+                               (unsigned)nx * (unsigned)k) >> 1) & 1u;
     }
     __syncthreads();
 
@@ -727,14 +769,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
   }
 }
 
-template <typename R, int STAGE, bool REC = false>
+template <typename R, int STAGE, bool REC = false, bool BRICK = false>
 __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel(
     StageArgs<R> a) {
   __shared__ StageShared<R> L;
   const int tile = tile_of_block(blockIdx.x, a.nblk);
-  stage_tile<R, STAGE, RECON_LINEAR, REC>(a, (tile % a.nbx) * TX,
-                                          ((tile / a.nbx) % a.nby) * TY,
-                                          (tile / (a.nbx * a.nby)) * a.tz, L);
+  stage_tile<R, STAGE, RECON_LINEAR, REC, BRICK>(a, (tile % a.nbx) * TX,
+                                                 ((tile / a.nbx) % a.nby) * TY,
+                                                 (tile / (a.nbx * a.nby)) * a.tz, L);
 }
 
 // The same stage with PARABOLIC reconstruction.
@@ -834,7 +876,36 @@ template <typename R> struct FlagShared {
   uint8_t sB[3][BH * BW];  // tests of plane p at (p + 3) % 3
 };
 
+// Bricks: whether the data of cell (x, y, z) is at hand -- a proper cell, or
+// a ghost (up to two cells out, edges and corners included) of BC_NEIGHBOR
+// faces only -- and its index (the halo of the first such face, whose block
+// covers the other ghost coordinates).  Ghosts of the other faces (outflow,
+// reflecting, or periodic within the brick) are not: every flag they could
+// set is cleared at that face of the chart.
 template <typename R>
+__device__ __forceinline__ bool brick_cell(const StageArgs<R> &a, int x, int y, int z,
+                                           unsigned *idx) {
+  const int c[3] = {x, y, z};
+  int face = -1;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (c[d] < 0) {
+      if (c[d] < -2 || a.bc[2 * d] != BC_NEIGHBOR) return false;
+      if (face < 0) face = 2 * d;
+    } else if (c[d] >= a.n[d]) {
+      if (c[d] > a.n[d] + 1 || a.bc[2 * d + 1] != BC_NEIGHBOR) return false;
+      if (face < 0) face = 2 * d + 1;
+    }
+  }
+  *idx = face < 0 ? lin(a.n, x, y, z) : halo_index(a, face, x, y, z);
+  return true;
+}
+
+// BRICK: the tests also run where the lower neighbour is a halo cell, and
+// ClearBoundary clears only at the faces of the chart (FluidFeatures_P__
+// Form.f90:333-338: iaBrick == 1 or nBricks, periodic or not), so the flags
+// of a brick's first and last layers survive at faces between bricks.
+template <typename R, bool BRICK = false>
 __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, int y0,
                                                  int z0, uint8_t *flags, FlagShared<R> &L) {
   auto &sQ = L.sQ;
@@ -846,19 +917,31 @@ __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, 
   const R mass = R(a.ph.mass);
   const double nmin = a.ph.nmin;
   const R sqrt_tiny = Arith<R>::sqrt_tiny;
+  auto avail = [&](int x, int y, int z) {
+    unsigned i;
+    return brick_cell(a, x, y, z, &i);
+  };
 
   // Each thread converts the plane-tile cells t = tid + j NT (j < QJ); their
   // raw values for plane k + 1 are loaded while plane k's tests run.
   constexpr int QJ = (QH * QW + NT - 1) / NT;
   R Un[QJ][5];
+  bool Uav[QJ];
   auto load_plane = [&](int k) {
 #pragma unroll
     for (int j = 0; j < QJ; j++) {
       const int t = tid + j * NT;
       const int x = x0 - 2 + t % QW, y = y0 - 2 + t / QW;
-      const bool in = t < QH * QW && k >= 0 && k < nz && (unsigned)x < (unsigned)nx &&
-                      (unsigned)y < (unsigned)ny;
-      const unsigned i = in ? lin(a.n, x, y, k) : 0u;
+      unsigned i = 0u;
+      bool in;
+      if constexpr (BRICK) {
+        in = t < QH * QW && brick_cell(a, x, y, k, &i);
+      } else {
+        in = t < QH * QW && k >= 0 && k < nz && (unsigned)x < (unsigned)nx &&
+             (unsigned)y < (unsigned)ny;
+        i = in ? lin(a.n, x, y, k) : 0u;
+      }
+      Uav[j] = in;
 #pragma unroll
       for (int f = 0; f < 5; f++) Un[j][f] = in ? ld(a.src + f * a.fs, i) : R(0);
     }
@@ -867,16 +950,14 @@ __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, 
   for (int k = z0 - 2; k <= zend; ++k) {
     R(*Q)[QH * QW] = sQ[(k + 2) & 1];
     R(*Qm)[QH * QW] = sQ[(k + 3) & 1];
-    const bool plane = k >= 0 && k < nz;
+    const bool plane = BRICK || (k >= 0 && k < nz);
     if (plane) {
 #pragma unroll
       for (int j = 0; j < QJ; j++) {
         const int t = tid + j * NT;
         if (t < QH * QW) {
-          const int x = x0 - 2 + t % QW, y = y0 - 2 + t / QW;
           R V[5] = {R(0), R(0), R(0), R(0), R(0)};
-          if ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny)
-            prim_from_cons(Un[j], mass, nmin, V);
+          if (Uav[j]) prim_from_cons(Un[j], mass, nmin, V);
           Q[0][t] = gm1 * V[4];
           Q[1][t] = V[1];
           Q[2][t] = V[2];
@@ -893,7 +974,9 @@ __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, 
         const int x = x0 - 1 + bx, y = y0 - 1 + by;
         const int q = (by + 1) * QW + (bx + 1);
         uint8_t b = 0;
-        if ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny) {
+        const bool here = BRICK ? avail(x, y, k)
+                                : ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny);
+        if (here) {
           const R P = Q[0][q];
           // lower neighbour in x, y (same plane) and z (plane k-1)
           const int cs[3] = {x, y, k};
@@ -902,7 +985,11 @@ __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, 
                            Q[3][q] - Qm[3][q]};
 #pragma unroll
           for (int d = 0; d < 3; d++) {
-            if (cs[d] == 0) continue;
+            if constexpr (BRICK) {
+              if (!avail(x - (d == 0), y - (d == 1), k - (d == 2))) continue;
+            } else {
+              if (cs[d] == 0) continue;
+            }
             const R dP = fabs(P - Pm[d]);
             const R P_Min = fmax(fmin(P, Pm[d]), sqrt_tiny);
             const R dLnP = Arith<R>::div(dP, P_Min);
@@ -919,7 +1006,13 @@ __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, 
       const int x = x0 + tx, y = y0 + ty;
       if (x < nx && y < ny) {
         uint8_t out = 0;
-        if (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && kf >= 1 && kf <= nz - 2) {
+        // the layers ClearBoundary leaves: all but the first and last at a
+        // face of the chart (the ghost layers hold no flags here)
+        const int cm = BRICK ? a.chart_face : 63;
+        const int lx = cm & 1 ? 1 : 0, hx = cm & 2 ? nx - 2 : nx - 1;
+        const int ly = cm & 4 ? 1 : 0, hy = cm & 8 ? ny - 2 : ny - 1;
+        const int lz = cm & 16 ? 1 : 0, hz = cm & 32 ? nz - 2 : nz - 1;
+        if (x >= lx && x <= hx && y >= ly && y <= hy && kf >= lz && kf <= hz) {
           const uint8_t *Bm = sB[(kf + 2) % 3], *B0 = sB[(kf + 3) % 3],
                         *Bp = sB[(kf + 4) % 3];
           const int i = (ty + 1) * BW + (tx + 1);
@@ -937,12 +1030,12 @@ __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, 
   }
 }
 
-template <typename R>
+template <typename R, bool BRICK = false>
 __global__ __launch_bounds__(NT) void shock_flags_kernel(StageArgs<R> a, uint8_t *flags) {
   __shared__ FlagShared<R> L;
   const int tile = tile_of_block(blockIdx.x, a.nblk);
-  shock_flags_tile(a, (tile % a.nbx) * TX, ((tile / a.nbx) % a.nby) * TY,
-                   (tile / (a.nbx * a.nby)) * a.tz, flags, L);
+  shock_flags_tile<R, BRICK>(a, (tile % a.nbx) * TX, ((tile / a.nbx) % a.nby) * TY,
+                             (tile / (a.nbx * a.nby)) * a.tz, flags, L);
 }
 
 // The 23 Current fields for download, proper cells, in FP64: those of
@@ -996,9 +1089,8 @@ __global__ __launch_bounds__(256) void current_kernel(StageArgs<R> a, double c_v
 // (F/Fluid_P_I__Form.f90:509-601): the density floor N<0 -> 0 is written back
 // to the primitive field, then D, S, G.
 template <typename R>
-__global__ __launch_bounds__(256) void from_primitive_kernel(long long ntot,
-                                                             double mass_d, R *P,
-                                                             R *U) {
+__global__ __launch_bounds__(256) void from_primitive_kernel(long long ntot, long long fs,
+                                                             double mass_d, R *P, R *U) {
   const R mass = R(mass_d);
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < ntot;
        i += (long long)gridDim.x * 256) {
@@ -1007,13 +1099,13 @@ __global__ __launch_bounds__(256) void from_primitive_kernel(long long ntot,
       N = R(0);
       P[i] = N;
     }
-    const R V1 = P[ntot + i], V2 = P[2 * ntot + i], V3 = P[3 * ntot + i];
+    const R V1 = P[fs + i], V2 = P[2 * fs + i], V3 = P[3 * fs + i];
     const R S1 = mass * N * V1, S2 = mass * N * V2, S3 = mass * N * V3;
     U[i] = N;
-    U[ntot + i] = S1;
-    U[2 * ntot + i] = S2;
-    U[3 * ntot + i] = S3;
-    U[4 * ntot + i] = P[4 * ntot + i] + R(0.5) * ((S1 * V1 + S2 * V2) + S3 * V3);
+    U[fs + i] = S1;
+    U[2 * fs + i] = S2;
+    U[3 * fs + i] = S3;
+    U[4 * fs + i] = P[4 * fs + i] + R(0.5) * ((S1 * V1 + S2 * V2) + S3 * V3);
   }
 }
 

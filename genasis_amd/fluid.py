@@ -64,7 +64,10 @@ def _pd(a: np.ndarray):
 class Step_RK2_C_ASC:
     """Device-resident RK2 step of one Fluid_P_I chart."""
 
-    def __init__(self, chart: Chart, params: FluidParameters | None = None):
+    def __init__(self, chart: Chart, params: FluidParameters | None = None,
+                 n_bricks=None):
+        """n_bricks: (nx, ny, nz) splits the chart into bricks in the GPU's
+        memory (Chart_SLD's nBricks; gx_create_bricks); None is one brick."""
         self.lib = _capi.load()
         self.chart = chart
         self.params = params or FluidParameters()
@@ -77,8 +80,13 @@ class Step_RK2_C_ASC:
             g.BoundaryCondition[i] = code
         h = ctypes.c_void_p()
         cp = self.params.to_c()
-        _capi.check(self.lib.gx_create(ctypes.byref(g), ctypes.byref(cp), ctypes.byref(h)),
-                    "gx_create")
+        if n_bricks is None:
+            _capi.check(self.lib.gx_create(ctypes.byref(g), ctypes.byref(cp), ctypes.byref(h)),
+                        "gx_create")
+        else:
+            nb = (ctypes.c_int * 3)(*[int(v) for v in n_bricks])
+            _capi.check(self.lib.gx_create_bricks(ctypes.byref(g), ctypes.byref(cp), nb,
+                                                  ctypes.byref(h)), "gx_create_bricks")
         self._h = h
         self.shape = chart.n[::-1]  # (n2, n1, n0), x fastest
 
@@ -86,6 +94,12 @@ class Step_RK2_C_ASC:
         if getattr(self, "_h", None) is not None and self._h.value:
             self.lib.gx_destroy(self._h)
             self._h = None
+
+    @property
+    def n_bricks(self):
+        nb = (ctypes.c_int * 3)()
+        _capi.check(self.lib.gx_get_bricks(self._h, nb), "gx_get_bricks")
+        return tuple(nb)
 
     def __del__(self):
         try:

@@ -139,6 +139,26 @@ void gx_default_params(gx_params *p);
 int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **ctx);
 int gx_destroy(gx_ctx *ctx);
 
+/* A chart split into nBricks[0] x nBricks[1] x nBricks[2] bricks in one
+ * GPU's memory: Chart_SLD with the nBricks parameter (Modules/Mathematics/
+ * Manifolds/Atlases/Charts/ChartBasics/Chart_Template.f90:383-455; ghost
+ * exchange Charts/SingleLevelCharts/Chart_SLD__Form.f90:131-201, 324-554,
+ * 557-1007).  nBricks must divide nCells in every dimension (SetBrick), with
+ * at least 2 cells per brick and dimension.  Each brick keeps its own state
+ * with a two-layer halo for every face it shares with another brick; every
+ * RK stage exchanges the halos, the CFL maximum is taken over all bricks,
+ * and shock flags are cleared only at the faces of the chart (FluidFeatures_
+ * P__Form.f90:333-338), so the chart steps bit for bit as the one-brick
+ * chart of gx_create does.  The context takes every call of a gx_create
+ * context on the whole chart's proper cells, except the source recording,
+ * profiling, checkpoints and the adjoint (GX_E_STATE); ReconstructionType
+ * must be GX_LINEAR.  The one-rank, many-GPU layout of the same bricks is
+ * described in the design notes; here all bricks share one GPU. */
+int gx_create_bricks(const gx_grid *grid, const gx_params *params, const int nBricks[3],
+                     gx_ctx **ctx);
+/* nBricks of a context (1, 1, 1 for gx_create). */
+int gx_get_bricks(gx_ctx *ctx, int nBricks[3]);
+
 /* Upload the Current from the reference layout Value(nValues, nVariables),
  * column-major: field f of value iv at value[(fieldMap[f]-1)*ldValue + iv],
  * nValues = prod(nCells+4) with x fastest.  fieldMap has GX_N_FIELDS entries,

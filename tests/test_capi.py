@@ -63,3 +63,21 @@ def test_default_params_and_argument_errors():
     assert lib.gx_step_rk2(None, 0.0, 1.0) == -1
     assert lib.gx_set_record_sources(None, 1) == -1
     assert lib.gx_get_sources(None, None) == -1 and b"gx_get_sources" in lib.gx_last_error()
+
+
+def test_brick_arguments_are_checked():
+    """nBricks must divide nCells and leave two cells per brick (host-side
+    checks, before any device work)."""
+    lib = _capi.load()
+    g = _capi.GxGrid()
+    for d in range(3):
+        g.nCells[d] = 12
+        g.MinCoordinate[d] = 0.0
+        g.MaxCoordinate[d] = 1.0
+    for i in range(6):
+        g.BoundaryCondition[i] = _capi.GX_BC_OUTFLOW
+    h = ctypes.c_void_p()
+    for nb in ((5, 1, 1), (1, 12, 1), (0, 1, 1)):
+        arr = (ctypes.c_int * 3)(*nb)
+        assert lib.gx_create_bricks(ctypes.byref(g), None, arr, ctypes.byref(h)) == -1, nb
+        assert b"nBricks" in lib.gx_last_error() or b"2 cells" in lib.gx_last_error()
