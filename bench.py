@@ -203,8 +203,10 @@ def batch_check(n_problems: int, steps: int, seed: int, z_chunk: int, sequential
 
     upload()
     t0 = time.perf_counter()
-    tuning = batch.autotune(n_repeat=2)
+    batch.autotune(n_repeat=2)
     t_tune = time.perf_counter() - t0
+    tuning = {str(c): {"tile": [t[0], t[1]], "z_chunk": t[2]}
+              for c, t in batch.tiles().items()}
     per_call = 10
     calls = max(1, steps // per_call)
     batch.step(per_call)  # untimed: graph capture and first launches
@@ -230,7 +232,7 @@ def batch_check(n_problems: int, steps: int, seed: int, z_chunk: int, sequential
         "value": cells * n_steps / (t1 - t0),
         "unit": "cell-updates/s",
         "ms_per_step": (t1 - t0) / n_steps * 1e3,
-        "z_chunk_by_size_class": tuning,
+        "tuning_by_size_class": tuning,
         "autotune_s": t_tune,
         "sequential_value": None,
         "speedup_vs_sequential": None,
@@ -371,6 +373,8 @@ def main():
     ap.add_argument("--steps", type=int, default=98)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--tile", type=int, choices=(16, 32), default=16,
+                    help="stage tile of the main run: 16 (16 x 16) or 32 (32 x 8)")
     ap.add_argument("--z-chunk", type=int, default=0,
                     help="stage z-chunk; 0 keeps the library's default for the chart")
     ap.add_argument("--cpu-baseline", choices=("auto", "off"), default="auto")
@@ -412,6 +416,7 @@ def main():
     params = FluidParameters(CourantFactor=0.8, Precision=args.precision)
     esz = 8 if args.precision == "FP64" else 4
     stepper = Step_RK2_C_ASC(chart, params)
+    stepper.set_tile(args.tile, 256 // args.tile)
     if args.z_chunk > 0:
         stepper.set_tuning(args.z_chunk)
     z_chunk = stepper.get_tuning()
@@ -490,7 +495,8 @@ def main():
         "data": "synthetic (seeded Kelvin-Helmholtz initial data)",
         "config": {"workload": f"Kelvin-Helmholtz {n}^3 periodic, PLM(MC theta=1.4)"
                                f"+HLLC, RK2, CFL 0.8, {args.precision}",
-                   "cells": chart.n_proper, "z_chunk": z_chunk},
+                   "cells": chart.n_proper, "z_chunk": z_chunk,
+                   "tile": [args.tile, 256 // args.tile]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},

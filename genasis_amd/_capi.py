@@ -46,7 +46,7 @@ EXPORTS = (
     "gx_compute_time_step_noalloc",
     "gx_step_rk2", "gx_sync", "gx_totals", "gx_set_state", "gx_set_primitive",
     "gx_get_conserved", "gx_get_primitive", "gx_get_flags", "gx_set_flags",
-    "gx_set_tuning", "gx_get_tuning", "gx_set_record_sources", "gx_get_sources",
+    "gx_set_tuning", "gx_get_tuning", "gx_set_tile", "gx_get_tile", "gx_set_record_sources", "gx_get_sources",
     "gx_stream", "gx_set_profiling", "gx_kernel_times",
     "gx_self_test_division", "gx_last_error",
     "gx_get_boundary_fluence", "gx_compute_tally",
@@ -54,7 +54,8 @@ EXPORTS = (
     "gx_batch_create", "gx_batch_destroy", "gx_batch_set_primitive",
     "gx_batch_set_integrator", "gx_batch_step", "gx_batch_sync", "gx_batch_cfl",
     "gx_batch_times", "gx_batch_get_conserved", "gx_batch_get_flags",
-    "gx_batch_set_tuning", "gx_batch_get_tuning", "gx_batch_autotune",
+    "gx_batch_set_tuning", "gx_batch_get_tuning", "gx_batch_set_tiles",
+    "gx_batch_get_tiles", "gx_batch_autotune",
     "gx_batch_cells", "gx_batch_stream",
     "gx_adjoint_forward", "gx_adjoint_backward", "gx_adjoint_kinetic_energy",
     "gx_adjoint_get_bar", "gx_adjoint_face_vjp", "gx_adjoint_slope_vjp",
@@ -131,6 +132,8 @@ def load(path: str = LIB_PATH):
         "gx_set_flags": (i, [vp, P(ctypes.c_ubyte)]),
         "gx_set_tuning": (i, [vp, i]),
         "gx_get_tuning": (i, [vp, P(ctypes.c_int)]),
+        "gx_set_tile": (i, [vp, i, i]),
+        "gx_get_tile": (i, [vp, P(ctypes.c_int), P(ctypes.c_int)]),
         "gx_set_record_sources": (i, [vp, i]),
         "gx_get_sources": (i, [vp, pd]),
         "gx_stream": (vp, [vp]),
@@ -151,6 +154,10 @@ def load(path: str = LIB_PATH):
         "gx_batch_set_tuning": (i, [vp, i, P(ctypes.c_int), P(ctypes.c_int), i]),
         "gx_batch_get_tuning": (i, [vp, i, P(ctypes.c_int), P(ctypes.c_int),
                                     P(ctypes.c_int)]),
+        "gx_batch_set_tiles": (i, [vp, i, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+                                   P(ctypes.c_int), i]),
+        "gx_batch_get_tiles": (i, [vp, i, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+                                   P(ctypes.c_int), P(ctypes.c_int)]),
         "gx_batch_autotune": (i, [vp, i]),
         "gx_batch_cells": (ll, [vp]),
         "gx_batch_stream": (vp, [vp]),

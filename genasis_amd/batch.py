@@ -132,7 +132,32 @@ class Batch_RK2_C_ASC:
                     "gx_batch_get_tuning")
         return {int(c): int(z) for c, z in zip(cls, tz)}
 
+    def set_tiles(self, tiles: dict, use_graph: bool = True):
+        """tiles: {size class: (tileX, tileY, z-chunk)}, the stage-tile shape
+        16 x 16 or 32 x 8."""
+        ip = ctypes.POINTER(ctypes.c_int)
+        cls = np.array(list(tiles.keys()), dtype=np.int32)
+        v = np.array(list(tiles.values()), dtype=np.int32).reshape(-1, 3)
+        tx, ty, tz = (np.ascontiguousarray(v[:, i]) for i in range(3))
+        _capi.check(self.lib.gx_batch_set_tiles(self._h, len(cls), cls.ctypes.data_as(ip),
+                                                tx.ctypes.data_as(ip), ty.ctypes.data_as(ip),
+                                                tz.ctypes.data_as(ip), int(bool(use_graph))),
+                    "gx_batch_set_tiles")
+
+    def tiles(self) -> dict:
+        """{size class: (tileX, tileY, z-chunk)}."""
+        n = ctypes.c_int()
+        ip = ctypes.POINTER(ctypes.c_int)
+        _capi.check(self.lib.gx_batch_get_tiles(self._h, 0, None, None, None, None,
+                                                ctypes.byref(n)), "gx_batch_get_tiles")
+        a = [np.empty(n.value, dtype=np.int32) for _ in range(4)]
+        _capi.check(self.lib.gx_batch_get_tiles(self._h, n.value, *[x.ctypes.data_as(ip) for x in a],
+                                                ctypes.byref(n)), "gx_batch_get_tiles")
+        return {int(c): (int(x), int(y), int(z)) for c, x, y, z in zip(*a)}
+
     def autotune(self, n_repeat: int = 2) -> dict:
+        """Tile shape and z-chunk of every size class by timing; returns the
+        z-chunks ({size class: z-chunk}; tiles() has the shapes)."""
         _capi.check(self.lib.gx_batch_autotune(self._h, int(n_repeat)), "gx_batch_autotune")
         return self.tuning()
 

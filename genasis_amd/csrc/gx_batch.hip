@@ -28,11 +28,20 @@
 
 namespace gx {
 
+// Tile counters per launch, NXCD each: stage 1 of each tile shape, stage 2
+// of each tile shape, the flags.
+constexpr int N_COUNTERS = 2 * N_TILE_SHAPES + 1;
+__host__ __device__ constexpr int counter_set(int stage, int shape) {
+  return stage == 3 ? 2 * N_TILE_SHAPES : (stage - 1) * N_TILE_SHAPES + shape;
+}
+
 // One problem of a batch, as the device reads it.
 struct BatchProblem {
   int n[3];
   int bc[6];
   int nbx, nby, tz;    // stage tiles; tz = z-chunk of the problem's size class
+  int shape;           // stage-tile shape of the size class (tile_x, tile_y)
+  int fnbx, fnby;      // shock-flag tiles (16 x 16)
   int tile_start;      // first stage tile of the problem in the tile list
   int flag_tile_start; // first shock-flag tile
   int gs;              // geometry stride
@@ -99,10 +108,12 @@ __host__ __device__ inline StageArgs<R> problem_args(const BatchArgs<R> &B,
   return a;
 }
 
-template <typename R, int STAGE>
+// One launch per stage-tile shape in use: its tile list holds the tiles of
+// the size classes of that shape.
+template <typename R, int STAGE, typename T>
 __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void batch_stage_kernel(
     BatchArgs<R> B) {
-  __shared__ StageShared<R> L;
+  __shared__ StageShared<R, T> L;
   __shared__ int sNext;
   const int xcd = blockIdx.x % NXCD;
   const int q = B.ntile / NXCD, r = B.ntile % NXCD;
@@ -117,8 +128,9 @@ __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void batch_stage_kern
     StageArgs<R> a = problem_args(B, P, p, STAGE == 1 && B.src_prim[p]);
     a.dt = B.dt[p];
     const int lt = t - P.tile_start;
-    stage_tile<R, STAGE>(a, (lt % P.nbx) * TX, ((lt / P.nbx) % P.nby) * TY,
-                         (lt / (P.nbx * P.nby)) * P.tz, L);
+    stage_tile<R, STAGE, RECON_LINEAR, false, false, T>(a, (lt % P.nbx) * T::X,
+                                                        ((lt / P.nbx) % P.nby) * T::Y,
+                                                        (lt / (P.nbx * P.nby)) * P.tz, L);
   }
 }
 
@@ -139,8 +151,8 @@ __global__ __launch_bounds__(NT) void batch_flags_kernel(BatchArgs<R> B) {
     StageArgs<R> a = problem_args(B, P, p, false);
     a.tz = FLAG_TZ;
     const int lt = t - P.flag_tile_start;
-    shock_flags_tile(a, (lt % P.nbx) * TX, ((lt / P.nbx) % P.nby) * TY,
-                     (lt / (P.nbx * P.nby)) * FLAG_TZ, B.flags + P.off, L);
+    shock_flags_tile(a, (lt % P.fnbx) * TX, ((lt / P.fnbx) % P.fnby) * TY,
+                     (lt / (P.fnbx * P.fnby)) * FLAG_TZ, B.flags + P.off, L);
   }
 }
 
@@ -153,7 +165,7 @@ struct BatchClock {
   int *src_prim;      // stage 1 of this cycle reads the uploaded primitives
   int *prim_pending;  // set by an upload, moved to src_prim by the next cycle
   unsigned long long *invdt;
-  int *counters;      // 3 * NXCD: stage 1, stage 2, flags
+  int *counters;      // N_COUNTERS * NXCD: stage 1 and 2 per shape, flags
   int ncounters;
 };
 
@@ -208,13 +220,16 @@ struct gx_batch {
   std::vector<BatchProblem> prob;  // host copy of the descriptors
   std::vector<int> size_class;     // per problem
   std::map<int, int> class_tz;     // z-chunk per size class
+  std::map<int, int> class_shape;  // stage-tile shape per size class
   long long total_cells = 0;
   // device buffers
   void *U = nullptr, *Y = nullptr, *W = nullptr, *P0 = nullptr, *geo = nullptr;
   uint8_t *flags = nullptr;
   BatchProblem *prob_d = nullptr;
-  int *tile_prob_d = nullptr, *flag_tile_prob_d = nullptr;
-  int ntile = 0, nflag_tile = 0, tile_cap = 0, flag_tile_cap = 0;
+  // stage tiles of each shape (problem of each tile), and the flag tiles
+  int *tile_prob_d[N_TILE_SHAPES] = {}, *flag_tile_prob_d = nullptr;
+  int ntile[N_TILE_SHAPES] = {}, tile_cap[N_TILE_SHAPES] = {};
+  int nflag_tile = 0, flag_tile_cap = 0;
   double *time = nullptr, *dt = nullptr;
   int *ints = nullptr;  // cycle, active, src_prim, prim_pending (np each), counters
   unsigned long long *invdt = nullptr;
@@ -224,7 +239,7 @@ struct gx_batch {
   int finish_cycle = 2147483647;
   int n_ramp = 100;
   int use_graph = 1;
-  int stage_blocks = 0, flag_blocks = 0;  // persistent grid sizes
+  int stage_blocks[N_TILE_SHAPES] = {}, flag_blocks = 0;  // persistent grid sizes
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
 
@@ -250,26 +265,37 @@ static int size_class_of(const gx_grid &g) {
 // Rebuilds the stage and flag tile lists from the per-class z-chunks; a
 // problem is listed when keep[p] (all when keep is empty).
 static int build_tiles(gx_batch *b, const std::vector<char> &keep) {
-  std::vector<int> tiles, ftiles;
+  std::vector<int> tiles[N_TILE_SHAPES], ftiles;
   for (int p = 0; p < b->np; p++) {
     BatchProblem &P = b->prob[p];
     auto it = b->class_tz.find(b->size_class[p]);
     P.tz = std::max(1, std::min(it == b->class_tz.end() ? 16 : it->second, P.n[2]));
-    P.nbx = (P.n[0] + TX - 1) / TX;
-    P.nby = (P.n[1] + TY - 1) / TY;
+    auto is = b->class_shape.find(b->size_class[p]);
+    P.shape = is == b->class_shape.end() ? 0 : is->second;
+    P.nbx = (P.n[0] + tile_x(P.shape) - 1) / tile_x(P.shape);
+    P.nby = (P.n[1] + tile_y(P.shape) - 1) / tile_y(P.shape);
+    P.fnbx = (P.n[0] + TX - 1) / TX;
+    P.fnby = (P.n[1] + TY - 1) / TY;
     const int nbz = (P.n[2] + P.tz - 1) / P.tz;
     const int nfz = (P.n[2] + FLAG_TZ - 1) / FLAG_TZ;
-    P.tile_start = (int)tiles.size();
+    std::vector<int> &tl = tiles[P.shape];
+    P.tile_start = (int)tl.size();
     P.flag_tile_start = (int)ftiles.size();
     if (!keep.empty() && !keep[p]) continue;
-    tiles.insert(tiles.end(), (size_t)P.nbx * P.nby * nbz, p);
-    ftiles.insert(ftiles.end(), (size_t)P.nbx * P.nby * nfz, p);
+    tl.insert(tl.end(), (size_t)P.nbx * P.nby * nbz, p);
+    ftiles.insert(ftiles.end(), (size_t)P.fnbx * P.fnby * nfz, p);
   }
-  if ((int)tiles.size() > b->tile_cap) {
-    (void)hipFree(b->tile_prob_d);
-    b->tile_prob_d = nullptr;
-    HIP_TRY(hipMalloc(&b->tile_prob_d, sizeof(int) * tiles.size()));
-    b->tile_cap = (int)tiles.size();
+  for (int sh = 0; sh < N_TILE_SHAPES; sh++) {
+    if ((int)tiles[sh].size() > b->tile_cap[sh]) {
+      (void)hipFree(b->tile_prob_d[sh]);
+      b->tile_prob_d[sh] = nullptr;
+      HIP_TRY(hipMalloc(&b->tile_prob_d[sh], sizeof(int) * tiles[sh].size()));
+      b->tile_cap[sh] = (int)tiles[sh].size();
+    }
+    b->ntile[sh] = (int)tiles[sh].size();
+    if (!tiles[sh].empty())
+      HIP_TRY(hipMemcpy(b->tile_prob_d[sh], tiles[sh].data(), sizeof(int) * tiles[sh].size(),
+                        hipMemcpyHostToDevice));
   }
   if ((int)ftiles.size() > b->flag_tile_cap) {
     (void)hipFree(b->flag_tile_prob_d);
@@ -277,11 +303,7 @@ static int build_tiles(gx_batch *b, const std::vector<char> &keep) {
     HIP_TRY(hipMalloc(&b->flag_tile_prob_d, sizeof(int) * ftiles.size()));
     b->flag_tile_cap = (int)ftiles.size();
   }
-  b->ntile = (int)tiles.size();
   b->nflag_tile = (int)ftiles.size();
-  if (!tiles.empty())
-    HIP_TRY(hipMemcpy(b->tile_prob_d, tiles.data(), sizeof(int) * tiles.size(),
-                      hipMemcpyHostToDevice));
   if (!ftiles.empty())
     HIP_TRY(hipMemcpy(b->flag_tile_prob_d, ftiles.data(), sizeof(int) * ftiles.size(),
                       hipMemcpyHostToDevice));
@@ -292,7 +314,7 @@ static int build_tiles(gx_batch *b, const std::vector<char> &keep) {
 }
 
 template <typename R>
-static BatchArgs<R> batch_args(const gx_batch *b, int stage) {
+static BatchArgs<R> batch_args(const gx_batch *b, int stage, int shape = 0) {
   BatchArgs<R> B;
   memset(&B, 0, sizeof(B));
   B.prob = b->prob_d;
@@ -310,26 +332,24 @@ static BatchArgs<R> batch_args(const gx_batch *b, int stage) {
   B.dt = b->dt;
   B.active = b->active();
   B.src_prim = b->src_prim();
+  B.counters = b->counters() + counter_set(stage, shape) * NXCD;
   if (stage == 1) {
-    B.tile_prob = b->tile_prob_d;
-    B.ntile = b->ntile;
+    B.tile_prob = b->tile_prob_d[shape];
+    B.ntile = b->ntile[shape];
     B.src = (const R *)b->U;
     B.aux = (const R *)b->U;
     B.out1 = (R *)b->Y;
     B.out2 = (R *)b->W;
-    B.counters = b->counters();
   } else if (stage == 2) {
-    B.tile_prob = b->tile_prob_d;
-    B.ntile = b->ntile;
+    B.tile_prob = b->tile_prob_d[shape];
+    B.ntile = b->ntile[shape];
     B.src = (const R *)b->Y;
     B.aux = (const R *)b->W;
     B.out1 = (R *)b->U;
-    B.counters = b->counters() + NXCD;
   } else {
     B.tile_prob = b->flag_tile_prob_d;
     B.ntile = b->nflag_tile;
     B.src = (const R *)b->U;
-    B.counters = b->counters() + 2 * NXCD;
   }
   return B;
 }
@@ -345,17 +365,28 @@ static int enqueue_cycle(gx_batch *b) {
   clk.prim_pending = b->prim_pending();
   clk.invdt = b->invdt;
   clk.counters = b->counters();
-  clk.ncounters = 3 * NXCD;
+  clk.ncounters = N_COUNTERS * NXCD;
   const int cb = std::max(1, (std::max(b->np, clk.ncounters) + 255) / 256);
   batch_clock_kernel<<<cb, 256, 0, b->stream>>>(b->np, clk, b->params.CourantFactor,
                                                  b->finish_time, b->finish_cycle, b->n_ramp);
   HIP_TRY(hipGetLastError());
   return by_precision(b->fp32, [&](auto tag) {
     using R = decltype(tag);
-    batch_stage_kernel<R, 1><<<b->stage_blocks, NT, 0, b->stream>>>(batch_args<R>(b, 1));
-    HIP_TRY(hipGetLastError());
-    batch_stage_kernel<R, 2><<<b->stage_blocks, NT, 0, b->stream>>>(batch_args<R>(b, 2));
-    HIP_TRY(hipGetLastError());
+    for (int stage = 1; stage <= 2; stage++) {
+      for (int sh = 0; sh < N_TILE_SHAPES; sh++) {
+        if (b->ntile[sh] == 0) continue;
+        const BatchArgs<R> B = batch_args<R>(b, stage, sh);
+        if (stage == 1 && sh == 0)
+          batch_stage_kernel<R, 1, Tile16x16><<<b->stage_blocks[0], NT, 0, b->stream>>>(B);
+        else if (stage == 1)
+          batch_stage_kernel<R, 1, Tile32x8><<<b->stage_blocks[1], NT, 0, b->stream>>>(B);
+        else if (sh == 0)
+          batch_stage_kernel<R, 2, Tile16x16><<<b->stage_blocks[0], NT, 0, b->stream>>>(B);
+        else
+          batch_stage_kernel<R, 2, Tile32x8><<<b->stage_blocks[1], NT, 0, b->stream>>>(B);
+        HIP_TRY(hipGetLastError());
+      }
+    }
     batch_flags_kernel<R><<<b->flag_blocks, NT, 0, b->stream>>>(batch_args<R>(b, 3));
     HIP_TRY(hipGetLastError());
     return GX_OK;
@@ -396,8 +427,8 @@ int gx_batch_destroy(gx_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
   for (void *p : {(void *)b->U, (void *)b->Y, (void *)b->W, (void *)b->P0, b->geo,
-                  (void *)b->flags, (void *)b->prob_d, (void *)b->tile_prob_d,
-                  (void *)b->flag_tile_prob_d, (void *)b->time, (void *)b->dt,
+                  (void *)b->flags, (void *)b->prob_d, (void *)b->tile_prob_d[0],
+                  (void *)b->tile_prob_d[1], (void *)b->flag_tile_prob_d, (void *)b->time, (void *)b->dt,
                   (void *)b->ints, (void *)b->invdt})
     (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -469,6 +500,7 @@ int gx_batch_create(int nProblems, const gx_grid *grids, const gx_params *params
     goff += (long long)(img.size() / b->esz);
     b->size_class[i] = size_class_of(grids[i]);
     b->class_tz.emplace(b->size_class[i], 16);
+    b->class_shape.emplace(b->size_class[i], 0);
   }
   b->total_cells = off;
   const size_t bytes = 5 * b->esz * (size_t)off;
@@ -485,27 +517,38 @@ int gx_batch_create(int nProblems, const gx_grid *grids, const gx_params *params
   TRY_CLEAN(hipMalloc(&b->dt, sizeof(double) * nProblems));
   TRY_CLEAN(hipMemset(b->time, 0, sizeof(double) * nProblems));
   TRY_CLEAN(hipMemset(b->dt, 0, sizeof(double) * nProblems));
-  const size_t nints = 4 * (size_t)nProblems + 3 * NXCD;
+  const size_t nints = 4 * (size_t)nProblems + N_COUNTERS * NXCD;
   TRY_CLEAN(hipMalloc(&b->ints, sizeof(int) * nints));
   TRY_CLEAN(hipMemset(b->ints, 0, sizeof(int) * nints));
   TRY_CLEAN(hipMalloc(&b->invdt, sizeof(unsigned long long) * nProblems));
   TRY_CLEAN(hipMemset(b->invdt, 0, sizeof(unsigned long long) * nProblems));
-  // Persistent grids: every block resident at once.
-  int dev = 0, ncu = 0, per_cu = 0, per_cu_f = 0;
+  // Persistent grids: every block resident at once (per shape, the smaller
+  // occupancy of its two stage kernels).
+  int dev = 0, ncu = 0, per_cu_f = 0;
   TRY_CLEAN(hipGetDevice(&dev));
   TRY_CLEAN(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const void *kern[N_TILE_SHAPES][2];
   if (b->fp32) {
-    TRY_CLEAN(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, (const void *)batch_stage_kernel<float, 2>, NT, 0));
+    kern[0][0] = (const void *)batch_stage_kernel<float, 1, Tile16x16>;
+    kern[0][1] = (const void *)batch_stage_kernel<float, 2, Tile16x16>;
+    kern[1][0] = (const void *)batch_stage_kernel<float, 1, Tile32x8>;
+    kern[1][1] = (const void *)batch_stage_kernel<float, 2, Tile32x8>;
     TRY_CLEAN(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu_f, (const void *)batch_flags_kernel<float>, NT, 0));
   } else {
-    TRY_CLEAN(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, (const void *)batch_stage_kernel<double, 2>, NT, 0));
+    kern[0][0] = (const void *)batch_stage_kernel<double, 1, Tile16x16>;
+    kern[0][1] = (const void *)batch_stage_kernel<double, 2, Tile16x16>;
+    kern[1][0] = (const void *)batch_stage_kernel<double, 1, Tile32x8>;
+    kern[1][1] = (const void *)batch_stage_kernel<double, 2, Tile32x8>;
     TRY_CLEAN(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu_f, (const void *)batch_flags_kernel<double>, NT, 0));
   }
-  b->stage_blocks = std::max(NXCD, ncu * std::max(1, per_cu));
+  for (int sh = 0; sh < N_TILE_SHAPES; sh++) {
+    int o1 = 0, o2 = 0;
+    TRY_CLEAN(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o1, kern[sh][0], NT, 0));
+    TRY_CLEAN(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o2, kern[sh][1], NT, 0));
+    b->stage_blocks[sh] = std::max(NXCD, ncu * std::max(1, std::min(o1, o2)));
+  }
   b->flag_blocks = std::max(NXCD, ncu * std::max(1, per_cu_f));
 #undef TRY_CLEAN
   rc = build_tiles(b, {});
@@ -685,6 +728,47 @@ int gx_batch_set_tuning(gx_batch *b, int nClasses, const int *sizeClass, const i
   GX_API_END("gx_batch_set_tuning")
 }
 
+int gx_batch_set_tiles(gx_batch *b, int nClasses, const int *sizeClass, const int *tileX,
+                       const int *tileY, const int *zChunk, int useGraph) {
+  GX_API_BEGIN
+  if (!b || nClasses < 0 || (nClasses > 0 && (!sizeClass || !tileX || !tileY || !zChunk)))
+    return fail(GX_E_ARG, "gx_batch_set_tiles: bad argument");
+  std::vector<int> shapes(nClasses, -1);
+  for (int i = 0; i < nClasses; i++) {
+    if (zChunk[i] < 1) return fail(GX_E_ARG, "gx_batch_set_tiles: zChunk must be >= 1");
+    for (int k = 0; k < N_TILE_SHAPES; k++)
+      if (tileX[i] == tile_x(k) && tileY[i] == tile_y(k)) shapes[i] = k;
+    if (shapes[i] < 0) return fail(GX_E_ARG, "gx_batch_set_tiles: the stage tile is 16 x 16 or 32 x 8");
+  }
+  for (int i = 0; i < nClasses; i++) {
+    b->class_tz[sizeClass[i]] = zChunk[i];
+    b->class_shape[sizeClass[i]] = shapes[i];
+  }
+  b->use_graph = useGraph != 0;
+  return build_tiles(b, {});
+  GX_API_END("gx_batch_set_tiles")
+}
+
+int gx_batch_get_tiles(gx_batch *b, int capacity, int *sizeClass, int *tileX, int *tileY,
+                       int *zChunk, int *nClasses) {
+  GX_API_BEGIN
+  if (!b || !nClasses) return fail(GX_E_ARG, "gx_batch_get_tiles: null argument");
+  int i = 0;
+  for (const auto &kv : b->class_tz) {
+    if (i < capacity && sizeClass && tileX && tileY && zChunk) {
+      const int sh = b->class_shape[kv.first];
+      sizeClass[i] = kv.first;
+      tileX[i] = tile_x(sh);
+      tileY[i] = tile_y(sh);
+      zChunk[i] = kv.second;
+    }
+    i++;
+  }
+  *nClasses = i;
+  return GX_OK;
+  GX_API_END("gx_batch_get_tiles")
+}
+
 int gx_batch_get_tuning(gx_batch *b, int capacity, int *sizeClass, int *zChunk,
                         int *nClasses) {
   GX_API_BEGIN
@@ -711,7 +795,7 @@ int gx_batch_autotune(gx_batch *b, int nRepeat) {
   // Every problem is made active with its current dt; the flags stay.  On
   // any failure the z-chunks in force before the call and the full tile
   // list are restored before returning.
-  const std::map<int, int> before = b->class_tz;
+  const std::map<int, int> before = b->class_tz, before_shape = b->class_shape;
   std::vector<int> ones(b->np, 1);
   HIP_TRY(hipMemcpy(b->active(), ones.data(), sizeof(int) * b->np, hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(b->src_prim(), 0, sizeof(int) * b->np));
@@ -724,7 +808,7 @@ int gx_batch_autotune(gx_batch *b, int nRepeat) {
   };
   hip_ok(hipEventCreate(&e0), "hipEventCreate");
   hip_ok(hipEventCreate(&e1), "hipEventCreate");
-  std::map<int, int> best;
+  std::map<int, int> best, best_shape;
   for (const auto &kv : before) {
     if (rc) break;
     const int cls = kv.first;
@@ -736,22 +820,30 @@ int gx_batch_autotune(gx_batch *b, int nRepeat) {
         nmax = std::max(nmax, b->prob[p].n[2]);
       }
     float best_ms = 1e30f;
-    int best_tz = kv.second;
-    for (int tz : {4, 8, 16, 32, 64}) {
+    int best_tz = kv.second, best_sh = before_shape.at(cls);
+    for (int trial = 0; trial < 5 * N_TILE_SHAPES; trial++) {
+      const int sh = trial / 5, tz = 4 << (trial % 5);  // z-chunks 4 .. 64
       if (rc) break;
       if (tz > nmax && tz != 4) continue;
       b->class_tz[cls] = tz;
+      b->class_shape[cls] = sh;
       rc = build_tiles(b, keep);
       float ms_min = 1e30f;
       for (int r = 0; r < nRepeat + 1 && rc == GX_OK; r++) {
-        if (!hip_ok(hipMemsetAsync(b->counters(), 0, sizeof(int) * NXCD, b->stream),
+        const int cset = counter_set(1, sh);
+        if (!hip_ok(hipMemsetAsync(b->counters() + cset * NXCD, 0, sizeof(int) * NXCD,
+                                   b->stream),
                     "hipMemsetAsync") ||
             !hip_ok(hipEventRecord(e0, b->stream), "hipEventRecord"))
           break;
         rc = by_precision(b->fp32, [&](auto tag) {
           using R = decltype(tag);
-          batch_stage_kernel<R, 1><<<b->stage_blocks, NT, 0, b->stream>>>(
-              batch_args<R>(b, 1));
+          if (sh == 1)
+            batch_stage_kernel<R, 1, Tile32x8><<<b->stage_blocks[1], NT, 0, b->stream>>>(
+                batch_args<R>(b, 1, 1));
+          else
+            batch_stage_kernel<R, 1, Tile16x16><<<b->stage_blocks[0], NT, 0, b->stream>>>(
+                batch_args<R>(b, 1, 0));
           HIP_TRY(hipGetLastError());
           return GX_OK;
         });
@@ -765,15 +857,20 @@ int gx_batch_autotune(gx_batch *b, int nRepeat) {
       if (rc == GX_OK && ms_min < best_ms) {
         best_ms = ms_min;
         best_tz = tz;
+        best_sh = sh;
       }
     }
     best[cls] = best_tz;
+    best_shape[cls] = best_sh;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   b->class_tz = before;
-  if (rc == GX_OK)
+  b->class_shape = before_shape;
+  if (rc == GX_OK) {
     for (const auto &kv : best) b->class_tz[kv.first] = kv.second;
+    for (const auto &kv : best_shape) b->class_shape[kv.first] = kv.second;
+  }
   const int rc2 = build_tiles(b, {});
   return rc ? rc : rc2;
   GX_API_END("gx_batch_autotune")

@@ -279,8 +279,9 @@ static StageArgs<R> brick_args(const gx_ctx *c, int b) {
   a.fs = B.fs;
   a.geo = (const R *)S.geo + B.geo_off;
   a.flags = S.flags + B.fbase;
-  a.nbx = (B.n[0] + TX - 1) / TX;
-  a.nby = (B.n[1] + TY - 1) / TY;
+  const int shape = stage_shape(c);
+  a.nbx = (B.n[0] + tile_x(shape) - 1) / tile_x(shape);
+  a.nby = (B.n[1] + tile_y(shape) - 1) / tile_y(shape);
   a.nblk = a.nbx * a.nby * ((B.n[2] + c->tz - 1) / c->tz);
   return a;
 }
@@ -651,7 +652,10 @@ int bricks_step(gx_ctx *c, double dt) {
       a.aux = U + B.base;
       a.out1 = Y + B.base;
       a.out2 = W + B.base;
-      stage_kernel<R, 1, false, true><<<a.nblk, NT, 0, st>>>(a);
+      if (stage_shape(c) == 1)
+        stage_kernel<R, 1, false, true, Tile32x8><<<a.nblk, NT, 0, st>>>(a);
+      else
+        stage_kernel<R, 1, false, true><<<a.nblk, NT, 0, st>>>(a);
       HIP_TRY(hipGetLastError());
       if (B.chart_face) {
         BoundaryArgs Bb = BA;
@@ -673,7 +677,10 @@ int bricks_step(gx_ctx *c, double dt) {
       a.src = Y + B.base;
       a.aux = W + B.base;
       a.out1 = U + B.base;
-      stage_kernel<R, 2, false, true><<<a.nblk, NT, 0, st>>>(a);
+      if (stage_shape(c) == 1)
+        stage_kernel<R, 2, false, true, Tile32x8><<<a.nblk, NT, 0, st>>>(a);
+      else
+        stage_kernel<R, 2, false, true><<<a.nblk, NT, 0, st>>>(a);
       HIP_TRY(hipGetLastError());
       if (B.chart_face) {
         BoundaryArgs Bb = BA;
@@ -694,6 +701,8 @@ int bricks_step(gx_ctx *c, double dt) {
       const BrickDesc &B = S.desc[b];
       StageArgs<R> a = brick_args<R>(c, b);
       a.src = U + B.base;
+      a.nbx = (B.n[0] + TX - 1) / TX;  // the shock-flag tile is 16 x 16
+      a.nby = (B.n[1] + TY - 1) / TY;
       a.tz = flag_tz(a.nbx * a.nby, B.n[2]);
       a.nblk = a.nbx * a.nby * ((B.n[2] + a.tz - 1) / a.tz);
       shock_flags_kernel<R, true><<<a.nblk, NT, 0, st>>>(a, S.flags + B.fbase);

@@ -48,6 +48,7 @@ struct gx_ctx {
   long long ncell;
   hipStream_t stream = nullptr;
   int tz = 16;                // z-chunk of the stage kernels (stage_tz at creation)
+  int tile = 0;               // stage-tile shape (gx_set_tile): 0 = 16 x 16, 1 = 32 x 8
   bool fp32 = false;          // state and sweeps in float (Precision = GX_FP32)
   size_t esz = sizeof(double);  // bytes per state value
   // device buffers, of the state's real type
@@ -87,6 +88,12 @@ struct gx_ctx {
 };
 
 namespace gx {
+// The stage-tile shape of the context's stage kernels: the chosen one for
+// LINEAR steps without source recording, 16 x 16 otherwise.
+inline int stage_shape(const gx_ctx *c) {
+  return c->params.ReconstructionType == GX_LINEAR && !c->record_sources ? c->tile : 0;
+}
+
 // Where one RK2 step reads and writes when a caller records it (the discrete
 // adjoint's forward run): the step input Uin (which it leaves intact), the
 // stage values Y and W, the new state Uout and the state before the

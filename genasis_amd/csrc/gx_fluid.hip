@@ -143,6 +143,8 @@ static void launch_stage(const gx_ctx *c, int nblk, hipStream_t st, const StageA
       stage_kernel<R, STAGE, true><<<nblk, NT, 0, st>>>(a);
   } else if (para) {
     stage_kernel_parabolic<R, STAGE><<<nblk, NT, 0, st>>>(a);
+  } else if (stage_shape(c) == 1) {
+    stage_kernel<R, STAGE, false, false, Tile32x8><<<nblk, NT, 0, st>>>(a);
   } else {
     stage_kernel<R, STAGE><<<nblk, NT, 0, st>>>(a);
   }
@@ -312,6 +314,29 @@ int gx_set_tuning(gx_ctx *c, int zChunk) {
   c->tz = std::min(zChunk, c->n[2]);  // a longer chunk is the whole chart
   return GX_OK;
   GX_API_END("gx_set_tuning")
+}
+
+int gx_set_tile(gx_ctx *c, int tileX, int tileY) {
+  GX_API_BEGIN
+  if (!c) return fail(GX_E_ARG, "gx_set_tile: null context");
+  int shape = -1;
+  for (int k = 0; k < N_TILE_SHAPES; k++)
+    if (tileX == tile_x(k) && tileY == tile_y(k)) shape = k;
+  if (shape < 0) return fail(GX_E_ARG, "gx_set_tile: the stage tile is 16 x 16 or 32 x 8");
+  if (shape != 0 && c->params.ReconstructionType != GX_LINEAR)
+    return fail(GX_E_ARG, "gx_set_tile: PARABOLIC steps take the 16 x 16 tile");
+  c->tile = shape;
+  return GX_OK;
+  GX_API_END("gx_set_tile")
+}
+
+int gx_get_tile(gx_ctx *c, int *tileX, int *tileY) {
+  GX_API_BEGIN
+  if (!c || !tileX || !tileY) return fail(GX_E_ARG, "gx_get_tile: null argument");
+  *tileX = tile_x(c->tile);
+  *tileY = tile_y(c->tile);
+  return GX_OK;
+  GX_API_END("gx_get_tile")
 }
 
 int gx_set_record_sources(gx_ctx *c, int on) {
@@ -766,9 +791,11 @@ namespace gx {
 
 int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
   if (rec && c->fp32) return fail(GX_E_ARG, "step_rk2: a recorded step is FP64");
-  const dim3 block(NT);
-  const int nbx = (c->n[0] + TX - 1) / TX, nby = (c->n[1] + TY - 1) / TY,
+  const int shape = stage_shape(c);
+  const int nbx = (c->n[0] + tile_x(shape) - 1) / tile_x(shape),
+            nby = (c->n[1] + tile_y(shape) - 1) / tile_y(shape),
             nbz = (c->n[2] + c->tz - 1) / c->tz;
+  const int fbx = (c->n[0] + TX - 1) / TX, fby = (c->n[1] + TY - 1) / TY;  // flag tiles
   const int nblk = nbx * nby * nbz;
   BoundaryArgs B;
   B.bf = c->fluence;
@@ -834,10 +861,10 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
     // Shock flags of the next step from the new state.
     StageArgs<R> a3 = base_args<R>(c);
     a3.src = Uout;
-    a3.tz = flag_tz(nbx * nby, c->n[2]);
-    a3.nbx = nbx;
-    a3.nby = nby;
-    a3.nblk = nbx * nby * ((c->n[2] + a3.tz - 1) / a3.tz);
+    a3.tz = flag_tz(fbx * fby, c->n[2]);
+    a3.nbx = fbx;
+    a3.nby = fby;
+    a3.nblk = fbx * fby * ((c->n[2] + a3.tz - 1) / a3.tz);
     shock_flags_kernel<R><<<a3.nblk, NT, 0, st>>>(a3, c->flags);
     HIP_TRY(hipGetLastError());
     return GX_OK;

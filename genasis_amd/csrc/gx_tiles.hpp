@@ -20,15 +20,29 @@
 
 namespace gx {
 
-constexpr int TX = 16;           // tile width in x (cells)
-constexpr int TY = 16;           // tile height in y
+constexpr int TX = 16;           // tile width in x (cells): the default stage tile
+constexpr int TY = 16;           // tile height in y; also the shock-flag tile
 constexpr int NT = TX * TY;      // threads per block
-constexpr int HX = TX + 4;       // plane tile with two ghost layers
-constexpr int HY = TY + 4;
-constexpr int SXW = TX + 2;      // x-slope row length
-constexpr int FXW = TX + 1;      // x-flux row length
 constexpr int WAVE = 64;
 constexpr int NXCD = 8;          // accelerator complex dies, one L2 each
+
+// Shape of a stage tile: X x Y cells, one thread each (NT threads); the
+// plane tile with two ghost layers is HX x HY, the x-slope and x-flux rows
+// are SXW and FXW long.
+template <int X_, int Y_>
+struct TileShape {
+  static constexpr int X = X_, Y = Y_;
+  static constexpr int HX = X + 4, HY = Y + 4, SXW = X + 2, FXW = X + 1;
+  static_assert(X * Y == NT, "a stage tile has one cell per thread");
+  static_assert(X <= WAVE && Y <= WAVE, "wave 0 takes the right column, wave 1 the top row");
+};
+using Tile16x16 = TileShape<16, 16>;
+using Tile32x8 = TileShape<32, 8>;
+// Tile shapes by number (gx_set_tile, gx_batch_set_tiles): 0 = 16 x 16,
+// 1 = 32 x 8.
+constexpr int N_TILE_SHAPES = 2;
+__host__ __device__ constexpr int tile_x(int shape) { return shape == 1 ? 32 : 16; }
+__host__ __device__ constexpr int tile_y(int shape) { return shape == 1 ? 8 : 16; }
 
 __host__ __device__ inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
@@ -258,11 +272,12 @@ __device__ __forceinline__ void halo_cell(int t, int x0, int y0, int tw, int th,
 }
 
 // Geometry of the tile in LDS: array g of dimension d (0 = x, 1 = y) at the
-// local index c - origin + 2 for c in [origin-2, origin+TX+2).
-template <typename R> using TileGeo = R[2][N_GEO][HX];
+// local index c - origin + 2 for c in [origin-2, origin+X+2) (the x extent
+// bounds both dimensions: X >= Y).
+template <typename R, typename T = Tile16x16> using TileGeo = R[2][N_GEO][T::HX];
 
-template <typename R>
-__device__ __forceinline__ R slope_lds(const TileGeo<R> &sG, int d, int l, R vm,
+template <typename R, typename G>
+__device__ __forceinline__ R slope_lds(const G &sG, int d, int l, R vm,
                                        R v0, R vp, R theta, int use_limiter) {
   return slope(vm, v0, vp, sG[d][G_DXL][l], sG[d][G_DXR][l], sG[d][G_DXL2][l],
                sG[d][G_DXR2][l], sG[d][G_DEN][l], sG[d][G_RDXL][l],
@@ -270,8 +285,8 @@ __device__ __forceinline__ R slope_lds(const TileGeo<R> &sG, int d, int l, R vm,
 }
 
 // Parabolic face values of the cell at local geometry index l.
-template <typename R>
-__device__ __forceinline__ void parabolic_lds(const TileGeo<R> &sG, int d, int l, R vm,
+template <typename R, typename G>
+__device__ __forceinline__ void parabolic_lds(const G &sG, int d, int l, R vm,
                                               R v0, R vp, R *vI, R *vO) {
   parabolic_faces(vm, v0, vp, sG[d][G_CENTER][l - 1], sG[d][G_CENTER][l],
                   sG[d][G_CENTER][l + 1], sG[d][G_WL][l], sG[d][G_WR][l], vI, vO);
@@ -288,9 +303,10 @@ __device__ __forceinline__ void face_to_lds(const Physics &ph, const R WL[5], co
 }
 
 // x-face i (between cells x0+i-1 and x0+i) of row j into sFx.
-template <typename R, typename SP, typename SS, typename SF>
+template <typename T, typename R, typename SP, typename SS, typename SF>
 __device__ __forceinline__ void x_face(const Physics &ph, SP sP, SS sSx, SF sFx,
-                                       const TileGeo<R> &sG, int i, int j, bool df) {
+                                       const TileGeo<R, T> &sG, int i, int j, bool df) {
+  constexpr int HX = T::HX, SXW = T::SXW, FXW = T::FXW;
   const int bl = (j + 2) * HX + (i + 1);
   const R wr = sG[0][G_WR][i + 1], wl = sG[0][G_WL][i + 2];
   R WL[5], WR[5], F[5];
@@ -305,9 +321,10 @@ __device__ __forceinline__ void x_face(const Physics &ph, SP sP, SS sSx, SF sFx,
 }
 
 // y-face j (between rows y0+j-1 and y0+j) of column i into sFy.
-template <typename R, typename SP, typename SS, typename SF>
+template <typename T, typename R, typename SP, typename SS, typename SF>
 __device__ __forceinline__ void y_face(const Physics &ph, SP sP, SS sSy, SF sFy,
-                                       const TileGeo<R> &sG, int i, int j, bool df) {
+                                       const TileGeo<R, T> &sG, int i, int j, bool df) {
+  constexpr int HX = T::HX, TX = T::X;
   const int bl = (j + 1) * HX + (i + 2);
   const R wr = sG[1][G_WR][j + 1], wl = sG[1][G_WL][j + 2];
   R WL[5], WR[5], F[5];
@@ -331,31 +348,34 @@ __device__ __forceinline__ void y_face(const Physics &ph, SP sP, SS sSy, SF sFy,
 // the slope and flux work.  Stage 2 applies the conserved fix and the CFL term
 // in FP64 whatever R is.
 // LDS of one stage tile.
-template <typename R> struct StageShared {
-  R sP[5][HY * HX];
-  R sSx[5][TY * SXW];       // x-slopes of cells x0-1 .. x0+tw
-  R sSy[5][(TY + 2) * TX];  // y-slopes of cells y0-1 .. y0+th
-  R sFx[5][TY * FXW];       // x-face fluxes x0 .. x0+tw
-  R sFy[5][(TY + 1) * TX];  // y-face fluxes y0 .. y0+th
+template <typename R, typename T = Tile16x16> struct StageShared {
+  R sP[5][T::HY * T::HX];
+  R sSx[5][T::Y * T::SXW];       // x-slopes of cells x0-1 .. x0+tw
+  R sSy[5][(T::Y + 2) * T::X];   // y-slopes of cells y0-1 .. y0+th
+  R sFx[5][T::Y * T::FXW];       // x-face fluxes x0 .. x0+tw
+  R sFy[5][(T::Y + 1) * T::X];   // y-face fluxes y0 .. y0+th
   double sRed[NT / WAVE];
-  TileGeo<R> sG;
+  TileGeo<R, T> sG;
 };
 
 // PARABOLIC only: the inner-face values of the cells just beyond the tile's
 // right column and top row (the slopes' LDS then holds outer-face values).
-template <typename R> struct EdgeShared {
-  R sEx[5][TY];
-  R sEy[5][TX];
+template <typename R, typename T = Tile16x16> struct EdgeShared {
+  R sEx[5][T::Y];
+  R sEy[5][T::X];
 };
 
 // REC: record the divergence sources (a.sources).  BRICK: a brick of a
 // chart split into bricks (ghosts of BC_NEIGHBOR faces from the halos, and
-// the flags of the faces beyond the brick from the flag halo).
+// the flags of the faces beyond the brick from the flag halo).  T: the
+// tile's shape.
 template <typename R, int STAGE, int RECON = RECON_LINEAR, bool REC = false,
-          bool BRICK = false>
+          bool BRICK = false, typename T = Tile16x16>
 __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0, int z0,
-                                           StageShared<R> &L, EdgeShared<R> *E = nullptr) {
+                                           StageShared<R, T> &L,
+                                           EdgeShared<R, T> *E = nullptr) {
   constexpr bool PARA = RECON == RECON_PARABOLIC;
+  constexpr int TX = T::X, TY = T::Y, HX = T::HX, SXW = T::SXW, FXW = T::FXW;
   auto &sP = L.sP;
   auto &sSx = L.sSx;
   auto &sSy = L.sSy;
@@ -431,7 +451,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
   // Halo cells of the plane tile.  FP64: waves 2 and 3 convert the next
   // plane's halo while waves 0 and 1 compute the tile-edge faces, so the
   // plane's load phase only stores them.
-  constexpr bool LATE = sizeof(R) == sizeof(double);
+  // (the shape's halo must fit waves 2 and 3)
+  constexpr bool LATE = sizeof(R) == sizeof(double) && 4 * (TX + TY) <= 2 * WAVE;
   const int wave = __builtin_amdgcn_readfirstlane(tid / WAVE);
   const int hid = LATE ? tid - 2 * WAVE : tid;
   const bool htask = hid >= 0 && hid < nhalo;
@@ -521,7 +542,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
     // field 4), so no wave adds more than two fields per edge task to its
     // own work.  FP32 has wave 0 take all five fields: with four blocks per
     // CU the others fill its barrier waits, and sharing measured slower.
-    constexpr bool SPREAD = sizeof(R) == sizeof(double);
+    // (with 2 (TX + TY) <= 64 edge cells, one wave-pass per field)
+  constexpr bool SPREAD = sizeof(R) == sizeof(double) && 2 * (TX + TY) <= WAVE;
     const int ut = SPREAD ? tid % WAVE : tid;
     if (ut < nxs + nys) {
       int base, stride, l, out, gd;
@@ -632,14 +654,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
       }
     } else {
       if (active) {
-        x_face(ph, sP, sSx, sFx, sG, tx, ty, flk & 1u);
+        x_face<T>(ph, sP, sSx, sFx, sG, tx, ty, flk & 1u);
         __builtin_amdgcn_sched_barrier(0);
-        y_face(ph, sP, sSy, sFy, sG, tx, ty, (flk >> 1) & 1u);
+        y_face<T>(ph, sP, sSy, sFy, sG, tx, ty, (flk >> 1) & 1u);
         __builtin_amdgcn_sched_barrier(0);
       }
-      if (tid < th) x_face(ph, sP, sSx, sFx, sG, tw, tid, flx);
+      if (tid < th) x_face<T>(ph, sP, sSx, sFx, sG, tw, tid, flx);
       __builtin_amdgcn_sched_barrier(0);
-      if (tid >= WAVE && tid - WAVE < tw) y_face(ph, sP, sSy, sFy, sG, tid - WAVE, th, flx);
+      if (tid >= WAVE && tid - WAVE < tw) y_face<T>(ph, sP, sSy, sFy, sG, tid - WAVE, th, flx);
     }
     if constexpr (LATE) {
       if (htask && more) raw_to_prim(a, hr, hflip, hv);  // the next plane's halo
@@ -769,14 +791,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
   }
 }
 
-template <typename R, int STAGE, bool REC = false, bool BRICK = false>
+template <typename R, int STAGE, bool REC = false, bool BRICK = false,
+          typename T = Tile16x16>
 __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel(
     StageArgs<R> a) {
-  __shared__ StageShared<R> L;
+  __shared__ StageShared<R, T> L;
   const int tile = tile_of_block(blockIdx.x, a.nblk);
-  stage_tile<R, STAGE, RECON_LINEAR, REC, BRICK>(a, (tile % a.nbx) * TX,
-                                                 ((tile / a.nbx) % a.nby) * TY,
-                                                 (tile / (a.nbx * a.nby)) * a.tz, L);
+  stage_tile<R, STAGE, RECON_LINEAR, REC, BRICK, T>(a, (tile % a.nbx) * T::X,
+                                                    ((tile / a.nbx) % a.nby) * T::Y,
+                                                    (tile / (a.nbx * a.nby)) * a.tz, L);
 }
 
 // The same stage with PARABOLIC reconstruction.

@@ -242,6 +242,16 @@ class Step_RK2_C_ASC:
     def set_tuning(self, z_chunk: int):
         _capi.check(self.lib.gx_set_tuning(self._h, int(z_chunk)), "gx_set_tuning")
 
+    def set_tile(self, tile_x: int, tile_y: int):
+        """Stage-tile shape: 16 x 16 (default) or 32 x 8."""
+        _capi.check(self.lib.gx_set_tile(self._h, int(tile_x), int(tile_y)), "gx_set_tile")
+
+    def get_tile(self):
+        x, y = ctypes.c_int(), ctypes.c_int()
+        _capi.check(self.lib.gx_get_tile(self._h, ctypes.byref(x), ctypes.byref(y)),
+                    "gx_get_tile")
+        return x.value, y.value
+
     def record_sources(self, on: bool = True):
         """Record Current % Sources of the divergence in every step
         (RecordDivergence); turning it on clears them."""

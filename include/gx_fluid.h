@@ -301,6 +301,14 @@ int gx_set_flags(gx_ctx *ctx, const unsigned char *flags);
 int gx_set_tuning(gx_ctx *ctx, int zChunk);
 int gx_get_tuning(gx_ctx *ctx, int *zChunk);
 
+/* Stage-tile shape of the fused stage kernels: tileX x tileY cells in x and y
+ * per thread block, 16 x 16 (the default) or 32 x 8.  The results do not
+ * depend on it (every cell takes the same operations); it changes the halo
+ * share and the partial tiles of a chart.  PARABOLIC steps and steps that
+ * record the divergence sources use 16 x 16. */
+int gx_set_tile(gx_ctx *ctx, int tileX, int tileY);
+int gx_get_tile(gx_ctx *ctx, int *tileX, int *tileY);
+
 /* Current % Sources of the divergence (ApplyDivergence_C's recording,
  * Step_RK_C_ASC__Template.f90:1548-1595 with RecordDivergenceKernel,
  * Step_RK_C_ASC__Kernel.f90:12-46): with recording on, every gx_step_rk2
@@ -385,9 +393,18 @@ int gx_batch_set_tuning(gx_batch *batch, int nClasses, const int *sizeClass,
 /* Up to capacity (class, z-chunk) pairs; *nClasses = number of classes. */
 int gx_batch_get_tuning(gx_batch *batch, int capacity, int *sizeClass, int *zChunk,
                         int *nClasses);
-/* Chooses the z-chunk of every size class by timing the stage-1 launch over
- * the problems of that class for z-chunks 4 .. 64 (best of nRepeat).  Needs
- * every problem's state; leaves the states unchanged. */
+/* Stage-tile shape (tileX x tileY: 16 x 16 or 32 x 8, as gx_set_tile) and
+ * z-chunk of each size class; useGraph as gx_batch_set_tuning.  The stage
+ * launches of a cycle run once per shape in use, each over the tiles of its
+ * size classes. */
+int gx_batch_set_tiles(gx_batch *batch, int nClasses, const int *sizeClass,
+                       const int *tileX, const int *tileY, const int *zChunk, int useGraph);
+int gx_batch_get_tiles(gx_batch *batch, int capacity, int *sizeClass, int *tileX,
+                       int *tileY, int *zChunk, int *nClasses);
+/* Chooses the stage-tile shape and the z-chunk of every size class by timing
+ * the stage-1 launch over the problems of that class for both shapes and
+ * z-chunks 4 .. 64 (best of nRepeat).  Needs every problem's state; leaves
+ * the states unchanged. */
 int gx_batch_autotune(gx_batch *batch, int nRepeat);
 
 /* Total proper cells of the batch, and its HIP stream (a hipStream_t). */

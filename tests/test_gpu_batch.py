@@ -126,6 +126,13 @@ def test_ensemble_size_classes_vs_oracle():
     batch.set_integrator(FinishTime=finish)
     tuned = batch.autotune(n_repeat=1)
     assert set(tuned) == classes, tuned
+    tiles = batch.tiles()
+    assert all(t[:2] in ((16, 16), (32, 8)) for t in tiles.values()), tiles
+    # both shapes in one cycle: alternate classes take 32 x 8
+    mixed = {c: ((32, 8) if i % 2 else (16, 16)) + (tiles[c][2],)
+             for i, c in enumerate(sorted(tiles))}
+    batch.set_tiles(mixed)
+    assert batch.tiles() == mixed
     batch.step(5)
     batch.sync()
     t, _, cyc = batch.times()

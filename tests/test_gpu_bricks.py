@@ -92,6 +92,23 @@ def test_bricks_sod_across_bricks():
     _lockstep(chart, problems.sod(chart), (8, 2, 1), FluidParameters(CourantFactor=0.8), 40)
 
 
+def test_bricks_tile_32x8():
+    chart = Chart((64, 32, 24), bc=MIXED)
+    one = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8))
+    many = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8), n_bricks=(2, 2, 2))
+    many.set_tile(32, 8)
+    prim = _shocky(chart, 11)
+    one.set_primitive(*prim)
+    many.set_primitive(*prim)
+    for s in range(6):
+        c = one.time_step_candidate()
+        assert c == many.time_step_candidate()
+        one.step(0.0, c)
+        many.step(0.0, c)
+        assert same_bits(one.conserved(), many.conserved()), s
+        assert np.array_equal(one.flags(), many.flags()), s
+
+
 def test_bricks_fp32():
     chart = Chart((48, 32, 24), bc=MIXED)
     _lockstep(chart, _shocky(chart, 7), (2, 2, 2),

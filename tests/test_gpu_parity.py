@@ -97,6 +97,35 @@ def test_random_reflecting_partial_tiles_bitwise():
     _run_lockstep(o, g, chart, 5, "random")
 
 
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_tile_32x8_bitwise(precision):
+    """The 32 x 8 stage tile takes the same operations per cell as the
+    16 x 16 one: FP64 equals the oracle bit for bit (partial tiles, every
+    boundary condition, shocks); FP32 equals the 16 x 16 FP32 context bit
+    for bit."""
+    chart = Chart((37, 11, 19), bc=(("REFLECTING", "OUTFLOW"), ("PERIODIC", "PERIODIC"),
+                                    ("OUTFLOW", "REFLECTING")))
+    prim = _random_state(chart, 9)
+    if precision == "FP64":
+        o, g = _pair(chart, prim)
+        g.set_tile(32, 8)
+        assert g.get_tile() == (32, 8)
+        _run_lockstep(o, g, chart, 6, "tile 32x8")
+        return
+    params = FluidParameters(CourantFactor=0.8, Precision="FP32")
+    a, b = Step_RK2_C_ASC(chart, params), Step_RK2_C_ASC(chart, params)
+    b.set_tile(32, 8)
+    for g in (a, b):
+        g.set_primitive(*prim)
+    for s in range(6):
+        ca, cb = a.time_step_candidate(), b.time_step_candidate()
+        assert ca == cb
+        a.step(0.0, ca)
+        b.step(0.0, ca)
+        assert same_bits(a.conserved(), b.conserved()), s
+        assert np.array_equal(a.flags(), b.flags()), s
+
+
 def test_hll_and_unlimited_bitwise():
     chart = Chart((20, 12, 10), bc=OUT)
     prim = _random_state(chart, 5)
