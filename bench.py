@@ -50,6 +50,8 @@ HBM_PEAK_GBS = 8000.0
 ALG_BYTES_STAGE2 = 120.0
 ALG_BYTES_STEP = 200.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_kh256_fp64_traffic.txt")
+SQ_FILE = os.path.join(ROOT, "profiles", "stage_kernel_kh256_fp64_sq_counters.txt")
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 
 
 def _read_traffic():
@@ -59,6 +61,19 @@ def _read_traffic():
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 if line.startswith("stage2_hbm_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def _read_stage2_flop():
+    """FP64 FLOP per stage-2 launch at 256^3 from the committed rocprofv3 SQ
+    counter summary (64 x SQ_INSTS_VALU_FLOPS_FP64)."""
+    try:
+        with open(SQ_FILE) as f:
+            for line in f:
+                if line.startswith("stage2_fp64_flop_per_launch"):
                     return float(line.split("=")[1])
     except (OSError, ValueError):
         pass
@@ -450,7 +465,16 @@ def main():
     alg_step = ALG_BYTES_STEP * esz / 8
     achieved = alg_stage2 * chart.n_proper / s2_avg / 1e9
     step_kernel_s = sum(v[0] for v in kt.values()) / max(s2_n, 1) * 1e-3
-    traffic = _read_traffic() if args.precision == "FP64" else None
+    traffic = _read_traffic() if args.precision == "FP64" and n == 256 else None
+    flop = _read_stage2_flop() if args.precision == "FP64" and n == 256 else None
+    valu = None
+    if flop is not None:
+        tf = flop / s2_avg / 1e12
+        valu = {"bound": "fp64 vector", "achieved": tf, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS,
+                "flop_per_launch": flop,
+                "source": "FP64 FLOP per stage-2 launch from the committed SQ counter "
+                          "summary, over this run's average stage-2 launch time"}
     stepper.close()
     print(f"# main run: {elapsed / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
 
@@ -500,6 +524,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},
+        "valu_roofline": valu,
         "step_roofline": {"alg_bytes_per_cell_update": alg_step,
                           "kernel_s_per_step": step_kernel_s,
                           "frac": alg_step * chart.n_proper / step_kernel_s

@@ -358,7 +358,7 @@ __device__ __forceinline__ void face_flux(const R WL[5], const R WR[5], int d, b
     const R FSdR = sel3(d, FR[1], FR[2], FR[3]);
     const R SN = AP * SdR + AM * SdL - FSdR + FSdL;
     const R MDinv = Arith<R>::div(fmax(MD, R(0)), fmax(MD * MD, tiny));
-    const R AC = R(1) * SN * MDinv;
+    const R AC = SN * MDinv;  // M_UU_11 = 1: 1 * SN is SN exactly
     // The star state on the side of the contact speed, written once for both
     // sides: side X = L with A = AM and s = +1 when AC >= 0, else X = R with
     // A = AP and s = -1.  A + s v is AM + vL or AP - vR, and (P + s a) - s b
@@ -389,9 +389,10 @@ __device__ __forceinline__ void face_flux(const R WL[5], const R WR[5], int d, b
   const R UL[5] = {L.D, L.S1, L.S2, L.S3, L.G};
   const R UR[5] = {Rt.D, Rt.S1, Rt.S2, Rt.S3, Rt.G};
   const R den = fmax(AP + AM, tiny);
+  const R APAM = AP * AM;  // DiffusionFactor 1: (1 * AP) * AM is AP * AM exactly
 #pragma unroll
   for (int f = 0; f < 5; f++)
-    F[f] = Arith<R>::div(AP * FL[f] + AM * FR[f] - R(1) * AP * AM * (UR[f] - UL[f]), den);
+    F[f] = Arith<R>::div(AP * FL[f] + AM * FR[f] - APAM * (UR[f] - UL[f]), den);
 }
 
 // Conserved fix of ComputeFromConserved on the new state (FP64), returning

@@ -408,6 +408,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
   // FP32 keeps its 128 registers instead)
   constexpr bool ZCARRY = sizeof(R) == sizeof(double) && STAGE == 1;
   R qz[5];
+  // Stage 1 takes the RK input U of its own cell from the z-stencil values
+  // it loaded two planes earlier (raw values of planes k and k+1 carried in
+  // registers) instead of loading U again, which by then has left L2 (40 B
+  // per cell of HBM reads).  After an upload the stencil holds primitives and
+  // U is loaded.
+  constexpr bool UCARRY = STAGE == 1;
+  R rk[5], rk1[5];
   R zr[5], hr[5];  // prefetched raw values: z-stencil k+2, halo of k
   int zflip = -1, hflip = -1, hpos = 0;
   unsigned flk = 0;     // flag byte of the own cell at plane k (then k+1)
@@ -417,8 +424,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
     R Vm2[5], Vm1[5], sm1[5], WL[5], WR[5];
     load_prim<BRICK>(a, x, y, z0 - 2, Vm2);
     load_prim<BRICK>(a, x, y, z0 - 1, Vm1);
-    load_prim<BRICK>(a, x, y, z0, Vk);
-    load_prim<BRICK>(a, x, y, z0 + 1, Vk1);
+    if constexpr (UCARRY) {
+      int fl;
+      load_raw<BRICK>(a, x, y, z0, rk, &fl);
+      raw_to_prim(a, rk, fl, Vk);
+      load_raw<BRICK>(a, x, y, z0 + 1, rk1, &fl);
+      raw_to_prim(a, rk1, fl, Vk1);
+    } else {
+      load_prim<BRICK>(a, x, y, z0, Vk);
+      load_prim<BRICK>(a, x, y, z0 + 1, Vk1);
+    }
     load_raw<BRICK>(a, x, y, z0 + 2, zr, &zflip);
     flk = ld(a.flags, lin(a.n, x, y, z0));
     if constexpr (PARA) {
@@ -487,8 +502,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
     }
     if (active) {
       const unsigned ic = lin(a.n, x, y, k);
+      if (UCARRY && !a.src_is_prim) {
 #pragma unroll
-      for (int f = 0; f < 5; f++) ar[f] = ld(a.aux + f * a.fs, ic);
+        for (int f = 0; f < 5; f++) ar[f] = rk[f];
+      } else {
+#pragma unroll
+        for (int f = 0; f < 5; f++) ar[f] = ld(a.aux + f * a.fs, ic);
+      }
       if (more) load_raw<BRICK>(a, x, y, k + 3, zn, &znflip);
       if (k + 1 < nz)
         fln = ld(a.flags, lin(a.n, x, y, k + 1));
@@ -763,6 +783,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         sk[f] = sk1[f];
         Vk[f] = Vk1[f];
         Vk1[f] = Vk2[f];
+        if constexpr (UCARRY) {
+          rk[f] = rk1[f];
+          rk1[f] = zr[f];
+        }
         zr[f] = zn[f];
       }
       zflip = znflip;
