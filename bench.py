@@ -12,7 +12,10 @@ Prints one JSON line.  ``roofline`` prices the dominant kernel (the stage-2
 kernel) against HBM: algorithmic bytes per launch (120 B per cell: read Y and
 U + K1/2, write U_new, FP64, five fields) over its average duration from HIP
 events in the timed region.  ``cpu_baseline`` times the oracle (the CPU
-restatement of the reference step) on the host cores on a bounded sample.
+restatement of the reference step) on the host cores: the same
+Kelvin-Helmholtz problem at 256^3 for 5 steps, and the Sod workload
+(256 x 16 x 16) to FinishTime 0.2, with the host's CPU model, sockets, cores
+and OpenMP threads.
 ``fp32`` runs the same seeded problem at 512^3 for 200 steps with the state
 and sweeps in FP32 and in FP64, with the same time steps, and reports the
 speed ratio and the per-field L1 / Linf error of FP32 against FP64.
@@ -80,11 +83,51 @@ def _read_stage2_flop():
     return None
 
 
+def host_cpu():
+    """The host's CPU as /proc/cpuinfo and the affinity mask give it (the
+    mask of the calling thread: read it before OpenMP binds that thread)."""
+    info = {"model": None, "sockets": None, "physical_cores": None,
+            "logical_cpus": os.cpu_count(),
+            "cpus_available": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else None}
+    try:
+        phys, cores = set(), set()
+        with open("/proc/cpuinfo") as f:
+            pid = cid = None
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and info["model"] is None:
+                    info["model"] = v
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    cid = v
+                elif not k and pid is not None:
+                    phys.add(pid)
+                    cores.add((pid, cid))
+                    pid = cid = None
+        info["sockets"] = len(phys) or None
+        info["physical_cores"] = len(cores) or None
+    except OSError:
+        pass
+    return info
+
+
 def cpu_baseline(n: int, steps: int, threads: int):
-    """Oracle (CPU restatement of the reference step, OpenMP) on the host."""
+    """The oracle (the C/OpenMP restatement of the reference step, the CPU
+    path this build has: the reference's Fortran needs MPI and Silo modules
+    the image lacks) on the host's cores: the Kelvin-Helmholtz workload at
+    n^3 (steps timed after one untimed step) and the Sod workload (256 x 16 x
+    16, outflow, CFL 0.8) to FinishTime 0.2."""
+    # OpenMP placement as the plan for the reference's CPU runs sets it; read
+    # when the oracle's OpenMP runtime starts
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle
     from genasis_amd import problems
     from genasis_amd.chart import Chart
+    from genasis_amd.integrator import Integrator_C_PS, IntegratorParameters
 
     chart = Chart((n, n, n))
     o = oracle.OracleFluid(chart, courant=0.8, n_threads=threads)
@@ -97,7 +140,31 @@ def cpu_baseline(n: int, steps: int, threads: int):
         o.step((s + 2) / 100.0 * c)
     t1 = time.perf_counter()
     o.close()
-    return chart.n_proper * steps / (t1 - t0)
+    kh = chart.n_proper * steps / (t1 - t0)
+
+    class _Stepper:
+        def __init__(self, o):
+            self.o = o
+
+        def time_step_candidate(self):
+            return self.o.time_step_candidate()[0]
+
+        def step(self, time_, dt):
+            self.o.step(dt)
+
+    sod_chart = Chart((256, 16, 16), bc=(("OUTFLOW", "OUTFLOW"),) * 3)
+    o = oracle.OracleFluid(sod_chart, courant=0.8, n_threads=threads)
+    o.set_primitive(*problems.sod(sod_chart))
+    it = Integrator_C_PS(_Stepper(o), IntegratorParameters(FinishTime=0.2, nWrite=1))
+    t0 = time.perf_counter()
+    it.evolve()
+    t1 = time.perf_counter()
+    o.close()
+    sod = sod_chart.n_proper * it.s.iCycle / (t1 - t0)
+    return kh, t1 - t0, {"value": sod, "unit": "cell-updates/s", "cycles": it.s.iCycle,
+                         "seconds": t1 - t0,
+                         "workload": "Sod 256x16x16 outflow, CFL 0.8, to FinishTime 0.2 "
+                                     "(integrator with the 100-cycle ramp)"}
 
 
 def _ramped(candidate: float, cycle: int, n_ramp: int = 100) -> float:
@@ -300,6 +367,12 @@ def adjoint_check(n: int, steps: int, every: int, n_dirs: int, seed: int):
     plain_forward()  # warm-up
     t_fwd = min(plain_forward() for _ in range(2))
     adj = DiscreteAdjoint(st)
+    # warm-up, as for the plain forward: the first run allocates the
+    # checkpoints and segment buffers, later runs of the same shape reuse them
+    st.set_primitive(*prim)
+    adj.forward(steps, every)
+    adj.kinetic_energy(component=1, lo=0.2, hi=0.8, keep_on_device=True)
+    adj.backward(None, fetch=False)
     st.set_primitive(*prim)
     st.sync()
     t0 = time.perf_counter()
@@ -346,8 +419,15 @@ def adjoint_check(n: int, steps: int, every: int, n_dirs: int, seed: int):
         "backward_ms": (t2 - t1) * 1e3,
         "gradient_to_host_ms": (t3 - t2) * 1e3,
         "adjoint_over_forward": (t2 - t1) / t_fwd,
+        "gradient_over_forward": ((t1 - t0) + (t2 - t1)) / t_fwd,
         "fd_step": h,
         "fd_directions": n_dirs,
+        "fd_direction_kind": "a*inner + b*(1-inner), a, b standard normal, inner = the "
+                             "indicator of the dense layer |y-0.5| < 0.25: the step is not "
+                             "differentiable at the limiter's ties inside the uniform layers, "
+                             "so a direction that varies cell by cell there has no derivative; "
+                             "the tests check arbitrary directions against torch autograd of "
+                             "the step's FP64 transcription at 32^3",
         "fd_max_rel_err": max(rel) if rel else None,
         "fd_gamma_rel_err": abs(fdg - gb) / abs(gb) if fdg is not None else None,
     }
@@ -393,8 +473,8 @@ def main():
     ap.add_argument("--z-chunk", type=int, default=0,
                     help="stage z-chunk; 0 keeps the library's default for the chart")
     ap.add_argument("--cpu-baseline", choices=("auto", "off"), default="auto")
-    ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--precision", choices=("FP64", "FP32"), default="FP64")
@@ -497,12 +577,25 @@ def main():
     if args.cpu_baseline == "auto":
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) \
             or os.cpu_count() or 1
-        cpu_value = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
-        cpu = {"value": cpu_value, "unit": "cell-updates/s", "cores": threads,
+        # read before the oracle's OpenMP runtime binds this thread to one core
+        host = host_cpu()
+        t0 = time.perf_counter()
+        kh_value, _, sod = cpu_baseline(args.cpu_n, args.cpu_steps, threads)
+        cpu = {"value": kh_value, "unit": "cell-updates/s", "cores": threads,
                "kind": "port",
-               "sample": (f"oracle (C/OpenMP restatement of the reference step) on "
-                          f"the same KH problem at {args.cpu_n}^3, {args.cpu_steps} "
-                          f"timed RK2 steps after 1 untimed step, {threads} threads")}
+               "sample": (f"oracle (C/OpenMP restatement of the reference step, gcc -O2 "
+                          f"-ffp-contract=off) on the same Kelvin-Helmholtz problem at "
+                          f"{args.cpu_n}^3, {args.cpu_steps} timed RK2 steps after 1 untimed "
+                          f"step, {threads} OpenMP threads (OMP_PROC_BIND "
+                          f"{os.environ.get('OMP_PROC_BIND')}, OMP_PLACES "
+                          f"{os.environ.get('OMP_PLACES')})"),
+               "omp_threads": threads,
+               "host": host,
+               "sod": sod,
+               "seconds_total": time.perf_counter() - t0,
+               "gpu_over_cpu": value / kh_value if args.cpu_n == n else None}
+        print(f"# cpu baseline: {kh_value:.3e} cell-updates/s KH, {sod['value']:.3e} Sod",
+              file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,

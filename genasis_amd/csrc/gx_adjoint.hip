@@ -840,6 +840,21 @@ static int recompute_segment(gx_ctx *c, int si) {
 
 static int ensure_state(gx_ctx *c, int nsteps, int every) {
   if (c->fp32) return fail(GX_E_ARG, "gx_adjoint: only Precision = GX_FP64");
+  if (c->adj && c->adj->nsteps == nsteps && c->adj->every == every) {
+    // a run of the same shape reuses the buffers (the tails of the partials
+    // stay 0; argmax is set up again in case a reverse pass stopped early)
+    AdjointState *A = c->adj;
+    const int init[2] = {0x7fffffff, 0};
+    HIP_TRY(hipMemcpy(A->argmax, init, sizeof(init), hipMemcpyHostToDevice));
+    A->dt.clear();
+    A->ts.clear();
+    A->invdt.clear();
+    A->ramp.clear();
+    for (SegmentBuffers &b : A->sb) b.held = -1;
+    A->seed_ready = A->bar_ready = A->forward_done = false;
+    HIP_TRY(hipStreamSynchronize(A->side));
+    return GX_OK;
+  }
   release_adjoint(c);
   AdjointState *A = new (std::nothrow) AdjointState();
   if (!A) return fail(GX_E_NOMEM, "gx_adjoint: host allocation failed");

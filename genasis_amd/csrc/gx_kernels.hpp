@@ -128,9 +128,11 @@ __device__ __forceinline__ void prim_from_cons<double>(const double U[5], double
 // in double; E = G - S.V/2 is formed in float, and where it lies within the
 // float evaluation's error bound of zero -- 2^-18 (|G| + S.V/2), some 60
 // times the bound of the three roundings of S/(M D) and the five of the sum
-// -- it is formed again in double, whose sign then decides the E < 0 repair.
-// Outside that band the float sign is the exact sign, so every cell takes
-// the branch FP64 arithmetic takes on the same float state.
+// -- its sign is taken in double from 2 G (M D) - |S|^2 (the product of two
+// floats is exact in double, so only the last three roundings remain).
+// Outside that band the float sign is the exact sign.  A cell whose E the
+// double test finds negative is repaired (E = 0); otherwise E is the float
+// value, clamped at 0.
 template <>
 __device__ __forceinline__ void prim_from_cons<float>(const float U[5], float mass,
                                                       double nmin, float V[5]) {
@@ -145,10 +147,9 @@ __device__ __forceinline__ void prim_from_cons<float>(const float U[5], float ma
     float E = U[4] - ke;
     if (!(fabsf(E) > 0x1p-18f * (fabsf(U[4]) + ke))) {
       const double MD = (double)mass * (double)D;
-      const double v1 = (double)U[1] / MD, v2 = (double)U[2] / MD, v3 = (double)U[3] / MD;
-      const double e =
-          (double)U[4] - 0.5 * (((double)U[1] * v1 + (double)U[2] * v2) + (double)U[3] * v3);
-      E = e < 0.0 ? -1.0f : (float)e;
+      const double s2 =
+          ((double)U[1] * (double)U[1] + (double)U[2] * (double)U[2]) + (double)U[3] * (double)U[3];
+      E = 2.0 * (double)U[4] * MD - s2 < 0.0 ? 0.0f : fmaxf(E, 0.0f);
     }
     V[4] = E < 0.0f ? 0.0f : E;
   } else {

@@ -63,7 +63,8 @@ struct StageArgs {
   double dt;
   Physics ph;
   const R *src;         // stage input: conserved (or primitive) state
-  const R *aux;         // stage 1: U (Solution); stage 2: W = U + K1/2
+  const R *aux;         // stage 1: U (Solution), the same buffer as src unless src
+                        // holds primitives; stage 2: W = U + K1/2
   R *out1;              // stage 1: Y = U + K1; stage 2: U_new
   R *out2;              // stage 1: W; stage 2: the state before the fix, if set
   const R *geo;
@@ -82,9 +83,15 @@ struct StageArgs {
 };
 
 // Waves per SIMD the stage kernel is compiled for: the FP32 tile uses half
-// the registers and LDS of the FP64 one.
-template <typename R> struct StageOccupancy { static constexpr int waves = 2; };
-template <> struct StageOccupancy<float> { static constexpr int waves = 3; };
+// the registers and LDS of the FP64 one.  The single-brick LINEAR stage fits
+// the 128 registers of four FP32 waves without spilling; the brick, batch and
+// PARABOLIC variants carry more state and keep three (about 168 registers).
+template <typename R, bool LEAN = false> struct StageOccupancy {
+  static constexpr int waves = 2;
+};
+template <bool LEAN> struct StageOccupancy<float, LEAN> {
+  static constexpr int waves = LEAN ? 4 : 3;
+};
 
 // Loads and stores at a 32-bit element offset from a base pointer: the byte
 // offset stays 32-bit (charts hold < 2^29 cells, checked at creation), so the
@@ -413,7 +420,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
   // registers) instead of loading U again, which by then has left L2 (40 B
   // per cell of HBM reads).  After an upload the stencil holds primitives and
   // U is loaded.
-  constexpr bool UCARRY = STAGE == 1;
+  // (FP64 only: FP32 keeps the 128 registers of four waves per SIMD)
+  constexpr bool UCARRY = STAGE == 1 && sizeof(R) == sizeof(double);
   R rk[5], rk1[5];
   R zr[5], hr[5];  // prefetched raw values: z-stencil k+2, halo of k
   int zflip = -1, hflip = -1, hpos = 0;
@@ -817,7 +825,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
 
 template <typename R, int STAGE, bool REC = false, bool BRICK = false,
           typename T = Tile16x16>
-__global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void stage_kernel(
+__global__ __launch_bounds__(NT, (StageOccupancy<R, !BRICK>::waves)) void stage_kernel(
     StageArgs<R> a) {
   __shared__ StageShared<R, T> L;
   const int tile = tile_of_block(blockIdx.x, a.nblk);

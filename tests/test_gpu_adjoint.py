@@ -167,3 +167,31 @@ def test_segment_lengths(every):
     assert abs(loss - J) <= 1e-14 * abs(J)
     np.testing.assert_allclose(dN, gP[0], rtol=1e-9, atol=1e-12 * np.abs(gP[0]).max())
     assert abs(dg - gg) <= 1e-9 * abs(gg), (dg, gg)
+
+
+def test_repeated_runs_reuse_or_replace_buffers():
+    """Runs of the same shape on one context reuse its checkpoint and segment
+    buffers, runs of another shape replace them; each gradient equals that of
+    a fresh context bit for bit."""
+    chart = Chart((12, 10, 8))
+    prim = problems.kelvin_helmholtz(chart, seed=8, padded=False)
+
+    def grad(adj, st, n_steps, every):
+        st.set_primitive(*prim)
+        adj.forward(n_steps, every)
+        loss, ub = adj.kinetic_energy()
+        pb, gb = adj.backward(ub)
+        return loss, pb, gb
+
+    fresh = {}
+    for shape in [(6, 2), (5, 3)]:
+        st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8))
+        fresh[shape] = grad(DiscreteAdjoint(st), st, *shape)
+        st.close()
+    st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8))
+    adj = DiscreteAdjoint(st)
+    for shape in [(6, 2), (6, 2), (5, 3), (6, 2)]:
+        loss, pb, gb = grad(adj, st, *shape)
+        f_loss, f_pb, f_gb = fresh[shape]
+        assert loss == f_loss and gb == f_gb and np.array_equal(pb, f_pb), shape
+    st.close()

@@ -105,3 +105,35 @@ def test_fp32_state_round_trip():
     want = np.stack(cons).astype(np.float32).astype(np.float64)
     assert np.array_equal(g.conserved(), want)
     assert np.array_equal(g.primitive(), np.stack(prim).astype(np.float32).astype(np.float64))
+
+
+def test_fp32_error_bound_at_512_cubed_200_steps():
+    """The error bound at the precision workload's own size: the seeded
+    Kelvin-Helmholtz problem at 512^3, 200 RK2 steps with the FP64 run's time
+    steps (ramped CFL) for both precisions; density L1 <= 1e-5 and every
+    field's L1 error stated against the FP64 state (bit-identical to the
+    oracle's)."""
+    n, steps = 512, 200
+    chart = Chart((n, n, n), bc=PER)
+    prim = problems.kelvin_helmholtz(chart, seed=1234, padded=False)
+    c64, dts, states = [], [], {}
+    for prec in ("FP64", "FP32"):
+        st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, Precision=prec))
+        st.set_primitive(*prim)
+        for s in range(steps):
+            c = st.time_step_candidate()
+            if prec == "FP64":
+                c64.append(c)
+                dts.append(_ramped(c, s))
+            else:  # FP64 CFL term of the float state
+                assert abs(c - c64[s]) <= 1e-5 * c64[s], (s, c, c64[s])
+            st.step(0.0, dts[s])
+        states[prec] = st.conserved()
+        st.close()
+        print(f"{prec}: {steps} steps at {n}^3 done", flush=True)
+    del prim
+    err = field_errors(states["FP32"], states["FP64"])
+    print({k: (v["L1"], v["Linf"]) for k, v in err.items()}, flush=True)
+    assert err["D"]["L1"] <= DENSITY_L1, err["D"]
+    for name, e in err.items():
+        assert e["L1"] <= 1e-6 and e["Linf"] <= 1e-4, (name, e)
