@@ -19,9 +19,14 @@ and OpenMP threads.
 ``fp32`` runs the same seeded problem at 512^3 for 200 steps with the state
 and sweeps in FP32 and in FP64, with the same time steps, and reports the
 speed ratio and the per-field L1 / Linf error of FP32 against FP64.
+``parabolic`` runs the 256^3 problem with PARABOLIC reconstruction beside
+LINEAR.  ``bricks`` runs it on one brick and split into 2x2x1, 2x2x2 and
+4x4x2 bricks in the GPU's memory (halo exchange per stage) with the same
+time steps, and reports the speed of each and whether the final states are
+bit-identical.
 ``batch`` advances an ensemble of 256 seeded 3-D Riemann problems (cubic
 edges log-uniform in [32, 128], outflow, FP64) for 50 steps with the batch
-entry points (autotuned z-chunks, HIP graph of the cycles) and, for
+entry points (autotuned tile shapes and z-chunks, HIP graph of the cycles) and, for
 comparison, the same problems one after another through single-chart
 contexts driven by the host integrator.
 ``adjoint`` takes the gradient of the vy kinetic energy in 0.2 < y < 0.8
@@ -266,6 +271,56 @@ def precision_check(n: int, steps: int, z_chunk: int):
     }
 
 
+def bricks_check(n: int, steps: int, splits):
+    """The seeded Kelvin-Helmholtz problem on one brick and split into bricks
+    in the GPU's memory (halo exchange per stage), with the one-brick run's
+    time steps: speed of each and whether the final states are identical."""
+    import numpy as np
+    from genasis_amd import problems
+    from genasis_amd.chart import Chart
+    from genasis_amd.fluid import FluidParameters, Step_RK2_C_ASC
+
+    chart = Chart((n, n, n))
+    prim = problems.kelvin_helmholtz(chart, seed=1234, padded=False)
+    dts, runs, ref = [], {}, None
+    for split in [None] + list(splits):
+        st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8), n_bricks=split)
+        st.set_primitive(*prim)
+        for s in range(2):  # warm-up, then the timed steps from the same state
+            c = st.time_step_candidate()
+            st.step(0.0, 1e-3 * c)
+        st.set_primitive(*prim)
+        st.sync()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            c = st.time_step_candidate()
+            if split is None:
+                dts.append(_ramped(c, s))
+            st.step(0.0, dts[s])
+        st.sync()
+        t1 = time.perf_counter()
+        u = st.conserved()
+        st.close()
+        key = "x".join(map(str, split)) if split else "1x1x1"
+        if ref is None:
+            ref = u
+        runs[key] = {"ms_per_step": (t1 - t0) / steps * 1e3,
+                     "bitwise_equal_to_one_brick": bool(np.array_equal(u.view(np.uint64),
+                                                                       ref.view(np.uint64)))}
+        print(f"# bricks {key}: {runs[key]['ms_per_step']:.3f} ms/step", file=sys.stderr,
+              flush=True)
+    one = runs["1x1x1"]["ms_per_step"]
+    for v in runs.values():
+        v["time_over_one_brick"] = v["ms_per_step"] / one
+    return {
+        "workload": f"Kelvin-Helmholtz {n}^3 periodic, seed 1234, {steps} RK2 steps FP64 "
+                    "(after an untimed warm-up), one brick and split into bricks in one "
+                    "GPU's HBM with a two-layer halo copy per stage; the one-brick run's "
+                    "time steps for all",
+        "runs": runs,
+    }
+
+
 def batch_check(n_problems: int, steps: int, seed: int, z_chunk: int, sequential: bool):
     """Ensemble of Riemann problems: batch throughput and the sequential one."""
     from genasis_amd import problems
@@ -488,6 +543,8 @@ def main():
     ap.add_argument("--adjoint-check", choices=("auto", "off"), default="auto")
     ap.add_argument("--parabolic-check", choices=("auto", "off"), default="auto")
     ap.add_argument("--parabolic-steps", type=int, default=30)
+    ap.add_argument("--bricks-check", choices=("auto", "off"), default="auto")
+    ap.add_argument("--bricks-steps", type=int, default=20)
     ap.add_argument("--adjoint-n", type=int, default=128)
     ap.add_argument("--adjoint-steps", type=int, default=50)
     ap.add_argument("--adjoint-fd-directions", type=int, default=8)
@@ -564,6 +621,9 @@ def main():
     parabolic = None
     if args.parabolic_check == "auto":
         parabolic = parabolic_check(args.n, args.parabolic_steps, args.z_chunk)
+    bricks = None
+    if args.bricks_check == "auto":
+        bricks = bricks_check(args.n, args.bricks_steps, [(2, 2, 1), (2, 2, 2), (4, 4, 2)])
     adjoint = None
     if args.adjoint_check == "auto":
         adjoint = adjoint_check(args.adjoint_n, args.adjoint_steps, 10,
@@ -630,6 +690,7 @@ def main():
         "batch": batch,
         "adjoint": adjoint,
         "parabolic": parabolic,
+        "bricks": bricks,
     }
     print(json.dumps(out))
 

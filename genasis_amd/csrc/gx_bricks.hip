@@ -75,6 +75,9 @@ struct BrickSet {
   double *tgeo = nullptr;
   double *cur = nullptr;  // download staging: GX_N_FIELDS x the largest brick
   int periodic = 0;       // bit d: the chart is periodic along d
+  // Stage and shock-flag launches over groups of bricks: the z-chunk of the
+  // shock-flag tiles, chosen for all bricks together.
+  int ftz = 0;  // 0: not chosen yet
 };
 
 // The chart-level view the halo kernels need.
@@ -263,6 +266,58 @@ static ChartView chart_view(const gx_ctx *c) {
   return cv;
 }
 
+// Up to BRICKS_PER_LAUNCH bricks of one stage (or of the shock flags) in one
+// launch: their arguments travel by value in the kernel-argument segment, a
+// brick's tiles follow the previous brick's (tstart), and the launch's tiles
+// are dealt to the XCDs as the single-chart kernel deals one chart's, so
+// small bricks fill the chip together.  A block reads its brick's arguments
+// in place (a reference into the argument segment, loaded as scalars), as
+// the single-chart kernel reads its own.
+constexpr int BRICKS_PER_LAUNCH = 8;
+
+template <typename R> struct BrickLaunch {
+  StageArgs<R> a[BRICKS_PER_LAUNCH];
+  int tstart[BRICKS_PER_LAUNCH + 1];
+  int count;  // bricks in this launch
+  int ntile;  // tiles in this launch (tstart[count])
+};
+
+// The brick of this launch whose run of tiles holds tile t (uniform).
+template <typename R>
+__device__ __forceinline__ int launch_brick(const BrickLaunch<R> &L, int t) {
+  int b = 0;
+#pragma unroll
+  for (int i = 1; i < BRICKS_PER_LAUNCH; i++) b += (i < L.count && t >= L.tstart[i]) ? 1 : 0;
+  return b;
+}
+
+template <typename R, int STAGE, typename T>
+__global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void bricks_stage_kernel(
+    BrickLaunch<R> L) {
+  __shared__ StageShared<R, T> S;
+  const int t = tile_of_block(blockIdx.x, L.ntile);
+  const int b = launch_brick(L, t);
+  const StageArgs<R> &a = L.a[b];
+  const int lt = t - L.tstart[b];
+  stage_tile<R, STAGE, RECON_LINEAR, false, true, T>(a, (lt % a.nbx) * T::X,
+                                                     ((lt / a.nbx) % a.nby) * T::Y,
+                                                     (lt / (a.nbx * a.nby)) * a.tz, S);
+}
+
+// The shock flags of the launch's bricks (16 x 16 tiles of a.tz planes),
+// written to each brick's flag array (a.flags).
+template <typename R>
+__global__ __launch_bounds__(NT) void bricks_flags_kernel(BrickLaunch<R> L) {
+  __shared__ FlagShared<R> S;
+  const int t = tile_of_block(blockIdx.x, L.ntile);
+  const int b = launch_brick(L, t);
+  const StageArgs<R> &a = L.a[b];
+  const int fnbx = (a.n[0] + TX - 1) / TX, fnby = (a.n[1] + TY - 1) / TY;
+  const int lt = t - L.tstart[b];
+  shock_flags_tile<R, true>(a, (lt % fnbx) * TX, ((lt / fnbx) % fnby) * TY,
+                            (lt / (fnbx * fnby)) * a.tz, const_cast<uint8_t *>(a.flags), S);
+}
+
 template <typename R>
 static StageArgs<R> brick_args(const gx_ctx *c, int b) {
   const BrickSet &S = *c->bricks;
@@ -284,6 +339,73 @@ static StageArgs<R> brick_args(const gx_ctx *c, int b) {
   a.nby = (B.n[1] + tile_y(shape) - 1) / tile_y(shape);
   a.nblk = a.nbx * a.nby * ((B.n[2] + c->tz - 1) / c->tz);
   return a;
+}
+
+// Launches of BrickLaunch groups: set(a, b) completes brick b's arguments
+// (buffers, dt, ...), tiles(a) gives its tile count.
+template <typename R, typename Set, typename Tiles, typename Launch>
+static int launch_groups(gx_ctx *c, Set set, Tiles tiles, Launch launch) {
+  BrickSet &S = *c->bricks;
+  for (int g0 = 0; g0 < S.count; g0 += BRICKS_PER_LAUNCH) {
+    BrickLaunch<R> L;
+    memset(&L, 0, sizeof(L));
+    L.count = std::min(BRICKS_PER_LAUNCH, S.count - g0);
+    long long sum = 0;
+    for (int i = 0; i < L.count; i++) {
+      L.a[i] = brick_args<R>(c, g0 + i);
+      set(L.a[i], g0 + i);
+      L.tstart[i] = (int)sum;
+      sum += tiles(L.a[i]);
+    }
+    if (sum >= (1ll << 31))
+      return fail(GX_E_ARG, "gx_step_rk2: too many tiles in one launch of bricks");
+    for (int i = L.count; i <= BRICKS_PER_LAUNCH; i++) L.tstart[i] = (int)sum;
+    L.ntile = (int)sum;
+    if (L.ntile > 0) {
+      launch(L);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  return GX_OK;
+}
+
+// One RK stage of every brick: set(a, b) gives the stage's buffers and dt.
+template <typename R, int STAGE, typename Set>
+static int launch_bricks_stage(gx_ctx *c, Set set) {
+  const int shape = stage_shape(c);
+  auto tiles = [](const StageArgs<R> &a) { return (long long)a.nblk; };
+  return launch_groups<R>(c, set, tiles, [&](const BrickLaunch<R> &L) {
+    if (shape == 1)
+      bricks_stage_kernel<R, STAGE, Tile32x8><<<L.ntile, NT, 0, c->stream>>>(L);
+    else
+      bricks_stage_kernel<R, STAGE, Tile16x16><<<L.ntile, NT, 0, c->stream>>>(L);
+  });
+}
+
+// The shock flags of every brick from src (16 x 16 tiles; the z-chunk is
+// chosen for all bricks together).
+template <typename R>
+static int launch_bricks_flags(gx_ctx *c, const R *src) {
+  BrickSet &S = *c->bricks;
+  if (S.ftz == 0) {
+    long long fxy = 0;
+    for (const BrickDesc &B : S.desc)
+      fxy += (long long)((B.n[0] + TX - 1) / TX) * ((B.n[1] + TY - 1) / TY);
+    // every brick has the chart's cells divided by nBricks: one z extent
+    S.ftz = flag_tz((int)std::min(fxy, 1ll << 30), S.desc[0].n[2]);
+  }
+  auto set = [&](StageArgs<R> &a, int b) {
+    const BrickDesc &B = S.desc[b];
+    a.src = src + B.base;
+    a.tz = S.ftz;
+  };
+  auto tiles = [](const StageArgs<R> &a) {
+    return (long long)((a.n[0] + TX - 1) / TX) * ((a.n[1] + TY - 1) / TY) *
+           ((a.n[2] + a.tz - 1) / a.tz);
+  };
+  return launch_groups<R>(c, set, tiles, [&](const BrickLaunch<R> &L) {
+    bricks_flags_kernel<R><<<L.ntile, NT, 0, c->stream>>>(L);
+  });
 }
 
 template <typename R>
@@ -643,52 +765,53 @@ int bricks_step(gx_ctx *c, double dt) {
       return bf;
     };
     // Stage 1: Current = uploaded primitives or the Solution.
-    for (int b = 0; b < S.count; b++) {
+    const int prim = c->prims_valid ? 1 : 0;
+    const R *src1 = prim ? P0 : U;
+    int r = launch_bricks_stage<R, 1>(c, [&](StageArgs<R> &a, int b) {
       const BrickDesc &B = S.desc[b];
-      StageArgs<R> a = brick_args<R>(c, b);
       a.dt = dt;
-      a.src_is_prim = c->prims_valid ? 1 : 0;
-      a.src = (c->prims_valid ? P0 : U) + B.base;
+      a.src_is_prim = prim;
+      a.src = src1 + B.base;
       a.aux = U + B.base;
       a.out1 = Y + B.base;
       a.out2 = W + B.base;
-      if (stage_shape(c) == 1)
-        stage_kernel<R, 1, false, true, Tile32x8><<<a.nblk, NT, 0, st>>>(a);
-      else
-        stage_kernel<R, 1, false, true><<<a.nblk, NT, 0, st>>>(a);
+    });
+    if (r) return r;
+    for (int b = 0; b < S.count; b++) {
+      const BrickDesc &B = S.desc[b];
+      if (!B.chart_face) continue;
+      StageArgs<R> a = brick_args<R>(c, b);
+      a.src_is_prim = prim;
+      a.src = src1 + B.base;
+      BoundaryArgs Bb = BA;
+      Bb.tg = S.tgeo + B.tg_off;
+      Bb.partial = c->bpartial + (size_t)B.bpart_off * NB_SUM;
+      brick_boundary_kernel<R, 1><<<B.nbpart, TALLY_BLOCK, 0, st>>>(a, Bb, faces(B));
       HIP_TRY(hipGetLastError());
-      if (B.chart_face) {
-        BoundaryArgs Bb = BA;
-        Bb.tg = S.tgeo + B.tg_off;
-        Bb.partial = c->bpartial + (size_t)B.bpart_off * NB_SUM;
-        brick_boundary_kernel<R, 1><<<B.nbpart, TALLY_BLOCK, 0, st>>>(a, Bb, faces(B));
-        HIP_TRY(hipGetLastError());
-      }
     }
-    int r = exchange_halo<R>(c, Y);
+    r = exchange_halo<R>(c, Y);
     if (r) return r;
     HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), st));
     HIP_TRY(hipMemsetAsync(c->bpartial, 0, sizeof(double) * NB_SUM * c->n_bpartial, st));
     // Stage 2: Current = Y.
-    for (int b = 0; b < S.count; b++) {
+    r = launch_bricks_stage<R, 2>(c, [&](StageArgs<R> &a, int b) {
       const BrickDesc &B = S.desc[b];
-      StageArgs<R> a = brick_args<R>(c, b);
       a.dt = dt;
       a.src = Y + B.base;
       a.aux = W + B.base;
       a.out1 = U + B.base;
-      if (stage_shape(c) == 1)
-        stage_kernel<R, 2, false, true, Tile32x8><<<a.nblk, NT, 0, st>>>(a);
-      else
-        stage_kernel<R, 2, false, true><<<a.nblk, NT, 0, st>>>(a);
+    });
+    if (r) return r;
+    for (int b = 0; b < S.count; b++) {
+      const BrickDesc &B = S.desc[b];
+      if (!B.chart_face) continue;
+      StageArgs<R> a = brick_args<R>(c, b);
+      a.src = Y + B.base;
+      BoundaryArgs Bb = BA;
+      Bb.tg = S.tgeo + B.tg_off;
+      Bb.partial = c->bpartial + (size_t)B.bpart_off * NB_SUM;
+      brick_boundary_kernel<R, 2><<<B.nbpart, TALLY_BLOCK, 0, st>>>(a, Bb, faces(B));
       HIP_TRY(hipGetLastError());
-      if (B.chart_face) {
-        BoundaryArgs Bb = BA;
-        Bb.tg = S.tgeo + B.tg_off;
-        Bb.partial = c->bpartial + (size_t)B.bpart_off * NB_SUM;
-        brick_boundary_kernel<R, 2><<<B.nbpart, TALLY_BLOCK, 0, st>>>(a, Bb, faces(B));
-        HIP_TRY(hipGetLastError());
-      }
     }
     boundary_tally_finish<<<1, TALLY_BLOCK, 0, st>>>(c->bpartial, c->n_bpartial,
                                                      c->tally_local);
@@ -697,17 +820,8 @@ int bricks_step(gx_ctx *c, double dt) {
     if (r) return r;
     // Shock flags of the next step from the new state, then the flags of
     // the faces beyond each brick.
-    for (int b = 0; b < S.count; b++) {
-      const BrickDesc &B = S.desc[b];
-      StageArgs<R> a = brick_args<R>(c, b);
-      a.src = U + B.base;
-      a.nbx = (B.n[0] + TX - 1) / TX;  // the shock-flag tile is 16 x 16
-      a.nby = (B.n[1] + TY - 1) / TY;
-      a.tz = flag_tz(a.nbx * a.nby, B.n[2]);
-      a.nblk = a.nbx * a.nby * ((B.n[2] + a.tz - 1) / a.tz);
-      shock_flags_kernel<R, true><<<a.nblk, NT, 0, st>>>(a, S.flags + B.fbase);
-      HIP_TRY(hipGetLastError());
-    }
+    r = launch_bricks_flags<R>(c, (const R *)U);
+    if (r) return r;
     return exchange_flag_halo(c);
   });
   if (rc) return rc;

@@ -135,14 +135,25 @@ __device__ __forceinline__ int map_coord(int c, int n, int bc_in, int bc_out,
 // block of two layers (layer 0 next to the brick) by the two transverse
 // extents with two ghost cells on each side (edges included), the layer
 // fastest, then the lower, then the higher transverse dimension.
+// v[i] for a run-time i, as a chain of selects over constant indices: the
+// arguments of a kernel that builds them in registers (one launch over many
+// bricks) then stay in registers instead of going to scratch memory.
+template <int N>
+__device__ __forceinline__ int pick(const int (&v)[N], int i) {
+  int r = v[0];
+#pragma unroll
+  for (int k = 1; k < N; k++) r = i == k ? v[k] : r;
+  return r;
+}
+
 template <typename A>
 __device__ __forceinline__ unsigned halo_index(const A &a, int b, int cx, int cy, int cz) {
   const int d = b >> 1;
   const int cd = d == 0 ? cx : (d == 1 ? cy : cz);
   const int cj = d == 0 ? cy : cx, ck = d == 2 ? cy : cz;
   const int nj = d == 0 ? a.n[1] : a.n[0];
-  const int layer = (b & 1) ? cd - a.n[d] : -1 - cd;
-  return (unsigned)a.hoff[b] + (unsigned)layer +
+  const int layer = (b & 1) ? cd - pick(a.n, d) : -1 - cd;
+  return (unsigned)pick(a.hoff, b) + (unsigned)layer +
          2u * ((unsigned)(cj + 2) + (unsigned)(nj + 4) * (unsigned)(ck + 2));
 }
 
@@ -159,7 +170,7 @@ __device__ __forceinline__ unsigned cell_index(const A &a, int x, int y, int z,
     return lin(a.n, x, y, z);
   if constexpr (BRICK) {
     const int b = x < 0 ? 0 : (x >= a.n[0] ? 1 : (y < 0 ? 2 : (y >= a.n[1] ? 3 : (z < 0 ? 4 : 5))));
-    if (a.bc[b] == BC_NEIGHBOR) return halo_index(a, b, x, y, z);
+    if (pick(a.bc, b) == BC_NEIGHBOR) return halo_index(a, b, x, y, z);
   }
   bool fx = false, fy = false, fz = false;
   const int cx = map_coord(x, a.n[0], a.bc[0], a.bc[1], &fx);
