@@ -83,9 +83,9 @@ struct StageArgs {
 };
 
 // Waves per SIMD the stage kernel is compiled for: the FP32 tile uses half
-// the registers and LDS of the FP64 one.  The single-brick LINEAR stage fits
-// the 128 registers of four FP32 waves without spilling; the brick, batch and
-// PARABOLIC variants carry more state and keep three (about 168 registers).
+// the registers and LDS of the FP64 one.  The single-chart LINEAR stage fits
+// the 128 registers of four FP32 waves without spilling; the other variants
+// are bounded at three (about 168 registers) and take four where they fit.
 template <typename R, bool LEAN = false> struct StageOccupancy {
   static constexpr int waves = 2;
 };
