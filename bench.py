@@ -678,6 +678,14 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},
         "valu_roofline": valu,
+        "hbm_measured": None if traffic is None else {
+            "bytes_per_launch": traffic,
+            "achieved_gbs": traffic / s2_avg / 1e9,
+            "frac": traffic / s2_avg / 1e9 / HBM_PEAK_GBS,
+            "amplification": traffic / (alg_stage2 * chart.n_proper),
+            "source": "stage-2 HBM bytes per launch from the committed FETCH_SIZE (x2, "
+                      "copy-calibrated) and WRITE_SIZE summary, over this run's average "
+                      "stage-2 launch time"},
         "step_roofline": {"alg_bytes_per_cell_update": alg_step,
                           "kernel_s_per_step": step_kernel_s,
                           "frac": alg_step * chart.n_proper / step_kernel_s

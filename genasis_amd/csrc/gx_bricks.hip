@@ -281,6 +281,9 @@ template <typename R> struct BrickLaunch {
   int count;  // bricks in this launch
   int ntile;  // tiles in this launch (tstart[count])
 };
+// kernel arguments are limited to 4 KB
+static_assert(sizeof(BrickLaunch<double>) <= 4096 && sizeof(BrickLaunch<float>) <= 4096,
+              "BrickLaunch must fit the kernel-argument segment");
 
 // The brick of this launch whose run of tiles holds tile t (uniform).
 template <typename R>
