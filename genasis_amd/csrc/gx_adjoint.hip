@@ -142,17 +142,10 @@ __device__ inline double cell_width(const StageArgs<double> &a, int e, int c) {
 // bars in the segment that owns its cell c.
 constexpr int LINE_OUT = 58;  // complete cells per wave
 
-// a/b as div_rcp forms it (y = RN(1/b)) without its division fallback: for
-// normal a it is div_rcp's quotient, zero numerators keep their signed zero,
-// and tiny nonzero numerators (|a| < 2^-900) may differ from the exact
-// quotient in the last bits -- a limiter branch picked at 1e-270 apart.
+// a/b exactly as the forward forms it (div_rcp, y = RN(1/b)), so that the
+// limiter picks the forward's branch.
 __device__ __forceinline__ double quot_rev(double a, double b, double y) {
-  double q = a * y;
-  double r = fma(-b, q, a);
-  q = fma(r, y, q);
-  r = fma(-b, q, a);
-  q = fma(r, y, q);
-  return a == 0.0 ? a : q;
+  return div_rcp(a, b, y);
 }
 
 // The limited slope of one cell and its reverse multipliers: s = slope(...)

@@ -333,8 +333,11 @@ int gx_kernel_times(gx_ctx *ctx, double ms[4], long long count[4]);
 
 /* Device self-test of the exactly rounded reciprocal division used for the
  * divisions by geometry constants and cell volumes: n pseudo-random operand
- * pairs, *mismatches = number of quotients whose bits differ from the IEEE
- * division.  Synchronises. */
+ * pairs (zeros, tiny, subnormal and NaN numerators among them),
+ * *mismatches = number of quotients whose bits differ from the IEEE division
+ * where the division is exact (a zero numerator, or |a| >= 2^-969 with a
+ * normal quotient) or that differ by more than one ulp elsewhere.
+ * Synchronises. */
 int gx_self_test_division(long long n, unsigned long long seed,
                           long long *mismatches);
 
