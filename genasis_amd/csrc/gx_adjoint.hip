@@ -41,6 +41,30 @@ namespace gx {
 
 constexpr int AB = 256;  // threads per block of the adjoint kernels
 
+// The reverse sweeps' work arrays (primitives and their bars on the padded
+// layout, the stage bars K1b, K2b) hold the five values of a cell together,
+// padded to six (48 bytes, three 16-byte loads): a y or z line pass, whose
+// lanes walk a line across rows or planes, then touches one cache line per
+// cell and array instead of five.
+constexpr int AOS = 6;
+
+__device__ __forceinline__ void load5(const double *base, long long cell, double v[5]) {
+  const double2 *p = reinterpret_cast<const double2 *>(base + AOS * cell);
+  const double2 a = p[0], b = p[1], c = p[2];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+  v[4] = c.x;
+}
+
+__device__ __forceinline__ void store5(double *base, long long cell, const double v[5]) {
+  double2 *p = reinterpret_cast<double2 *>(base + AOS * cell);
+  p[0] = make_double2(v[0], v[1]);
+  p[1] = make_double2(v[2], v[3]);
+  p[2] = make_double2(v[4], 0.0);
+}
+
 struct PadGeom {
   int n[3];
   int m[3];
@@ -73,8 +97,9 @@ struct AdjointState {
   hipEvent_t mark = nullptr;
   uint8_t *stepF = nullptr;    // the shock flags each step used, nsteps ncell
   double *trash = nullptr;
-  double *Vp = nullptr, *Vb = nullptr;  // primitives and their bars, 5 np
-  double *Ub = nullptr, *K1b = nullptr, *K2b = nullptr, *P0b = nullptr;  // 5 ncell
+  double *Vp = nullptr, *Vb = nullptr;  // primitives and their bars, AOS np
+  double *Ub = nullptr, *P0b = nullptr;    // 5 ncell, field slabs
+  double *K1b = nullptr, *K2b = nullptr;   // AOS ncell, cell-interleaved
   bool seed_ready = false;     // Ub holds dJ/dU_final (gx_adjoint_kinetic_energy)
   bool bar_ready = false;      // P0b holds the last reverse pass's result
   double *partials = nullptr;  // 2 per line block, both stages, three dimensions
@@ -108,8 +133,7 @@ __global__ __launch_bounds__(AB) void adj_prims_kernel(StageArgs<double> a, PadG
     const int out = (x < 0 || x >= g.n[0]) + (y < 0 || y >= g.n[1]) + (z < 0 || z >= g.n[2]);
     double V[5] = {0, 0, 0, 0, 0};
     if (out <= 1) load_prim(a, x, y, z, V);
-#pragma unroll
-    for (int f = 0; f < 5; f++) Vp[f * g.np + t] = V[f];
+    store5(Vp, t, V);
   }
 }
 
@@ -331,15 +355,15 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
     double volc = 0.0, volm = 0.0;
     const double wy = cell_width(a, 1, y), wz = cell_width(a, 2, z);
     const double A = wy * wz;
+    load5(Vp, pl + (valid ? c : 0), V);
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      V[f] = valid ? Vp[f * g.np + pl + c] : 0.0;
+      V[f] = valid ? V[f] : 0.0;
       K[f] = 0.0;
     }
     if (c >= 0 && c < n) {
       const unsigned ic = lin(a.n, c, y, z);
-#pragma unroll
-      for (int f = 0; f < 5; f++) K[f] = Kb[f * a.fs + ic];
+      load5(Kb, ic, K);
       df = (a.flags[ic] >> 0) & 1;
       volc = cell_width(a, 0, c) * wy * wz;
     }
@@ -349,8 +373,7 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
     line_vjp<0>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
     if (own) {
       // the x pass is the first to touch the interior rows: it sets them
-#pragma unroll
-      for (int f = 0; f < 5; f++) Vb[f * g.np + pl + c] = out[f];
+      store5(Vb, pl + c, out);
     }
   }
   const double s1 = block_sum(dtb, sh);
@@ -398,18 +421,20 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
     bool df = false;
     const bool interior = c >= 0 && c < n;
     const bool own = valid && lane >= 3 && lane <= 60;
+    // pv is in the padded array for every lane (q[D] = 0 off the sequence)
+    load5(Vp, pv, V);
+    // the x pass's bars of interior cells, fetched now so that their
+    // (strided) latency overlaps the sweep
+    load5(Vb, pv, Vold);
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      V[f] = valid ? Vp[f * g.np + pv] : 0.0;
+      V[f] = valid ? V[f] : 0.0;
       K[f] = 0.0;
-      // the x pass's bars of interior cells, fetched now so that their
-      // (strided) latency overlaps the sweep
-      Vold[f] = own && interior ? Vb[f * g.np + pv] : 0.0;
+      Vold[f] = own && interior ? Vold[f] : 0.0;
     }
     if (interior) {
       const unsigned ik = lin(a.n, q[0], q[1], q[2]);
-#pragma unroll
-      for (int f = 0; f < 5; f++) K[f] = Kb[f * a.fs + ik];
+      load5(Kb, ik, K);
       df = (a.flags[ik] >> D) & 1;
     }
     const double wo = cell_width(a, O, o);
@@ -424,8 +449,10 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
     double out[5];
     line_vjp<D>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
     if (own) {
+      double nv[5];
 #pragma unroll
-      for (int f = 0; f < 5; f++) Vb[f * g.np + pv] = Vold[f] + out[f];
+      for (int f = 0; f < 5; f++) nv[f] = Vold[f] + out[f];
+      store5(Vb, pv, nv);
     }
   }
   const double s1 = block_sum(dtb, sh);
@@ -459,8 +486,7 @@ __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGe
                        (int)(t / ((long long)g.n[0] * g.n[1]))};
     const long long p = plin(g, c3[0], c3[1], c3[2]);
     double V[5];
-#pragma unroll
-    for (int f = 0; f < 5; f++) V[f] = Vb[f * g.np + p];
+    load5(Vb, p, V);
     for (int d = 0; d < 3; d++) {
       const int c = c3[d], n = g.n[d];
       const int bi = a.bc[2 * d], bo = a.bc[2 * d + 1];
@@ -479,11 +505,10 @@ __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGe
       }
       for (int k = 0; k < ng; k++) {
         const long long pg = p + (long long)(gh[k] - c) * g.s[d];
+        double gv[5];
+        load5(Vb, pg, gv);
 #pragma unroll
-        for (int f = 0; f < 5; f++) {
-          const double v = Vb[f * g.np + pg];
-          V[f] += (fl[k] && f == 1 + d) ? -v : v;
-        }
+        for (int f = 0; f < 5; f++) V[f] += (fl[k] && f == 1 + d) ? -gv[f] : gv[f];
       }
     }
     double y[5] = {0, 0, 0, 0, 0}, U[5];
@@ -503,12 +528,14 @@ __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGe
       if ((unsigned long long)__double_as_longlong(tc) == bits) atomicMin(idx, (int)t);
     }
     if (K2b) {
+      double k2[5], k1[5];
+      load5(K2b, t, k2);
 #pragma unroll
       for (int f = 0; f < 5; f++) {
-        const double k2 = K2b[f * ncell + t];
-        out[f * ncell + t] = y[f] + 2.0 * k2;
-        K1b[f * ncell + t] = y[f] + k2;
+        out[f * ncell + t] = y[f] + 2.0 * k2[f];
+        k1[f] = y[f] + k2[f];
       }
+      store5(K1b, t, k1);
     } else {
       double o[5];
 #pragma unroll
@@ -522,7 +549,8 @@ __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGe
         for (int f = 0; f < 5; f++) z[f] = Zp[f * ncell + t];
         adj::fix_vjp(z, a.ph.mass, a.ph.nmin, o, zb);
 #pragma unroll
-        for (int f = 0; f < 5; f++) K2p[f * ncell + t] = 0.5 * zb[f];
+        for (int f = 0; f < 5; f++) zb[f] *= 0.5;
+        store5(K2p, t, zb);
       }
     }
   }
@@ -542,7 +570,8 @@ __global__ __launch_bounds__(AB) void adj_fix_kernel(long long ncell, double mas
     }
     adj::fix_vjp(z, mass, nmin, ub, zb);
 #pragma unroll
-    for (int f = 0; f < 5; f++) K2b[f * ncell + t] = 0.5 * zb[f];
+    for (int f = 0; f < 5; f++) zb[f] *= 0.5;
+    store5(K2b, t, zb);
   }
 }
 
@@ -614,7 +643,8 @@ __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const d
         o[f] = out[f * ncell + t];
       }
       adj::fix_vjp(z, a.ph.mass, a.ph.nmin, o, zb);
-      for (int f = 0; f < 5; f++) K2p[f * ncell + t] = 0.5 * zb[f];
+      for (int f = 0; f < 5; f++) zb[f] *= 0.5;
+      store5(K2p, t, zb);
     }
   }
   *gbar += gb;
@@ -865,7 +895,6 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   g.s[1] = g.m[0];
   g.s[2] = (long long)g.m[0] * g.m[1];
   const size_t s5 = 5 * sizeof(double) * (size_t)c->ncell;
-  const size_t p5 = 5 * sizeof(double) * (size_t)g.np;
   const int nck = (nsteps + every - 1) / every + 1;
   A->ckpt.assign(nck, nullptr);
   for (int i = 0; i < nck; i++) HIP_TRY(hipMalloc(&A->ckpt[i], s5));
@@ -876,9 +905,12 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   HIP_TRY(hipEventCreateWithFlags(&A->mark, hipEventDisableTiming));
   HIP_TRY(hipStreamCreate(&A->side));
   HIP_TRY(hipMalloc(&A->stepF, (size_t)c->ncell * nsteps));
-  for (double **p : {&A->trash, &A->Ub, &A->K1b, &A->K2b, &A->P0b})
-    HIP_TRY(hipMalloc(p, s5));
-  for (double **p : {&A->Vp, &A->Vb}) HIP_TRY(hipMalloc(p, p5));
+  for (double **p : {&A->trash, &A->Ub, &A->P0b}) HIP_TRY(hipMalloc(p, s5));
+  // cell-interleaved work arrays (AOS values per cell)
+  for (double **p : {&A->K1b, &A->K2b})
+    HIP_TRY(hipMalloc(p, AOS * sizeof(double) * (size_t)c->ncell));
+  for (double **p : {&A->Vp, &A->Vb})
+    HIP_TRY(hipMalloc(p, AOS * sizeof(double) * (size_t)g.np));
   A->part_stride = 0;
   for (int d = 0; d < 3; d++) A->part_stride = std::max(A->part_stride, line_blocks(g, d));
   // every line block writes its pair; the tails of the shorter passes stay 0
