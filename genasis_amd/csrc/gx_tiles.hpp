@@ -917,7 +917,9 @@ __global__ __launch_bounds__(256) void cfl_kernel(StageArgs<R> a) {
 // plane k-1 follow from the ring.
 constexpr int QW = TX + 3, QH = TY + 3;  // primitives of x0-2 .. x0+TX, y0-2 .. y0+TY
 constexpr int BW = TX + 2, BH = TY + 2;  // tests of x0-1 .. x0+TX, y0-1 .. y0+TY
-constexpr int FLAG_TZ = 32;              // planes per block
+constexpr int FLAG_TZ = 16;              // planes per block (at 256^3: 4096 blocks,
+                                         // 2.7 rounds of 1536; 32 left a third of a
+                                         // round, 8 or 64 were slower)
 
 // Default planes per stage block (z-chunk): 32, halved (down to 4) while a
 // launch would hold fewer than 512 blocks -- two per CU of 256, what the FP64
