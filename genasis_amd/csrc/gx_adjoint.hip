@@ -1095,6 +1095,7 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   HIP_TRY(hipMemsetAsync(A->argmax + 1, 0, sizeof(int), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->invdt_valid = false;  // the recomputation reused the CFL accumulator
+  c->cfl_pending = false;
   if (missed)
     return fail(GX_E_STATE,
                 "gx_adjoint_backward: the recomputed CFL terms did not reproduce the "

@@ -830,6 +830,7 @@ int bricks_step(gx_ctx *c, double dt) {
   if (rc) return rc;
   c->prims_valid = false;
   c->invdt_valid = true;
+  c->cfl_pending = false;
   return GX_OK;
 }
 

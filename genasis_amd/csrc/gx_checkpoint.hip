@@ -237,6 +237,7 @@ int gx_read_checkpoint(gx_ctx *c, const char *path, int mode, double *time,
     c->uploaded = true;
     c->prims_valid = h.primsValid != 0;
     c->invdt_valid = h.invdtValid != 0;
+    c->cfl_pending = false;
   }
   if (time) *time = h.time;
   if (cycle) *cycle = h.cycle;

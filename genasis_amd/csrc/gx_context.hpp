@@ -77,11 +77,22 @@ struct gx_ctx {
   double tally_total[gx::TALLY_N] = {};   // TallyTotal
   double tally_change[gx::TALLY_N] = {};  // TallyChange
   // per-kernel timing (gx_set_profiling): stage 1, stage 2, shock flags,
-  // boundary fluence and tally
+  // boundary fluence and tally.  Alternate steps record into the two sets of
+  // events, and a set is read when it comes round again (or by
+  // gx_kernel_times), so a profiled step does not wait for its own kernels.
   bool profiling = false;
-  hipEvent_t ev[6] = {};
+  hipEvent_t ev[2][6] = {};
+  bool ev_used[2] = {false, false};
+  int prof_step = 0;
   double kernel_ms[4] = {0, 0, 0, 0};
   long long kernel_n[4] = {0, 0, 0, 0};
+  // The CFL maximum a step's stage 2 left, copied to pinned host memory right
+  // after stage 2 (ev_cfl): gx_compute_time_step waits for that copy alone,
+  // not for the boundary and shock-flag kernels queued behind it, so the
+  // next step's launches overlap them.
+  unsigned long long *invdt_host = nullptr;
+  hipEvent_t ev_cfl = nullptr;
+  bool cfl_pending = false;  // invdt_host holds the current invdt (after ev_cfl)
   ChartGeometry geom;
   gx::AdjointState *adj = nullptr;  // discrete adjoint (gx_adjoint_*), on demand
   gx::BrickSet *bricks = nullptr;   // a chart split into bricks (gx_create_bricks)
@@ -104,6 +115,8 @@ struct StepRecord {
 };
 // gx_step_rk2 with an optional record (FP64 contexts only when rec is set).
 int step_rk2(gx_ctx *c, double dt, const StepRecord *rec);
+// Kernel times of the profiled step recorded in event set k, into the totals.
+int collect_times(gx_ctx *c, int k);
 }  // namespace gx
 
 template <typename R>

@@ -206,8 +206,11 @@ int gx_destroy(gx_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   gx::release_adjoint(c);
   gx::release_bricks(c);
-  for (int e = 0; e < 6; e++)
-    if (c->ev[e]) (void)hipEventDestroy(c->ev[e]);
+  for (int k = 0; k < 2; k++)
+    for (int e = 0; e < 6; e++)
+      if (c->ev[k][e]) (void)hipEventDestroy(c->ev[k][e]);
+  if (c->ev_cfl) (void)hipEventDestroy(c->ev_cfl);
+  if (c->invdt_host) (void)hipHostFree(c->invdt_host);
   (void)hipFree(c->fluence);
   (void)hipFree(c->tgeo);
   (void)hipFree(c->tally_local);
@@ -283,6 +286,8 @@ int gx_create(const gx_grid *grid, const gx_params *params, gx_ctx **out) {
   TRY_CLEAN(hipMalloc(&c->flags, (size_t)c->ncell));
   TRY_CLEAN(hipMemset(c->flags, 0, (size_t)c->ncell));
   TRY_CLEAN(hipMalloc(&c->invdt, sizeof(unsigned long long)));
+  TRY_CLEAN(hipHostMalloc((void **)&c->invdt_host, sizeof(unsigned long long)));
+  TRY_CLEAN(hipEventCreateWithFlags(&c->ev_cfl, hipEventDisableTiming));
   c->n_partial = grid_blocks(c->ncell);
   if (c->n_partial > 1024) c->n_partial = 1024;
   TRY_CLEAN(hipMalloc(&c->partial, sizeof(double) * 5 * c->n_partial));
@@ -382,8 +387,13 @@ int gx_set_profiling(gx_ctx *c, int on) {
   GX_API_BEGIN
   if (!c) return fail(GX_E_ARG, "gx_set_profiling: null context");
   if (c->bricks) return fail(GX_E_STATE, "gx_set_profiling: not available on a chart split into bricks");
-  if (on && !c->ev[0])
-    for (int e = 0; e < 6; e++) HIP_TRY(hipEventCreate(&c->ev[e]));
+  if (on && !c->ev[0][0])
+    for (int k = 0; k < 2; k++)
+      for (int e = 0; e < 6; e++) HIP_TRY(hipEventCreate(&c->ev[k][e]));
+  for (int k = 0; k < 2; k++) {  // the steps recorded so far are dropped
+    if (c->ev_used[k]) HIP_TRY(hipEventSynchronize(c->ev[k][5]));
+    c->ev_used[k] = false;
+  }
   c->profiling = on != 0;
   for (int k = 0; k < 4; k++) {
     c->kernel_ms[k] = 0.0;
@@ -396,6 +406,10 @@ int gx_set_profiling(gx_ctx *c, int on) {
 int gx_kernel_times(gx_ctx *c, double ms[4], long long count[4]) {
   GX_API_BEGIN
   if (!c || !ms || !count) return fail(GX_E_ARG, "gx_kernel_times: null argument");
+  for (int k = 0; k < 2; k++) {
+    const int rc = gx::collect_times(c, k);
+    if (rc) return rc;
+  }
   for (int k = 0; k < 4; k++) {
     ms[k] = c->kernel_ms[k];
     count[k] = c->kernel_n[k];
@@ -436,6 +450,7 @@ int gx_set_state(gx_ctx *c, const double *primitive, const double *conserved) {
   c->uploaded = true;
   c->prims_valid = true;
   c->invdt_valid = false;
+  c->cfl_pending = false;
   return GX_OK;
   GX_API_END("gx_set_state")
 }
@@ -452,6 +467,7 @@ int gx_set_primitive(gx_ctx *c, const double *primitive) {
     c->uploaded = true;
     c->prims_valid = true;
     c->invdt_valid = false;
+    c->cfl_pending = false;
     return GX_OK;
   }
   const int rc = put_values(c->stream, c->fp32, c->P0, primitive, 5 * (size_t)c->ncell);
@@ -470,6 +486,7 @@ int gx_set_primitive(gx_ctx *c, const double *primitive) {
   c->uploaded = true;
   c->prims_valid = true;
   c->invdt_valid = false;
+  c->cfl_pending = false;
   return GX_OK;
   GX_API_END("gx_set_primitive")
 }
@@ -689,8 +706,13 @@ int gx_compute_time_step(gx_ctx *c, double *candidate) {
     HIP_TRY(hipGetLastError());
   }
   unsigned long long bits = 0;
-  HIP_TRY(hipMemcpyAsync(&bits, c->invdt, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->invdt_valid && c->cfl_pending) {
+    HIP_TRY(hipEventSynchronize(c->ev_cfl));
+    bits = *c->invdt_host;
+  } else {
+    HIP_TRY(hipMemcpyAsync(&bits, c->invdt, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
   double invdt;
   memcpy(&invdt, &bits, sizeof(invdt));
   c->invdt_valid = true;
@@ -808,7 +830,13 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
   B.partial = c->bpartial;
   hipStream_t st = c->stream;
   const bool prof = c->profiling;
-  if (prof) HIP_TRY(hipEventRecord(c->ev[0], st));
+  const int slot = c->prof_step & 1;
+  hipEvent_t *ev = c->ev[slot];
+  if (prof) {
+    const int rp = collect_times(c, slot);  // the step before last, long done
+    if (rp) return rp;
+    HIP_TRY(hipEventRecord(ev[0], st));
+  }
   const int rc = by_precision(c->fp32, [&](auto tag) {
     using R = decltype(tag);
     // Stage 1: Current = uploaded primitives or the Solution.
@@ -832,13 +860,13 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
     }
     launch_stage<R, 1>(c, nblk, st, a1);
     HIP_TRY(hipGetLastError());
-    if (prof) HIP_TRY(hipEventRecord(c->ev[1], st));
+    if (prof) HIP_TRY(hipEventRecord(ev[1], st));
     // Stage-1 boundary fluxes from the stage-1 input (stage 2 overwrites the
     // Solution).
     launch_boundary<R, 1>(c, st, a1, B);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), st));
-    if (prof) HIP_TRY(hipEventRecord(c->ev[2], st));
+    if (prof) HIP_TRY(hipEventRecord(ev[2], st));
     // Stage 2: Current = Y.
     StageArgs<R> a2 = base_args<R>(c);
     a2.dt = dt;
@@ -853,7 +881,11 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
     a2.rdt = a1.rdt;
     launch_stage<R, 2>(c, nblk, st, a2);
     HIP_TRY(hipGetLastError());
-    if (prof) HIP_TRY(hipEventRecord(c->ev[3], st));
+    // the next candidate's CFL maximum, for the host as soon as it is formed
+    HIP_TRY(hipMemcpyAsync(c->invdt_host, c->invdt, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(c->ev_cfl, st));
+    if (prof) HIP_TRY(hipEventRecord(ev[3], st));
     // Stage-2 boundary fluxes (input Y, the step's flags) and the step's
     // surface integral into the boundary tally.
     launch_boundary<R, 2>(c, st, a2, B);
@@ -861,7 +893,7 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
     boundary_tally_finish<<<1, TALLY_BLOCK, 0, st>>>(c->bpartial, c->n_bpartial,
                                                      c->tally_local);
     HIP_TRY(hipGetLastError());
-    if (prof) HIP_TRY(hipEventRecord(c->ev[4], st));
+    if (prof) HIP_TRY(hipEventRecord(ev[4], st));
     // Shock flags of the next step from the new state.
     StageArgs<R> a3 = base_args<R>(c);
     a3.src = Uout;
@@ -875,19 +907,31 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
   });
   if (rc) return rc;
   if (prof) {
-    // The boundary interval holds the invdt reset, a few-byte fill.
-    HIP_TRY(hipEventRecord(c->ev[5], st));
-    HIP_TRY(hipEventSynchronize(c->ev[5]));
-    float t[5];
-    for (int k = 0; k < 5; k++) HIP_TRY(hipEventElapsedTime(&t[k], c->ev[k], c->ev[k + 1]));
-    c->kernel_ms[0] += t[0];
-    c->kernel_ms[1] += t[2];
-    c->kernel_ms[2] += t[4];
-    c->kernel_ms[3] += t[1] + t[3];
-    for (int k = 0; k < 4; k++) c->kernel_n[k] += 1;
+    HIP_TRY(hipEventRecord(ev[5], st));
+    c->ev_used[slot] = true;
+    c->prof_step++;
   }
   c->prims_valid = false;
   c->invdt_valid = true;
+  c->cfl_pending = true;
+  return GX_OK;
+}
+
+// Adds the kernel times of the step recorded in event set k (waiting for it
+// if it is still running) and frees the set.  The stage-2 interval holds the
+// CFL copy and the boundary interval the invdt reset, a few-byte fill.
+int collect_times(gx_ctx *c, int k) {
+  if (!c->ev_used[k]) return GX_OK;
+  hipEvent_t *ev = c->ev[k];
+  HIP_TRY(hipEventSynchronize(ev[5]));
+  float t[5];
+  for (int e = 0; e < 5; e++) HIP_TRY(hipEventElapsedTime(&t[e], ev[e], ev[e + 1]));
+  c->kernel_ms[0] += t[0];
+  c->kernel_ms[1] += t[2];
+  c->kernel_ms[2] += t[4];
+  c->kernel_ms[3] += t[1] + t[3];
+  for (int j = 0; j < 4; j++) c->kernel_n[j] += 1;
+  c->ev_used[k] = false;
   return GX_OK;
 }
 
