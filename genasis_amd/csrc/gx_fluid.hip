@@ -881,11 +881,11 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
     a2.rdt = a1.rdt;
     launch_stage<R, 2>(c, nblk, st, a2);
     HIP_TRY(hipGetLastError());
+    if (prof) HIP_TRY(hipEventRecord(ev[3], st));
     // the next candidate's CFL maximum, for the host as soon as it is formed
     HIP_TRY(hipMemcpyAsync(c->invdt_host, c->invdt, sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(c->ev_cfl, st));
-    if (prof) HIP_TRY(hipEventRecord(ev[3], st));
     // Stage-2 boundary fluxes (input Y, the step's flags) and the step's
     // surface integral into the boundary tally.
     launch_boundary<R, 2>(c, st, a2, B);
@@ -918,8 +918,8 @@ int step_rk2(gx_ctx *c, double dt, const StepRecord *rec) {
 }
 
 // Adds the kernel times of the step recorded in event set k (waiting for it
-// if it is still running) and frees the set.  The stage-2 interval holds the
-// CFL copy and the boundary interval the invdt reset, a few-byte fill.
+// if it is still running) and frees the set.  The boundary intervals hold
+// the invdt reset (a few-byte fill) and the 8-byte CFL copy.
 int collect_times(gx_ctx *c, int k) {
   if (!c->ev_used[k]) return GX_OK;
   hipEvent_t *ev = c->ev[k];
