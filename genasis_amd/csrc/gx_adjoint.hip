@@ -177,36 +177,46 @@ __device__ __forceinline__ double quot_rev(double a, double b, double y) {
 // the limiter's branch), and kl, kr with
 // ds = kl d(dVL) + kr d(dVR), dVL = v0 - vm, dVR = vp - v0.  Branch-free:
 // every case is formed and selected, so lanes on different limiter branches
-// do not serialise.  Multipliers use the geometry reciprocals (a gradient
-// needs no exactly rounded quotient).  Rules at ties as in the header of
-// gx_adjoint.hpp: the centred slope when it equals the smaller one-sided
-// one; where exactly one difference is zero, the mean of the two sides.
+// do not serialise.  Multipliers use the cell's geometry factors m (a
+// gradient needs no exactly rounded quotient), formed once per cell for the
+// five fields: cl = dxr^2/den, cr = dxl^2/den, tl = theta/dxl,
+// tr = theta/dxr.  Rules at ties as in the header of gx_adjoint.hpp: the
+// centred slope when it equals the smaller one-sided one; where exactly one
+// difference is zero, the mean of the two sides (tl/2 or tr/2).
+struct SlopeFactors {
+  double cl, cr, tl, tr;
+};
+
 __device__ __forceinline__ double slope_fwd_rev(double vm, double v0, double vp, double dxl,
                                                 double dxr, double dxl2, double dxr2,
                                                 double den, double rdxl, double rdxr,
                                                 double rden, double theta, int use_limiter,
-                                                double &kl, double &kr) {
+                                                const SlopeFactors &m, double &kl,
+                                                double &kr) {
   const double dVL = v0 - vm, dVR = vp - v0;
   const double c = quot_rev(dxr2 * dVL + dxl2 * dVR, den, rden);
   if (!use_limiter) {  // uniform over the launch
-    kl = dxr2 * rden;
-    kr = dxl2 * rden;
+    kl = m.cl;
+    kr = m.cr;
     return c;
   }
   const double ql = quot_rev(theta * dVL, dxl, rdxl), qr = quot_rev(theta * dVR, dxr, rdxr);
   const double tl = fabs(ql), tr = fabs(qr), ac = fabs(c);
+  // sf = sign(1/2, dVL) + sign(1/2, dVR), in {-1, 0, 1}: products with it
+  // are exact
   const double sf = copysign(0.5, dVL) + copysign(0.5, dVR);
   const double s = sf * fmin(fmin(tl, tr), ac);
   const double m1 = tl <= tr ? tl : tr;
   const bool zL = dVL == 0.0, zR = dVR == 0.0;
   const bool cen = ac <= m1, lef = !cen && tl <= tr;
-  const double factor = (signbit(dVL) ? -0.5 : 0.5) + (signbit(dVR) ? -0.5 : 0.5);
-  const double fc = factor * adj::sgn1(c);
-  double l = cen ? (fc * dxr2) * rden : (lef ? ((factor * adj::sgn1(ql)) * theta) * rdxl : 0.0);
-  double r = cen ? (fc * dxl2) * rden : (lef ? 0.0 : ((factor * adj::sgn1(qr)) * theta) * rdxr);
+  // d|q|/dq = sign(q): the branch's factor carries the sign of its quotient
+  double l = cen ? copysign(m.cl, c) : (lef ? copysign(m.tl, ql) : 0.0);
+  double r = cen ? copysign(m.cr, c) : (lef ? 0.0 : copysign(m.tr, qr));
+  l *= sf;
+  r *= sf;
   if (zL != zR) {  // exactly one difference is zero: one-sided, mean of the sides
-    l = zL ? theta * (0.5 * rdxl) : 0.0;
-    r = zL ? 0.0 : theta * (0.5 * rdxr);
+    l = zL ? 0.5 * m.tl : 0.0;
+    r = zL ? 0.0 : 0.5 * m.tr;
   }
   kl = l;
   kr = r;
@@ -215,16 +225,17 @@ __device__ __forceinline__ double slope_fwd_rev(double vm, double v0, double vp,
 
 // Neighbour lanes by DPP wavefront shifts (wave_shr:1, wave_shl:1), a VALU
 // move instead of an LDS permute: lane_up gives lane i the value of lane
-// i - 1, lane_down that of lane i + 1; the end lanes get 0 and are never
-// used (every use is guarded by the lane range).
+// i - 1, lane_down that of lane i + 1; the end lanes, which have no source
+// lane, read 0 (bound_ctrl), so no destination has to be cleared first; they
+// are never used (every use is guarded by the lane range).
 __device__ __forceinline__ double lane_up(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double lane_down(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -243,18 +254,19 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   const bool has_slope = c >= -1 && c <= n && lane >= 1 && lane <= 62;
   double kl[5], kr[5];  // the limiter's reverse multipliers of cell c
   {
-    const double dxl = has_slope ? geo(a, D, G_DXL, c + 2) : 1.0;
-    const double dxr = has_slope ? geo(a, D, G_DXR, c + 2) : 1.0;
-    const double dxl2 = has_slope ? geo(a, D, G_DXL2, c + 2) : 1.0;
-    const double dxr2 = has_slope ? geo(a, D, G_DXR2, c + 2) : 1.0;
-    const double den = has_slope ? geo(a, D, G_DEN, c + 2) : 1.0;
-    const double rdxl = has_slope ? geo(a, D, G_RDXL, c + 2) : 1.0;
-    const double rdxr = has_slope ? geo(a, D, G_RDXR, c + 2) : 1.0;
-    const double rden = has_slope ? geo(a, D, G_RDEN, c + 2) : 1.0;
+    // positions -2 .. n+1 of the geometry (n + 4 entries, finite and
+    // positive); a lane without a slope reads a clamped entry and its slope
+    // is cleared below
+    const int q = min(max(c + 2, 0), n + 3);
+    const double dxl = geo(a, D, G_DXL, q), dxr = geo(a, D, G_DXR, q);
+    const double dxl2 = geo(a, D, G_DXL2, q), dxr2 = geo(a, D, G_DXR2, q);
+    const double den = geo(a, D, G_DEN, q), rden = geo(a, D, G_RDEN, q);
+    const double rdxl = geo(a, D, G_RDXL, q), rdxr = geo(a, D, G_RDXR, q);
+    const SlopeFactors m = {dxr2 * rden, dxl2 * rden, a.ph.theta * rdxl, a.ph.theta * rdxr};
 #pragma unroll
     for (int f = 0; f < 5; f++) {
       s[f] = slope_fwd_rev(Vm[f], V[f], Vq[f], dxl, dxr, dxl2, dxr2, den, rdxl, rdxr, rden,
-                           a.ph.theta, a.ph.use_limiter, kl[f], kr[f]);
+                           a.ph.theta, a.ph.use_limiter, m, kl[f], kr[f]);
       if (!has_slope) s[f] = kl[f] = kr[f] = 0.0;
     }
   }
@@ -269,20 +281,16 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
     const double wr = geo(a, D, G_WR, c + 1), wl = geo(a, D, G_WL, c + 2);
     double WL[5], WR[5], Gf[5];
 #pragma unroll
-    for (int f = 0; f < 5; f++) {
+    for (int f = 0; f < 5; f++) {  // the face states in the forward's order
       WL[f] = Vm[f] + wr * sm[f];
       WR[f] = V[f] - wl * s[f];
-      Gf[f] = 0.0;
     }
-    if (c < n) {
-      const double r = A * adj::rcp_rev(volc);
+    {
+      GX_REV_FMA
+      const double rc = c < n ? A * adj::rcp_rev(volc) : 0.0;
+      const double rm = c >= 1 ? A * adj::rcp_rev(volm) : 0.0;
 #pragma unroll
-      for (int f = 0; f < 5; f++) Gf[f] += r * Kb[f];
-    }
-    if (c >= 1) {
-      const double r = A * adj::rcp_rev(volm);
-#pragma unroll
-      for (int f = 0; f < 5; f++) Gf[f] -= r * Kbm[f];
+      for (int f = 0; f < 5; f++) Gf[f] = rc * Kb[f] - rm * Kbm[f];
     }
     double Fb[5], F[5], gl = 0.0;
 #pragma unroll
@@ -290,6 +298,7 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
     adj::face_flux_vjp(WL, WR, D, df && c < n, a.ph.gamma, a.ph.mass, a.ph.hllc != 0, Fb, WLb,
                        WRb, gl, F);
     if (own) {
+      GX_REV_FMA
 #pragma unroll
       for (int f = 0; f < 5; f++) dtb += F[f] * Gf[f];
       gbs += gl;
@@ -303,6 +312,7 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
     dm[f] = d0[f] = dp[f] = 0.0;
   }
   if (has_slope && lane >= 2 && lane <= 61) {
+    GX_REV_FMA
     const int q = c + 2;
     const double wr = geo(a, D, G_WR, q), wl = geo(a, D, G_WL, q);
 #pragma unroll

@@ -29,6 +29,19 @@ constexpr double kSqrtTiny = 1.4916681462400413e-154;
 
 GX_HD double sgn1(double x) { return std::signbit(x) ? -1.0 : 1.0; }
 
+// On the device the face flux's reverse lets the compiler fuse a * b + c
+// into one FMA: its values are gradients, or recomputed forward values whose
+// rounding picks no branch the forward replays exactly (those already come
+// from the refined hardware reciprocal and square root below).  The limited
+// slope's quotients and the face states are formed outside these functions,
+// in the forward's order.  The host keeps separate roundings, so its
+// evaluation repeats the forward bit for bit.
+#ifdef __HIP_DEVICE_COMPILE__
+#define GX_REV_FMA _Pragma("clang fp contract(fast)")
+#else
+#define GX_REV_FMA
+#endif
+
 // 1/x for the reverse sweeps' multipliers (x positive and normal), where a
 // gradient needs no correctly rounded quotient: on the device the hardware
 // reciprocal refined by two Newton steps (within an ulp or two), on the host
@@ -60,16 +73,22 @@ GX_HD double div_rev(double a, double b, double rb) {
 #endif
 }
 
-GX_HD double sqrt_rev(double x) {  // x > 0, normal
+// sqrt(x) for x > 0 and normal; half_rcp receives 1/(2 sqrt(x)) for the
+// reverse sweep (on the device the Goldschmidt iterate, one refinement of
+// the hardware estimate).
+GX_HD double sqrt_rev(double x, double &half_rcp) {
 #ifdef __HIP_DEVICE_COMPILE__
   const double y = __builtin_amdgcn_rsq(x);
   double g = x * y, h = 0.5 * y;
   const double r = fma(-g, h, 0.5);
   g = fma(g, r, g);
   h = fma(h, r, h);
+  half_rcp = h;
   return fma(fma(-g, g, x), h, g);
 #else
-  return sqrt(x);
+  const double c = sqrt(x);
+  half_rcp = 0.5 / c;
+  return c;
 #endif
 }
 
@@ -117,6 +136,7 @@ GX_HD void slope_vjp(double vm, double v0, double vp, double dxl, double dxr,
 // ---- face state after ComputeFromPrimitiveCommon --------------------------
 struct FState {
   double N, V[3], E, P, q, CS, Nc, D, S[3], G;
+  double rMN, hCS;  // 1/(M N) and 1/(2 CS), where ok
   bool ok;
 };
 
@@ -125,6 +145,7 @@ struct FBar {
 };
 
 GX_HD void fstate(const double W[5], double gamma, double mass, FState &s) {
+  GX_REV_FMA
   s.N = W[0];
   s.V[0] = W[1];
   s.V[1] = W[2];
@@ -133,8 +154,9 @@ GX_HD void fstate(const double W[5], double gamma, double mass, FState &s) {
   s.P = (gamma - 1.0) * s.E;
   s.ok = s.N > 0.0 && s.P > 0.0;
   const double MN = s.ok ? mass * s.N : 1.0;
-  s.q = s.ok ? div_rev(gamma * s.P, MN, rcp_rev(MN)) : 0.0;
-  s.CS = s.ok ? sqrt_rev(s.q) : 0.0;
+  s.rMN = rcp_rev(MN);
+  s.q = s.ok ? div_rev(gamma * s.P, MN, s.rMN) : 0.0;
+  s.CS = s.ok ? sqrt_rev(s.ok ? s.q : 1.0, s.hCS) : 0.0;
   s.Nc = s.N < 0.0 ? 0.0 : s.N;
   s.D = s.Nc;
   for (int k = 0; k < 3; k++) s.S[k] = mass * s.Nc * s.V[k];
@@ -144,6 +166,7 @@ GX_HD void fstate(const double W[5], double gamma, double mass, FState &s) {
 // Bars of the face state -> bars of its primitive W (added) and gamma.
 GX_HD void fstate_vjp(const FState &s, FBar b, double gamma, double mass, double Wb[5],
                       double &gb) {
+  GX_REV_FMA
   b.E += b.G;
   for (int k = 0; k < 3; k++) {
     b.S[k] += 0.5 * b.G * s.V[k];
@@ -156,12 +179,11 @@ GX_HD void fstate_vjp(const FState &s, FBar b, double gamma, double mass, double
   }
   double Nb = s.N < 0.0 ? 0.0 : Ncb;
   if (s.ok && s.CS > 0.0) {
-    // q = gamma P / (M N), CS = sqrt(q)
-    const double qb = (b.CS * 0.5) * rcp_rev(s.CS);
-    const double rMN = rcp_rev(mass * s.N);
-    gb += (qb * s.P) * rMN;
-    b.P += (qb * gamma) * rMN;
-    Nb -= (qb * s.q) * rcp_rev(s.N);
+    // q = gamma P / (M N), CS = sqrt(q); dq/dN = -q/N = -q M/(M N)
+    const double qb = b.CS * s.hCS;
+    gb += (qb * s.P) * s.rMN;
+    b.P += (qb * gamma) * s.rMN;
+    Nb -= (qb * s.q) * (mass * s.rMN);
   }
   b.E += b.P * (gamma - 1.0);
   gb += b.P * s.E;
@@ -172,6 +194,7 @@ GX_HD void fstate_vjp(const FState &s, FBar b, double gamma, double mass, double
 
 GX_HD void raw(double D, const double S[3], double G, double P, double vd, int d,
                double F[5]) {
+  GX_REV_FMA
   F[0] = D * vd;
   F[1] = S[0] * vd;
   F[2] = S[1] * vd;
@@ -183,6 +206,7 @@ GX_HD void raw(double D, const double S[3], double G, double P, double vd, int d
 GX_HD void raw_vjp(double D, const double S[3], double G, double P, double vd, int d,
                    const double Fb[5], double &Db, double Sb[3], double &Gb, double &Pb,
                    double &vdb) {
+  GX_REV_FMA
   Db += Fb[0] * vd;
   vdb += Fb[0] * D;
   for (int k = 0; k < 3; k++) {
@@ -200,6 +224,7 @@ GX_HD void raw_vjp(double D, const double S[3], double G, double P, double vd, i
 GX_HD void face_flux_vjp(const double WL[5], const double WR[5], int d, bool df,
                          double gamma, double mass, bool hllc, const double Fb[5],
                          double WLb[5], double WRb[5], double &gb, double F[5]) {
+  GX_REV_FMA
   FState L, R;
   fstate(WL, gamma, mass, L);
   fstate(WR, gamma, mass, R);
