@@ -244,7 +244,7 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
                                          bool own, double A, double dt, const double V[5],
                                          const double Kb[5], bool df, double volc,
                                          double volm, double Vb[5], double &dtb,
-                                         double &gbs) {
+                                         double &gbs, double (*kst)[AB]) {
   double Vm[5], Vq[5], s[5], sm[5], Kbm[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
@@ -270,6 +270,14 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
       if (!has_slope) s[f] = kl[f] = kr[f] = 0.0;
     }
   }
+  // the multipliers wait in LDS while the face flux's reverse runs (ten
+  // registers pairs fewer at its peak)
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    kst[f][threadIdx.x] = kl[f];
+    kst[5 + f][threadIdx.x] = kr[f];
+  }
+  __asm__ volatile("" ::: "memory");
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     sm[f] = lane_up(s[f]);
@@ -292,15 +300,19 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
 #pragma unroll
       for (int f = 0; f < 5; f++) Gf[f] = rc * Kb[f] - rm * Kbm[f];
     }
+#pragma unroll
+    for (int f = 0; f < 5; f++) kst[10 + f][threadIdx.x] = Gf[f];
+    __asm__ volatile("" ::: "memory");
     double Fb[5], F[5], gl = 0.0;
 #pragma unroll
     for (int f = 0; f < 5; f++) Fb[f] = dt * Gf[f];
     adj::face_flux_vjp(WL, WR, D, df && c < n, a.ph.gamma, a.ph.mass, a.ph.hllc != 0, Fb, WLb,
                        WRb, gl, F);
+    __asm__ volatile("" ::: "memory");
     if (own) {
       GX_REV_FMA
 #pragma unroll
-      for (int f = 0; f < 5; f++) dtb += F[f] * Gf[f];
+      for (int f = 0; f < 5; f++) dtb += F[f] * kst[10 + f][threadIdx.x];
       gbs += gl;
     }
   }
@@ -310,6 +322,12 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   for (int f = 0; f < 5; f++) {
     up[f] = lane_down(WLb[f]);  // bar of c as the left state of face c+1
     dm[f] = d0[f] = dp[f] = 0.0;
+  }
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int f = 0; f < 5; f++) {
+    kl[f] = kst[f][threadIdx.x];
+    kr[f] = kst[5 + f][threadIdx.x];
   }
   if (has_slope && lane >= 2 && lane <= 61) {
     GX_REV_FMA
@@ -347,6 +365,7 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
                                                         double dt, double *Vb,
                                                         double *partial) {
   __shared__ double sh[AB];
+  __shared__ double kst[15][AB];
   const int n = g.n[0];
   const int L = n + 4;
   const long long T = (long long)L * g.n[1] * g.n[2];
@@ -380,7 +399,7 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
     if (c >= 1 && c <= n) volm = cell_width(a, 0, c - 1) * wy * wz;
     const bool own = valid && lane >= 3 && lane <= 60;
     double out[5];
-    line_vjp<0>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
+    line_vjp<0>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs, kst);
     if (own) {
       // the x pass is the first to touch the interior rows: it sets them
       store5(Vb, pl + c, out);
@@ -408,6 +427,7 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
                                                         double *partial) {
   constexpr int O = D == 1 ? 2 : 1;  // the other transverse dimension
   __shared__ double sh[AB];
+  __shared__ double kst[20][AB];
   const int n = g.n[D];
   const int L = n + 4;
   const int T = L * g.n[O];
@@ -456,12 +476,17 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
     };
     const double volc = (c >= 0 && c < n) ? vol_of(c) : 0.0;
     const double volm = (c >= 1 && c <= n) ? vol_of(c - 1) : 0.0;
+    // the x pass's bars wait in LDS during the sweep
+#pragma unroll
+    for (int f = 0; f < 5; f++) kst[15 + f][threadIdx.x] = Vold[f];
+    __asm__ volatile("" ::: "memory");
     double out[5];
-    line_vjp<D>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs);
+    line_vjp<D>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs, kst);
+    __asm__ volatile("" ::: "memory");
     if (own) {
       double nv[5];
 #pragma unroll
-      for (int f = 0; f < 5; f++) nv[f] = Vold[f] + out[f];
+      for (int f = 0; f < 5; f++) nv[f] = kst[15 + f][threadIdx.x] + out[f];
       store5(Vb, pv, nv);
     }
   }
