@@ -97,7 +97,9 @@ struct AdjointState {
   hipEvent_t mark = nullptr;
   uint8_t *stepF = nullptr;    // the shock flags each step used, nsteps ncell
   double *trash = nullptr;
-  double *Vp = nullptr, *Vb = nullptr;  // primitives and their bars, AOS np
+  double *Vp = nullptr;               // primitives of a stage input, AOS np
+  double *Vb[2] = {nullptr, nullptr}; // bars of stage 2 and stage 1, AOS np: the x
+                                      // pass of one stage folds in the other's
   double *Ub = nullptr, *P0b = nullptr;    // 5 ncell, field slabs
   double *K1b = nullptr, *K2b = nullptr;   // AOS ncell, cell-interleaved
   bool seed_ready = false;     // Ub holds dJ/dU_final (gx_adjoint_kinetic_energy)
@@ -122,9 +124,13 @@ __global__ __launch_bounds__(AB) void adj_fill_kernel(double *p, long long n, do
 }
 
 // Primitives of the stage input on the padded layout, ghosts by the ghost
-// rule (cells with two or more coordinates outside are unused and 0).
+// rule (cells with two or more coordinates outside are unused and 0).  With
+// idx given (stage 1), it also finds the first cell (x fastest) whose CFL
+// term has the bits of the step's maximum (atomicMin; *idx starts at
+// INT_MAX): the term is formed from the same primitives as the forward's.
 __global__ __launch_bounds__(AB) void adj_prims_kernel(StageArgs<double> a, PadGeom g,
-                                                       double *Vp) {
+                                                       double *Vp, unsigned long long bits,
+                                                       int *idx) {
   for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < g.np;
        t += (long long)gridDim.x * AB) {
     const int x = (int)(t % g.m[0]) - 2;
@@ -134,6 +140,12 @@ __global__ __launch_bounds__(AB) void adj_prims_kernel(StageArgs<double> a, PadG
     double V[5] = {0, 0, 0, 0, 0};
     if (out <= 1) load_prim(a, x, y, z, V);
     store5(Vp, t, V);
+    if (idx && out == 0) {
+      const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, x + 2),
+                                 geo(a, 1, G_WIDTH, y + 2), geo(a, 2, G_WIDTH, z + 2));
+      if ((unsigned long long)__double_as_longlong(tc) == bits)
+        atomicMin(idx, (int)lin(a.n, x, y, z));
+    }
   }
 }
 
@@ -355,13 +367,113 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   }
 }
 
+// The fold of a finished stage: the bar of an interior cell on the padded
+// layout plus the bars of the ghost values copied from it (normal velocity
+// negated across a REFLECTING face).
+__device__ __forceinline__ void gather_bars(const StageArgs<double> &a, const PadGeom &g,
+                                            const double *Vb, const int c3[3], long long p,
+                                            double V[5]) {
+  load5(Vb, p, V);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int c = c3[d], n = g.n[d];
+    const int bi = a.bc[2 * d], bo = a.bc[2 * d + 1];
+    int gh[2], fl[2], ng = 0;
+    if (bi == BC_PERIODIC) {
+      if (c >= n - 2) { gh[ng] = c - n; fl[ng++] = 0; }
+    } else if (c <= 1) {
+      gh[ng] = -1 - c;
+      fl[ng++] = bi == BC_REFLECTING;
+    }
+    if (bo == BC_PERIODIC) {
+      if (c <= 1) { gh[ng] = c + n; fl[ng++] = 0; }
+    } else if (c >= n - 2) {
+      gh[ng] = 2 * n - 1 - c;
+      fl[ng++] = bo == BC_REFLECTING;
+    }
+    for (int k = 0; k < ng; k++) {
+      const long long pg = p + (long long)(gh[k] - c) * g.s[d];
+      double gv[5];
+      load5(Vb, pg, gv);
+#pragma unroll
+      for (int f = 0; f < 5; f++) V[f] += (fl[k] && f == 1 + d) ? -gv[f] : gv[f];
+    }
+  }
+}
+
+// What the x pass of a stage's transpose folds in before its sweep, per
+// interior cell of its lanes (all lanes form the cell's bar K of their
+// stage; the lanes that own the cell store what they change):
+//   FOLD_SEED (the first reverse step): K2b = fix^T(Ub)/2 at the step's Z;
+//   FOLD_STAGE1 (the stage-1 transpose of the step after is finished):
+//     K2b as above of Ub + prim^T(gathered bars) at that step's input;
+//   FOLD_STAGE2 (this step's stage-2 transpose is finished): with
+//     y = prim^T(gathered bars) at Y, Ub = y + 2 K2b (Y = U + K1,
+//     W = U + K1/2, Z = W + K2/2: Yb = y, Wb = 2 K2b) and K1b = y + K2b.
+enum { FOLD_SEED = 0, FOLD_STAGE1 = 1, FOLD_STAGE2 = 2 };
+
+struct FoldIn {
+  const double *Vb;   // the finished stage's bars on the padded layout (AOS)
+  const double *src;  // that stage's input (conserved field slabs)
+  double *out;        // Ub (field slabs)
+  const double *K2b;  // FOLD_STAGE2: the stage-2 bar (AOS)
+  const double *Z;    // FOLD_SEED, FOLD_STAGE1: the state before the fix (field slabs)
+  double *Kout;       // the bar of the x pass's own stage: K1b or K2b (AOS)
+};
+
+template <int MODE>
+__device__ __forceinline__ void fold_cell(const StageArgs<double> &a, const PadGeom &g,
+                                          const FoldIn &fi, const int c3[3], long long p,
+                                          unsigned ic, bool own, double K[5]) {
+  const long long fs = (long long)g.n[0] * g.n[1] * g.n[2];
+  double o[5];
+  if (MODE != FOLD_SEED) {
+    double Vg[5], U[5], yv[5] = {0, 0, 0, 0, 0};
+    gather_bars(a, g, fi.Vb, c3, p, Vg);
+#pragma unroll
+    for (int f = 0; f < 5; f++) U[f] = fi.src[f * fs + ic];
+    adj::prim_vjp(U, a.ph.mass, a.ph.nmin, Vg, yv);
+    if (MODE == FOLD_STAGE2) {
+      double k2[5];
+      load5(fi.K2b, ic, k2);
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        o[f] = yv[f] + 2.0 * k2[f];
+        K[f] = yv[f] + k2[f];
+      }
+      if (own) {
+#pragma unroll
+        for (int f = 0; f < 5; f++) fi.out[f * fs + ic] = o[f];
+        store5(fi.Kout, ic, K);
+      }
+      return;
+    }
+    // (Ub itself is not stored: the stage-2 fold of this step replaces it)
+#pragma unroll
+    for (int f = 0; f < 5; f++) o[f] = fi.out[f * fs + ic] + yv[f];
+  } else {
+#pragma unroll
+    for (int f = 0; f < 5; f++) o[f] = fi.out[f * fs + ic];
+  }
+  double z[5];
+#pragma unroll
+  for (int f = 0; f < 5; f++) z[f] = fi.Z[f * fs + ic];
+  adj::fix_vjp(z, a.ph.mass, a.ph.nmin, o, K);
+#pragma unroll
+  for (int f = 0; f < 5; f++) K[f] *= 0.5;
+  if (own) store5(fi.Kout, ic, K);
+}
+
 // D = 0: the rows (y, z) of the padded layout, cells -2 .. n+1 each, are
 // consecutive in memory; waves take 58 positions each of that sequence
 // (lanes along x, coalesced), so a wave may hold the end of one row and the
 // start of the next.  Every use of a neighbour lane is guarded by the own
 // row's cell range, and the lanes of another row contribute zeros there.
+// Before the sweep the lanes fold in the finished stage (fold_cell), so that
+// the fold's memory traffic overlaps the sweep's arithmetic.
+template <int MODE>
 __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, PadGeom g,
-                                                        const double *Vp, const double *Kb,
+                                                        const double *Vp, FoldIn fi,
                                                         double dt, double *Vb,
                                                         double *partial) {
   __shared__ double sh[AB];
@@ -390,14 +502,15 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
       V[f] = valid ? V[f] : 0.0;
       K[f] = 0.0;
     }
+    const bool own = valid && lane >= 3 && lane <= 60;
     if (c >= 0 && c < n) {
       const unsigned ic = lin(a.n, c, y, z);
-      load5(Kb, ic, K);
+      const int c3[3] = {c, y, z};
+      fold_cell<MODE>(a, g, fi, c3, pl + c, ic, own, K);
       df = (a.flags[ic] >> 0) & 1;
       volc = cell_width(a, 0, c) * wy * wz;
     }
     if (c >= 1 && c <= n) volm = cell_width(a, 0, c - 1) * wy * wz;
-    const bool own = valid && lane >= 3 && lane <= 60;
     double out[5];
     line_vjp<0>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs, kst);
     if (own) {
@@ -498,115 +611,29 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
   }
 }
 
-// Interior cells: the primitive bar of the cell plus the bars of the ghost
-// values copied from it (normal velocity negated across a REFLECTING face),
-// then the conserved -> primitive reverse (or the primitive bar itself for an
-// uploaded-primitive input): y.  Stage 1 adds y to out.  Stage 2 (K2b given)
-// ends the RK reverse too: with Yb = y, Wb = 2 K2b (Y = U + K1, W = U + K1/2,
-// Z = W + K2/2), out = Ub = Yb + Wb and K1b = Yb + Wb/2.  With idx given
-// (stage 1), it also finds the first cell (x fastest) whose CFL term has the
-// bits of the step's maximum (atomicMin; *idx starts at INT_MAX).  With Zp
-// given (stage 1, the previous step's state before the fix), it also starts
-// that step's reverse: K2p = fix^T(out)/2 (the dt kernel redoes the one cell
-// it changes afterwards).
+// The fold of the last stage transpose (stage 1 of the first step): out +=
+// the gathered bars of each cell, through the conserved -> primitive
+// reverse unless the stage input is the uploaded primitives.
 __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGeom g,
-                                                      const double *Vb, double *out,
-                                                      const double *K2b, double *K1b,
-                                                      unsigned long long bits, int *idx,
-                                                      const double *Zp, double *K2p) {
+                                                      const double *Vb, double *out) {
   const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
   for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
        t += (long long)gridDim.x * AB) {
     const int c3[3] = {(int)(t % g.n[0]), (int)((t / g.n[0]) % g.n[1]),
                        (int)(t / ((long long)g.n[0] * g.n[1]))};
-    const long long p = plin(g, c3[0], c3[1], c3[2]);
-    double V[5];
-    load5(Vb, p, V);
-    for (int d = 0; d < 3; d++) {
-      const int c = c3[d], n = g.n[d];
-      const int bi = a.bc[2 * d], bo = a.bc[2 * d + 1];
-      int gh[2], fl[2], ng = 0;
-      if (bi == BC_PERIODIC) {
-        if (c >= n - 2) { gh[ng] = c - n; fl[ng++] = 0; }
-      } else if (c <= 1) {
-        gh[ng] = -1 - c;
-        fl[ng++] = bi == BC_REFLECTING;
-      }
-      if (bo == BC_PERIODIC) {
-        if (c <= 1) { gh[ng] = c + n; fl[ng++] = 0; }
-      } else if (c >= n - 2) {
-        gh[ng] = 2 * n - 1 - c;
-        fl[ng++] = bo == BC_REFLECTING;
-      }
-      for (int k = 0; k < ng; k++) {
-        const long long pg = p + (long long)(gh[k] - c) * g.s[d];
-        double gv[5];
-        load5(Vb, pg, gv);
-#pragma unroll
-        for (int f = 0; f < 5; f++) V[f] += (fl[k] && f == 1 + d) ? -gv[f] : gv[f];
-      }
-    }
-    double y[5] = {0, 0, 0, 0, 0}, U[5];
-#pragma unroll
-    for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];  // the stage input
+    double V[5], y[5] = {0, 0, 0, 0, 0};
+    gather_bars(a, g, Vb, c3, plin(g, c3[0], c3[1], c3[2]), V);
     if (a.src_is_prim) {
 #pragma unroll
       for (int f = 0; f < 5; f++) y[f] = V[f];
     } else {
+      double U[5];
+#pragma unroll
+      for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];
       adj::prim_vjp(U, a.ph.mass, a.ph.nmin, V, y);
     }
-    if (idx) {
-      double P[5];
-      raw_to_prim(a, U, -1, P);
-      const double tc = cfl_term(P, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, c3[0] + 2),
-                                 geo(a, 1, G_WIDTH, c3[1] + 2), geo(a, 2, G_WIDTH, c3[2] + 2));
-      if ((unsigned long long)__double_as_longlong(tc) == bits) atomicMin(idx, (int)t);
-    }
-    if (K2b) {
-      double k2[5], k1[5];
-      load5(K2b, t, k2);
 #pragma unroll
-      for (int f = 0; f < 5; f++) {
-        out[f * ncell + t] = y[f] + 2.0 * k2[f];
-        k1[f] = y[f] + k2[f];
-      }
-      store5(K1b, t, k1);
-    } else {
-      double o[5];
-#pragma unroll
-      for (int f = 0; f < 5; f++) {
-        o[f] = out[f * ncell + t] + y[f];
-        out[f * ncell + t] = o[f];
-      }
-      if (Zp) {
-        double z[5], zb[5];
-#pragma unroll
-        for (int f = 0; f < 5; f++) z[f] = Zp[f * ncell + t];
-        adj::fix_vjp(z, a.ph.mass, a.ph.nmin, o, zb);
-#pragma unroll
-        for (int f = 0; f < 5; f++) zb[f] *= 0.5;
-        store5(K2p, t, zb);
-      }
-    }
-  }
-}
-
-// Zb = fix^T Ub' ; K2b = Zb/2 (and Wb = Zb = 2 K2b, formed by the fold).
-__global__ __launch_bounds__(AB) void adj_fix_kernel(long long ncell, double mass, double nmin,
-                                                     const double *Z, const double *Ubn,
-                                                     double *K2b) {
-  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
-       t += (long long)gridDim.x * AB) {
-    double z[5], ub[5], zb[5];
-#pragma unroll
-    for (int f = 0; f < 5; f++) {
-      z[f] = Z[f * ncell + t];
-      ub[f] = Ubn[f * ncell + t];
-    }
-    adj::fix_vjp(z, mass, nmin, ub, zb);
-#pragma unroll
-    for (int f = 0; f < 5; f++) zb[f] *= 0.5;
-    store5(K2b, t, zb);
+    for (int f = 0; f < 5; f++) out[f * ncell + t] += y[f];
   }
 }
 
@@ -635,11 +662,11 @@ __global__ __launch_bounds__(AB) void adj_partsum_kernel(const double *partial, 
 // The dt bar of one step: sum of the per-block partials (fixed order), the
 // CFL maximum's reverse at its cell (dt = ts = ramp * C / invdt), added to
 // out through the conserved -> primitive reverse; gamma bars into *gbar.
+// The next x pass folds the stage-1 bars into out after it, so the cell's
+// conserved-fix reverse sees this term too.
 __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const double *partial,
                                                     int nparts, double ts, double invdt,
-                                                    int *idx, double *out,
-                                                    double *gbar, const double *Zp,
-                                                    double *K2p) {
+                                                    int *idx, double *out, double *gbar) {
   __shared__ double sh[AB];
   double sd = 0.0, sg = 0.0;
   for (int i = threadIdx.x; i < nparts; i += AB) {
@@ -671,19 +698,9 @@ __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const d
       adj::prim_vjp(U, a.ph.mass, a.ph.nmin, Vb, Ub);
       for (int f = 0; f < 5; f++) out[f * ncell + t] += Ub[f];
     }
-    if (Zp) {  // the fold's K2p of this cell predates the dt term
-      double z[5], o[5], zb[5];
-      for (int f = 0; f < 5; f++) {
-        z[f] = Zp[f * ncell + t];
-        o[f] = out[f * ncell + t];
-      }
-      adj::fix_vjp(z, a.ph.mass, a.ph.nmin, o, zb);
-      for (int f = 0; f < 5; f++) zb[f] *= 0.5;
-      store5(K2p, t, zb);
-    }
   }
   *gbar += gb;
-  idx[0] = 0x7fffffff;  // ready for the next step's fold
+  idx[0] = 0x7fffffff;  // ready for the next step's search
 }
 
 // ComputeFromInitial's reverse: D = N, S_k = (M N) V_k, G = E + S.V/2 with
@@ -762,7 +779,8 @@ void release_adjoint(gx_ctx *c) {
   }
   if (A->mark) (void)hipEventDestroy(A->mark);
   if (A->side) (void)hipStreamDestroy(A->side);
-  for (void *p : {(void *)A->stepF, (void *)A->trash, (void *)A->Vp, (void *)A->Vb,
+  for (void *p : {(void *)A->stepF, (void *)A->trash, (void *)A->Vp, (void *)A->Vb[0],
+                  (void *)A->Vb[1],
                   (void *)A->Ub, (void *)A->K1b, (void *)A->K2b,
                   (void *)A->P0b, (void *)A->partials, (void *)A->partials2, (void *)A->gb, (void *)A->argmax,
                   (void *)A->band, (void *)A->lossp})
@@ -835,36 +853,33 @@ static int line_blocks(const PadGeom &g, int d) {
   return (int)(nxb * nseg);
 }
 
-// Transposed residual: out += (dt R(src))^T Kb; dt and gamma bar partials at
-// part (3 face passes).
-static int stage_vjp(gx_ctx *c, const double *src, bool prim, const double *Kb, double dt,
-                     const uint8_t *flags, double *out, double *part,
-                     const double *K2b = nullptr, double *K1b = nullptr,
-                     unsigned long long cfl_bits = 0, int *argmax = nullptr,
-                     const double *Zp = nullptr, double *K2p = nullptr) {
+// Transposed residual of one stage: the primitives pass of its input src
+// (with the CFL argmax search when idx is given), the x pass folding in the
+// finished stage (fi, MODE) and forming this stage's bar Kout, then the y
+// and z passes with that bar; the stage's bars land in Vb, the dt and gamma
+// bar partials at part (3 passes).
+template <int MODE>
+static int stage_vjp(gx_ctx *c, const double *src, bool prim, const FoldIn &fi, double dt,
+                     const uint8_t *flags, double *Vb, double *part,
+                     unsigned long long cfl_bits = 0, int *idx = nullptr) {
   AdjointState *A = c->adj;
   const PadGeom &g = A->pg;
   StageArgs<double> a = base_args<double>(c);
   a.src = src;
   a.src_is_prim = prim ? 1 : 0;
   a.flags = flags;
-  adj_prims_kernel<<<blocks_for(g.np), AB, 0, c->stream>>>(a, g, A->Vp);
+  adj_prims_kernel<<<blocks_for(g.np), AB, 0, c->stream>>>(a, g, A->Vp, cfl_bits, idx);
   HIP_TRY(hipGetLastError());
   // the x pass sets Vb on the interior rows, the y and z passes on their
   // ghost positions; cells with two coordinates outside stay unused
-  for (int d = 0; d < 3; d++) {
-    double *pp = part + 2 * (size_t)d * A->part_stride;
-    const int nb = line_blocks(g, d);
-    if (d == 0)
-      adj_line_x_kernel<<<nb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb, pp);
-    else if (d == 1)
-      adj_line_t_kernel<1><<<nb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb, pp);
-    else
-      adj_line_t_kernel<2><<<nb, AB, 0, c->stream>>>(a, g, A->Vp, Kb, dt, A->Vb, pp);
-    HIP_TRY(hipGetLastError());
-  }
-  adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, g, A->Vb, out, K2b, K1b,
-                                                                cfl_bits, argmax, Zp, K2p);
+  adj_line_x_kernel<MODE><<<line_blocks(g, 0), AB, 0, c->stream>>>(a, g, A->Vp, fi, dt, Vb,
+                                                                   part);
+  HIP_TRY(hipGetLastError());
+  adj_line_t_kernel<1><<<line_blocks(g, 1), AB, 0, c->stream>>>(
+      a, g, A->Vp, fi.Kout, dt, Vb, part + 2 * (size_t)A->part_stride);
+  HIP_TRY(hipGetLastError());
+  adj_line_t_kernel<2><<<line_blocks(g, 2), AB, 0, c->stream>>>(
+      a, g, A->Vp, fi.Kout, dt, Vb, part + 4 * (size_t)A->part_stride);
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }
@@ -944,7 +959,7 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   // cell-interleaved work arrays (AOS values per cell)
   for (double **p : {&A->K1b, &A->K2b})
     HIP_TRY(hipMalloc(p, AOS * sizeof(double) * (size_t)c->ncell));
-  for (double **p : {&A->Vp, &A->Vb})
+  for (double **p : {&A->Vp, &A->Vb[0], &A->Vb[1]})
     HIP_TRY(hipMalloc(p, AOS * sizeof(double) * (size_t)g.np));
   A->part_stride = 0;
   for (int d = 0; d < 3; d++) A->part_stride = std::max(A->part_stride, line_blocks(g, d));
@@ -1061,7 +1076,7 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   if (rc) return rc;
   const double *P0 = (const double *)c->P0;
   const int every = A->every, N = A->nsteps;
-  bool k2_ready = false;  // K2b of the next step to reverse is formed already
+  const double *Unext = nullptr;  // the stage-1 input of the step reversed last
   for (int si = (N - 1) / every; si >= 0; si--) {
     const int seg0 = si * every, len = std::min(every, N - seg0);
     // Segment si's states (recomputed from its checkpoint with the recorded
@@ -1082,40 +1097,54 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       const int n = seg0 + j;
       const double *Un = j == 0 ? ck : b.U + (size_t)j * n5;
       const double *Yn = b.Y + (size_t)j * n5;
+      const double *Zn = b.Z + (size_t)j * n5;
       const uint8_t *Fn = A->stepF + (size_t)n * c->ncell;
       const bool first = n == 0;
       const double dt = A->dt[n];
-      if (!k2_ready) {
-        adj_fix_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(
-            c->ncell, c->params.BaryonMassReference, c->params.BaryonDensityMin,
-            b.Z + (size_t)j * n5, A->Ub, A->K2b);
-        HIP_TRY(hipGetLastError());
+      // stage 2's transpose; its x pass first folds in stage 1 of step n + 1
+      // (or takes the seed) and forms K2b = fix^T(Ub)/2
+      FoldIn f2 = {};
+      f2.out = A->Ub;
+      f2.Z = Zn;
+      f2.Kout = A->K2b;
+      double *part2 = A->partials + 6 * (size_t)A->part_stride;
+      if (Unext) {
+        f2.Vb = A->Vb[1];
+        f2.src = Unext;
+        rc = stage_vjp<FOLD_STAGE1>(c, Yn, false, f2, dt, Fn, A->Vb[0], part2);
+      } else {
+        rc = stage_vjp<FOLD_SEED>(c, Yn, false, f2, dt, Fn, A->Vb[0], part2);
       }
-      // stage 2's transpose, ending with Ub = Yb + Wb and K1b = Yb + Wb/2
-      rc = stage_vjp(c, Yn, false, A->K2b, dt, Fn, A->Ub,
-                     A->partials + 6 * (size_t)A->part_stride, A->K2b, A->K1b);
       if (rc) return rc;
-      double *out = first ? A->P0b : A->Ub;
-      // stage 1's transpose; its fold also finds the cell of the CFL maximum
-      // and, when the previous step's Z is in the segment, forms its K2b
+      // stage 1's transpose; its x pass folds in stage 2 (Ub = Yb + Wb,
+      // K1b = Yb + Wb/2), its primitives pass finds the CFL maximum's cell
+      FoldIn f1 = {};
+      f1.Vb = A->Vb[0];
+      f1.src = Yn;
+      f1.out = A->Ub;
+      f1.K2b = A->K2b;
+      f1.Kout = A->K1b;
       unsigned long long bits;
       memcpy(&bits, &A->invdt[n], sizeof(bits));
-      const double *Zp = j > 0 ? b.Z + (size_t)(j - 1) * n5 : nullptr;
-      rc = stage_vjp(c, first ? P0 : Un, first, A->K1b, dt, Fn, out, A->partials, nullptr,
-                     nullptr, bits, A->argmax, Zp, Zp ? A->K2b : nullptr);
+      rc = stage_vjp<FOLD_STAGE2>(c, first ? P0 : Un, first, f1, dt, Fn, A->Vb[1],
+                                  A->partials, bits, A->argmax);
       if (rc) return rc;
-      k2_ready = Zp != nullptr;
-      // dt = ts(invdt(U^n))
       StageArgs<double> a = base_args<double>(c);
       a.src = first ? P0 : Un;
       a.src_is_prim = first ? 1 : 0;
+      double *out = first ? A->P0b : A->Ub;
+      if (first) {  // the last stage transpose: its own fold
+        adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, A->pg, A->Vb[1], out);
+        HIP_TRY(hipGetLastError());
+      }
+      // dt = ts(invdt(U^n))
       adj_partsum_kernel<<<PART_BLOCKS, AB, 0, c->stream>>>(A->partials, 6 * A->part_stride,
                                                             A->partials2);
       HIP_TRY(hipGetLastError());
       adj_dt_kernel<<<1, AB, 0, c->stream>>>(a, A->partials2, PART_BLOCKS, A->ts[n],
-                                             A->invdt[n], A->argmax, out, A->gb, Zp,
-                                             Zp ? A->K2b : nullptr);
+                                             A->invdt[n], A->argmax, out, A->gb);
       HIP_TRY(hipGetLastError());
+      Unext = Un;
     }
   }
   // U^0 = ComputeFromInitial(P0)
