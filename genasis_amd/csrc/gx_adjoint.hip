@@ -65,6 +65,29 @@ __device__ __forceinline__ void store5(double *base, long long cell, const doubl
   p[2] = make_double2(v[4], 0.0);
 }
 
+// A stage bar K of a cell with the cell's shock-flag byte in the sixth slot,
+// so that the y and z passes, whose lanes read cells on 64 rows or planes,
+// take the flags with K's third 16-byte access.
+__device__ __forceinline__ void store5_flags(double *base, long long cell, const double v[5],
+                                             unsigned flags) {
+  double2 *p = reinterpret_cast<double2 *>(base + AOS * cell);
+  p[0] = make_double2(v[0], v[1]);
+  p[1] = make_double2(v[2], v[3]);
+  p[2] = make_double2(v[4], __longlong_as_double((long long)flags));
+}
+
+__device__ __forceinline__ unsigned load5_flags(const double *base, long long cell,
+                                                double v[5]) {
+  const double2 *p = reinterpret_cast<const double2 *>(base + AOS * cell);
+  const double2 a = p[0], b = p[1], c = p[2];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+  v[4] = c.x;
+  return (unsigned)__double_as_longlong(c.y);
+}
+
 struct PadGeom {
   int n[3];
   int m[3];
@@ -251,13 +274,16 @@ __device__ __forceinline__ double lane_down(double v) {
   return __hiloint2double(hi, lo);
 }
 
-template <int D>
+// get_K(K) gives the lane's stage bar K of its cell (0 off the interior); it
+// is called after the slopes, so the loads behind it overlap their
+// arithmetic.
+template <int D, typename GetK>
 __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int c, int lane,
                                          bool own, double A, double dt, const double V[5],
-                                         const double Kb[5], bool df, double volc,
+                                         GetK get_K, bool df, double volc,
                                          double volm, double Vb[5], double &dtb,
                                          double &gbs, double (*kst)[AB]) {
-  double Vm[5], Vq[5], s[5], sm[5], Kbm[5];
+  double Vm[5], Vq[5], s[5], sm[5], Kbm[5], Kb[5];
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     Vm[f] = lane_up(V[f]);
@@ -290,6 +316,7 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
     kst[5 + f][threadIdx.x] = kr[f];
   }
   __asm__ volatile("" ::: "memory");
+  get_K(Kb);
 #pragma unroll
   for (int f = 0; f < 5; f++) {
     sm[f] = lane_up(s[f]);
@@ -421,47 +448,62 @@ struct FoldIn {
   double *Kout;       // the bar of the x pass's own stage: K1b or K2b (AOS)
 };
 
+// The loads of a cell's fold (issued before the sweep's slopes) and its
+// arithmetic and stores (after them).
+struct FoldVals {
+  double Vg[5], U[5], o[5], w[5];  // w: Z (FOLD_SEED, FOLD_STAGE1) or K2b
+};
+
 template <int MODE>
-__device__ __forceinline__ void fold_cell(const StageArgs<double> &a, const PadGeom &g,
+__device__ __forceinline__ void fold_load(const StageArgs<double> &a, const PadGeom &g,
                                           const FoldIn &fi, const int c3[3], long long p,
-                                          unsigned ic, bool own, double K[5]) {
+                                          unsigned ic, FoldVals &v) {
   const long long fs = (long long)g.n[0] * g.n[1] * g.n[2];
-  double o[5];
   if (MODE != FOLD_SEED) {
-    double Vg[5], U[5], yv[5] = {0, 0, 0, 0, 0};
-    gather_bars(a, g, fi.Vb, c3, p, Vg);
+    gather_bars(a, g, fi.Vb, c3, p, v.Vg);
 #pragma unroll
-    for (int f = 0; f < 5; f++) U[f] = fi.src[f * fs + ic];
-    adj::prim_vjp(U, a.ph.mass, a.ph.nmin, Vg, yv);
+    for (int f = 0; f < 5; f++) v.U[f] = fi.src[f * fs + ic];
+  }
+  if (MODE == FOLD_STAGE2) {
+    load5(fi.K2b, ic, v.w);
+  } else {
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+      v.o[f] = fi.out[f * fs + ic];
+      v.w[f] = fi.Z[f * fs + ic];
+    }
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void fold_finish(const StageArgs<double> &a, const PadGeom &g,
+                                            const FoldIn &fi, unsigned ic, bool own,
+                                            unsigned flags, FoldVals &v, double K[5]) {
+  const long long fs = (long long)g.n[0] * g.n[1] * g.n[2];
+  if (MODE != FOLD_SEED) {
+    double yv[5] = {0, 0, 0, 0, 0};
+    adj::prim_vjp(v.U, a.ph.mass, a.ph.nmin, v.Vg, yv);
     if (MODE == FOLD_STAGE2) {
-      double k2[5];
-      load5(fi.K2b, ic, k2);
 #pragma unroll
       for (int f = 0; f < 5; f++) {
-        o[f] = yv[f] + 2.0 * k2[f];
-        K[f] = yv[f] + k2[f];
+        v.o[f] = yv[f] + 2.0 * v.w[f];
+        K[f] = yv[f] + v.w[f];
       }
       if (own) {
 #pragma unroll
-        for (int f = 0; f < 5; f++) fi.out[f * fs + ic] = o[f];
-        store5(fi.Kout, ic, K);
+        for (int f = 0; f < 5; f++) fi.out[f * fs + ic] = v.o[f];
+        store5_flags(fi.Kout, ic, K, flags);
       }
       return;
     }
     // (Ub itself is not stored: the stage-2 fold of this step replaces it)
 #pragma unroll
-    for (int f = 0; f < 5; f++) o[f] = fi.out[f * fs + ic] + yv[f];
-  } else {
-#pragma unroll
-    for (int f = 0; f < 5; f++) o[f] = fi.out[f * fs + ic];
+    for (int f = 0; f < 5; f++) v.o[f] += yv[f];
   }
-  double z[5];
-#pragma unroll
-  for (int f = 0; f < 5; f++) z[f] = fi.Z[f * fs + ic];
-  adj::fix_vjp(z, a.ph.mass, a.ph.nmin, o, K);
+  adj::fix_vjp(v.w, a.ph.mass, a.ph.nmin, v.o, K);
 #pragma unroll
   for (int f = 0; f < 5; f++) K[f] *= 0.5;
-  if (own) store5(fi.Kout, ic, K);
+  if (own) store5_flags(fi.Kout, ic, K, flags);
 }
 
 // D = 0: the rows (y, z) of the padded layout, cells -2 .. n+1 each, are
@@ -491,28 +533,34 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
     const int c = valid ? (int)(p - r * L) - 2 : -100;  // -100: in no cell range
     const int y = (int)(r % g.n[1]), z = (int)(r / g.n[1]);
     const long long pl = plin(g, 0, y, z);
-    double V[5], K[5];
+    double V[5];
     bool df = false;
     double volc = 0.0, volm = 0.0;
     const double wy = cell_width(a, 1, y), wz = cell_width(a, 2, z);
     const double A = wy * wz;
     load5(Vp, pl + (valid ? c : 0), V);
 #pragma unroll
-    for (int f = 0; f < 5; f++) {
-      V[f] = valid ? V[f] : 0.0;
-      K[f] = 0.0;
-    }
+    for (int f = 0; f < 5; f++) V[f] = valid ? V[f] : 0.0;
     const bool own = valid && lane >= 3 && lane <= 60;
-    if (c >= 0 && c < n) {
-      const unsigned ic = lin(a.n, c, y, z);
+    const bool interior = c >= 0 && c < n;
+    const unsigned ic = interior ? lin(a.n, c, y, z) : 0u;
+    FoldVals fv;
+    unsigned fl = 0;
+    if (interior) {
       const int c3[3] = {c, y, z};
-      fold_cell<MODE>(a, g, fi, c3, pl + c, ic, own, K);
-      df = (a.flags[ic] >> 0) & 1;
+      fold_load<MODE>(a, g, fi, c3, pl + c, ic, fv);
+      fl = a.flags[ic];
+      df = fl & 1;
       volc = cell_width(a, 0, c) * wy * wz;
     }
     if (c >= 1 && c <= n) volm = cell_width(a, 0, c - 1) * wy * wz;
     double out[5];
-    line_vjp<0>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs, kst);
+    auto get_K = [&](double K[5]) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) K[f] = 0.0;
+      if (interior) fold_finish<MODE>(a, g, fi, ic, own, fl, fv, K);
+    };
+    line_vjp<0>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
     if (own) {
       // the x pass is the first to touch the interior rows: it sets them
       store5(Vb, pl + c, out);
@@ -577,8 +625,7 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
     }
     if (interior) {
       const unsigned ik = lin(a.n, q[0], q[1], q[2]);
-      load5(Kb, ik, K);
-      df = (a.flags[ik] >> D) & 1;
+      df = (load5_flags(Kb, ik, K) >> D) & 1;  // the flags the x pass stored
     }
     const double wo = cell_width(a, O, o);
     const double wx = cell_width(a, 0, x);
@@ -594,7 +641,11 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
     for (int f = 0; f < 5; f++) kst[15 + f][threadIdx.x] = Vold[f];
     __asm__ volatile("" ::: "memory");
     double out[5];
-    line_vjp<D>(a, n, c, lane, own, A, dt, V, K, df, volc, volm, out, dtb, gbs, kst);
+    auto get_K = [&](double Kc[5]) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) Kc[f] = K[f];
+    };
+    line_vjp<D>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
     __asm__ volatile("" ::: "memory");
     if (own) {
       double nv[5];
