@@ -121,8 +121,9 @@ struct AdjointState {
   uint8_t *stepF = nullptr;    // the shock flags each step used, nsteps ncell
   double *trash = nullptr;
   double *Vp = nullptr;               // primitives of a stage input, AOS np
-  double *Vb[2] = {nullptr, nullptr}; // bars of stage 2 and stage 1, AOS np: the x
-                                      // pass of one stage folds in the other's
+  // bars of stage 2 and stage 1 (AOS np), one array per line pass: the x
+  // pass of one stage folds in the other's three
+  double *Vb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double *Ub = nullptr, *P0b = nullptr;    // 5 ncell, field slabs
   double *K1b = nullptr, *K2b = nullptr;   // AOS ncell, cell-interleaved
   bool seed_ready = false;     // Ub holds dJ/dU_final (gx_adjoint_kinetic_energy)
@@ -394,13 +395,26 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
   }
 }
 
+// Bars of a stage on the padded layout, one array per line pass: pass d
+// writes the interior cells and the ghost positions along d.
+struct StageBars {
+  double *d[3];
+};
+
 // The fold of a finished stage: the bar of an interior cell on the padded
-// layout plus the bars of the ghost values copied from it (normal velocity
-// negated across a REFLECTING face).
+// layout (the sum of the three passes' bars) plus the bars of the ghost
+// values copied from it (normal velocity negated across a REFLECTING face).
 __device__ __forceinline__ void gather_bars(const StageArgs<double> &a, const PadGeom &g,
-                                            const double *Vb, const int c3[3], long long p,
+                                            const StageBars &Vb, const int c3[3], long long p,
                                             double V[5]) {
-  load5(Vb, p, V);
+  {
+    double vy[5], vz[5];
+    load5(Vb.d[0], p, V);
+    load5(Vb.d[1], p, vy);
+    load5(Vb.d[2], p, vz);
+#pragma unroll
+    for (int f = 0; f < 5; f++) V[f] = (V[f] + vy[f]) + vz[f];
+  }
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const int c = c3[d], n = g.n[d];
@@ -421,7 +435,7 @@ __device__ __forceinline__ void gather_bars(const StageArgs<double> &a, const Pa
     for (int k = 0; k < ng; k++) {
       const long long pg = p + (long long)(gh[k] - c) * g.s[d];
       double gv[5];
-      load5(Vb, pg, gv);
+      load5(Vb.d[d], pg, gv);
 #pragma unroll
       for (int f = 0; f < 5; f++) V[f] += (fl[k] && f == 1 + d) ? -gv[f] : gv[f];
     }
@@ -440,7 +454,7 @@ __device__ __forceinline__ void gather_bars(const StageArgs<double> &a, const Pa
 enum { FOLD_SEED = 0, FOLD_STAGE1 = 1, FOLD_STAGE2 = 2 };
 
 struct FoldIn {
-  const double *Vb;   // the finished stage's bars on the padded layout (AOS)
+  StageBars Vb;       // the finished stage's bars on the padded layout (AOS)
   const double *src;  // that stage's input (conserved field slabs)
   double *out;        // Ub (field slabs)
   const double *K2b;  // FOLD_STAGE2: the stage-2 bar (AOS)
@@ -561,10 +575,7 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
       if (interior) fold_finish<MODE>(a, g, fi, ic, own, fl, fv, K);
     };
     line_vjp<0>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
-    if (own) {
-      // the x pass is the first to touch the interior rows: it sets them
-      store5(Vb, pl + c, out);
-    }
+    if (own) store5(Vb, pl + c, out);  // this pass's own array
   }
   const double s1 = block_sum(dtb, sh);
   const double s2 = block_sum(gbs, sh);
@@ -578,8 +589,9 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
 // of the sequence of lines along D (cells -2 .. n+1 of line o, then of line
 // o+1, ...) in column x; the four waves of a block take four consecutive x
 // at the same positions, so each strided load of a wave shares its cache
-// lines with the other three.  Ghost positions along D are touched by this
-// pass alone and are set; interior cells add to the x pass.
+// lines with the other three.  The pass writes its bars (interior cells and
+// the ghost positions along D) to an array of its own, which the fold adds
+// to the other passes', so it reads no earlier bars.
 constexpr int DXW = AB / WAVE;  // x columns per block
 template <int D>
 __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, PadGeom g,
@@ -588,7 +600,7 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
                                                         double *partial) {
   constexpr int O = D == 1 ? 2 : 1;  // the other transverse dimension
   __shared__ double sh[AB];
-  __shared__ double kst[20][AB];
+  __shared__ double kst[15][AB];
   const int n = g.n[D];
   const int L = n + 4;
   const int T = L * g.n[O];
@@ -608,20 +620,16 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
     q[D] = valid ? c : 0;
     q[O] = o;
     const long long pv = plin(g, q[0], q[1], q[2]);
-    double V[5], K[5], Vold[5];
+    double V[5], K[5];
     bool df = false;
     const bool interior = c >= 0 && c < n;
     const bool own = valid && lane >= 3 && lane <= 60;
     // pv is in the padded array for every lane (q[D] = 0 off the sequence)
     load5(Vp, pv, V);
-    // the x pass's bars of interior cells, fetched now so that their
-    // (strided) latency overlaps the sweep
-    load5(Vb, pv, Vold);
 #pragma unroll
     for (int f = 0; f < 5; f++) {
       V[f] = valid ? V[f] : 0.0;
       K[f] = 0.0;
-      Vold[f] = own && interior ? Vold[f] : 0.0;
     }
     if (interior) {
       const unsigned ik = lin(a.n, q[0], q[1], q[2]);
@@ -636,23 +644,13 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
     };
     const double volc = (c >= 0 && c < n) ? vol_of(c) : 0.0;
     const double volm = (c >= 1 && c <= n) ? vol_of(c - 1) : 0.0;
-    // the x pass's bars wait in LDS during the sweep
-#pragma unroll
-    for (int f = 0; f < 5; f++) kst[15 + f][threadIdx.x] = Vold[f];
-    __asm__ volatile("" ::: "memory");
     double out[5];
     auto get_K = [&](double Kc[5]) {
 #pragma unroll
       for (int f = 0; f < 5; f++) Kc[f] = K[f];
     };
     line_vjp<D>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
-    __asm__ volatile("" ::: "memory");
-    if (own) {
-      double nv[5];
-#pragma unroll
-      for (int f = 0; f < 5; f++) nv[f] = kst[15 + f][threadIdx.x] + out[f];
-      store5(Vb, pv, nv);
-    }
+    if (own) store5(Vb, pv, out);  // this pass's own array
   }
   const double s1 = block_sum(dtb, sh);
   const double s2 = block_sum(gbs, sh);
@@ -666,7 +664,7 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
 // the gathered bars of each cell, through the conserved -> primitive
 // reverse unless the stage input is the uploaded primitives.
 __global__ __launch_bounds__(AB) void adj_fold_kernel(StageArgs<double> a, PadGeom g,
-                                                      const double *Vb, double *out) {
+                                                      StageBars Vb, double *out) {
   const long long ncell = (long long)g.n[0] * g.n[1] * g.n[2];
   for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < ncell;
        t += (long long)gridDim.x * AB) {
@@ -830,8 +828,9 @@ void release_adjoint(gx_ctx *c) {
   }
   if (A->mark) (void)hipEventDestroy(A->mark);
   if (A->side) (void)hipStreamDestroy(A->side);
-  for (void *p : {(void *)A->stepF, (void *)A->trash, (void *)A->Vp, (void *)A->Vb[0],
-                  (void *)A->Vb[1],
+  for (int s = 0; s < 2; s++)
+    for (int d = 0; d < 3; d++) (void)hipFree(A->Vb[s][d]);
+  for (void *p : {(void *)A->stepF, (void *)A->trash, (void *)A->Vp,
                   (void *)A->Ub, (void *)A->K1b, (void *)A->K2b,
                   (void *)A->P0b, (void *)A->partials, (void *)A->partials2, (void *)A->gb, (void *)A->argmax,
                   (void *)A->band, (void *)A->lossp})
@@ -907,11 +906,11 @@ static int line_blocks(const PadGeom &g, int d) {
 // Transposed residual of one stage: the primitives pass of its input src
 // (with the CFL argmax search when idx is given), the x pass folding in the
 // finished stage (fi, MODE) and forming this stage's bar Kout, then the y
-// and z passes with that bar; the stage's bars land in Vb, the dt and gamma
-// bar partials at part (3 passes).
+// and z passes with that bar; pass d's bars land in Vb.d[d], the dt and
+// gamma bar partials at part (3 passes).
 template <int MODE>
 static int stage_vjp(gx_ctx *c, const double *src, bool prim, const FoldIn &fi, double dt,
-                     const uint8_t *flags, double *Vb, double *part,
+                     const uint8_t *flags, const StageBars &Vb, double *part,
                      unsigned long long cfl_bits = 0, int *idx = nullptr) {
   AdjointState *A = c->adj;
   const PadGeom &g = A->pg;
@@ -921,16 +920,15 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const FoldIn &fi, 
   a.flags = flags;
   adj_prims_kernel<<<blocks_for(g.np), AB, 0, c->stream>>>(a, g, A->Vp, cfl_bits, idx);
   HIP_TRY(hipGetLastError());
-  // the x pass sets Vb on the interior rows, the y and z passes on their
-  // ghost positions; cells with two coordinates outside stay unused
-  adj_line_x_kernel<MODE><<<line_blocks(g, 0), AB, 0, c->stream>>>(a, g, A->Vp, fi, dt, Vb,
-                                                                   part);
+  // cells with two coordinates outside are never written nor read
+  adj_line_x_kernel<MODE><<<line_blocks(g, 0), AB, 0, c->stream>>>(a, g, A->Vp, fi, dt,
+                                                                   Vb.d[0], part);
   HIP_TRY(hipGetLastError());
   adj_line_t_kernel<1><<<line_blocks(g, 1), AB, 0, c->stream>>>(
-      a, g, A->Vp, fi.Kout, dt, Vb, part + 2 * (size_t)A->part_stride);
+      a, g, A->Vp, fi.Kout, dt, Vb.d[1], part + 2 * (size_t)A->part_stride);
   HIP_TRY(hipGetLastError());
   adj_line_t_kernel<2><<<line_blocks(g, 2), AB, 0, c->stream>>>(
-      a, g, A->Vp, fi.Kout, dt, Vb, part + 4 * (size_t)A->part_stride);
+      a, g, A->Vp, fi.Kout, dt, Vb.d[2], part + 4 * (size_t)A->part_stride);
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }
@@ -1010,7 +1008,8 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   // cell-interleaved work arrays (AOS values per cell)
   for (double **p : {&A->K1b, &A->K2b})
     HIP_TRY(hipMalloc(p, AOS * sizeof(double) * (size_t)c->ncell));
-  for (double **p : {&A->Vp, &A->Vb[0], &A->Vb[1]})
+  for (double **p : {&A->Vp, &A->Vb[0][0], &A->Vb[0][1], &A->Vb[0][2], &A->Vb[1][0],
+                     &A->Vb[1][1], &A->Vb[1][2]})
     HIP_TRY(hipMalloc(p, AOS * sizeof(double) * (size_t)g.np));
   A->part_stride = 0;
   for (int d = 0; d < 3; d++) A->part_stride = std::max(A->part_stride, line_blocks(g, d));
@@ -1159,25 +1158,27 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       f2.Z = Zn;
       f2.Kout = A->K2b;
       double *part2 = A->partials + 6 * (size_t)A->part_stride;
+      const StageBars bars2 = {{A->Vb[0][0], A->Vb[0][1], A->Vb[0][2]}};
+      const StageBars bars1 = {{A->Vb[1][0], A->Vb[1][1], A->Vb[1][2]}};
       if (Unext) {
-        f2.Vb = A->Vb[1];
+        f2.Vb = bars1;
         f2.src = Unext;
-        rc = stage_vjp<FOLD_STAGE1>(c, Yn, false, f2, dt, Fn, A->Vb[0], part2);
+        rc = stage_vjp<FOLD_STAGE1>(c, Yn, false, f2, dt, Fn, bars2, part2);
       } else {
-        rc = stage_vjp<FOLD_SEED>(c, Yn, false, f2, dt, Fn, A->Vb[0], part2);
+        rc = stage_vjp<FOLD_SEED>(c, Yn, false, f2, dt, Fn, bars2, part2);
       }
       if (rc) return rc;
       // stage 1's transpose; its x pass folds in stage 2 (Ub = Yb + Wb,
       // K1b = Yb + Wb/2), its primitives pass finds the CFL maximum's cell
       FoldIn f1 = {};
-      f1.Vb = A->Vb[0];
+      f1.Vb = bars2;
       f1.src = Yn;
       f1.out = A->Ub;
       f1.K2b = A->K2b;
       f1.Kout = A->K1b;
       unsigned long long bits;
       memcpy(&bits, &A->invdt[n], sizeof(bits));
-      rc = stage_vjp<FOLD_STAGE2>(c, first ? P0 : Un, first, f1, dt, Fn, A->Vb[1],
+      rc = stage_vjp<FOLD_STAGE2>(c, first ? P0 : Un, first, f1, dt, Fn, bars1,
                                   A->partials, bits, A->argmax);
       if (rc) return rc;
       StageArgs<double> a = base_args<double>(c);
@@ -1185,7 +1186,7 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       a.src_is_prim = first ? 1 : 0;
       double *out = first ? A->P0b : A->Ub;
       if (first) {  // the last stage transpose: its own fold
-        adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, A->pg, A->Vb[1], out);
+        adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, A->pg, bars1, out);
         HIP_TRY(hipGetLastError());
       }
       // dt = ts(invdt(U^n))
