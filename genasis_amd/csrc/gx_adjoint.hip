@@ -147,32 +147,6 @@ __global__ __launch_bounds__(AB) void adj_fill_kernel(double *p, long long n, do
     p[i] = v;
 }
 
-// Primitives of the stage input on the padded layout, ghosts by the ghost
-// rule (cells with two or more coordinates outside are unused and 0).  With
-// idx given (stage 1), it also finds the first cell (x fastest) whose CFL
-// term has the bits of the step's maximum (atomicMin; *idx starts at
-// INT_MAX): the term is formed from the same primitives as the forward's.
-__global__ __launch_bounds__(AB) void adj_prims_kernel(StageArgs<double> a, PadGeom g,
-                                                       double *Vp, unsigned long long bits,
-                                                       int *idx) {
-  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < g.np;
-       t += (long long)gridDim.x * AB) {
-    const int x = (int)(t % g.m[0]) - 2;
-    const int y = (int)((t / g.m[0]) % g.m[1]) - 2;
-    const int z = (int)(t / ((long long)g.m[0] * g.m[1])) - 2;
-    const int out = (x < 0 || x >= g.n[0]) + (y < 0 || y >= g.n[1]) + (z < 0 || z >= g.n[2]);
-    double V[5] = {0, 0, 0, 0, 0};
-    if (out <= 1) load_prim(a, x, y, z, V);
-    store5(Vp, t, V);
-    if (idx && out == 0) {
-      const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, x + 2),
-                                 geo(a, 1, G_WIDTH, y + 2), geo(a, 2, G_WIDTH, z + 2));
-      if ((unsigned long long)__double_as_longlong(tc) == bits)
-        atomicMin(idx, (int)lin(a.n, x, y, z));
-    }
-  }
-}
-
 __device__ inline double block_sum(double v, double *sh) {
   sh[threadIdx.x] = v;
   __syncthreads();
@@ -712,26 +686,35 @@ __global__ __launch_bounds__(AB) void adj_partsum_kernel(const double *partial, 
 // CFL maximum's reverse at its cell (dt = ts = ramp * C / invdt), added to
 // out through the conserved -> primitive reverse; gamma bars into *gbar.
 // The next x pass folds the stage-1 bars into out after it, so the cell's
-// conserved-fix reverse sees this term too.
-__global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const double *partial,
-                                                    int nparts, double ts, double invdt,
-                                                    int *idx, double *out, double *gbar) {
-  __shared__ double sh[AB];
+// conserved-fix reverse sees this term too.  One block: the dt kernel of the
+// first step, and for the other steps the last block of the next stage
+// transpose's primitives pass, where its latency hides behind that pass.
+struct DtArgs {
+  StageArgs<double> a;  // the step's stage-1 input
+  const double *partial;
+  int nparts;
+  double ts, invdt;
+  int *idx;
+  double *out, *gbar;
+};
+
+__device__ __forceinline__ void dt_work(const DtArgs &d, double *sh) {
+  const StageArgs<double> &a = d.a;
   double sd = 0.0, sg = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += AB) {
-    sd += partial[2 * i];
-    sg += partial[2 * i + 1];
+  for (int i = threadIdx.x; i < d.nparts; i += AB) {
+    sd += d.partial[2 * i];
+    sg += d.partial[2 * i + 1];
   }
   const double dtb = block_sum(sd, sh);
   const double gsum = block_sum(sg, sh);
   if (threadIdx.x != 0) return;
   double gb = gsum;
   const long long ncell = (long long)a.n[0] * a.n[1] * a.n[2];
-  const int t = idx[0];
-  if (invdt > DBL_MIN && ts > 0.0 && !(t >= 0 && t < ncell)) {
-    idx[1] += 1;  // no cell reproduced the maximum: gx_adjoint_backward fails
-  } else if (invdt > DBL_MIN && ts > 0.0) {
-    const double invb = dtb * (-ts / invdt);
+  const int t = d.idx[0];
+  if (d.invdt > DBL_MIN && d.ts > 0.0 && !(t >= 0 && t < ncell)) {
+    d.idx[1] += 1;  // no cell reproduced the maximum: gx_adjoint_backward fails
+  } else if (d.invdt > DBL_MIN && d.ts > 0.0) {
+    const double invb = dtb * (-d.ts / d.invdt);
     const int x = t % a.n[0], y = (t / a.n[0]) % a.n[1], z = t / (a.n[0] * a.n[1]);
     double V[5];
     load_prim(a, x, y, z, V);
@@ -740,16 +723,54 @@ __global__ __launch_bounds__(AB) void adj_dt_kernel(StageArgs<double> a, const d
     double Vb[5] = {0, 0, 0, 0, 0};
     adj::cfl_term_vjp(V, a.ph.gamma, a.ph.mass, w, invb, Vb, gb);
     if (a.src_is_prim) {
-      for (int f = 0; f < 5; f++) out[f * ncell + t] += Vb[f];
+      for (int f = 0; f < 5; f++) d.out[f * ncell + t] += Vb[f];
     } else {
       double U[5], Ub[5] = {0, 0, 0, 0, 0};
       for (int f = 0; f < 5; f++) U[f] = a.src[f * a.fs + t];
       adj::prim_vjp(U, a.ph.mass, a.ph.nmin, Vb, Ub);
-      for (int f = 0; f < 5; f++) out[f * ncell + t] += Ub[f];
+      for (int f = 0; f < 5; f++) d.out[f * ncell + t] += Ub[f];
     }
   }
-  *gbar += gb;
-  idx[0] = 0x7fffffff;  // ready for the next step's search
+  *d.gbar += gb;
+  d.idx[0] = 0x7fffffff;  // ready for the next step's search
+}
+
+__global__ __launch_bounds__(AB) void adj_dt_kernel(DtArgs d) {
+  __shared__ double sh[AB];
+  dt_work(d, sh);
+}
+
+// Primitives of the stage input on the padded layout, ghosts by the ghost
+// rule (cells with two or more coordinates outside are unused and 0).  With
+// idx given (stage 1), it also finds the first cell (x fastest) whose CFL
+// term has the bits of the step's maximum (atomicMin; *idx starts at
+// INT_MAX): the term is formed from the same primitives as the forward's.
+// With with_dt, its last block does the dt work of the step reversed before.
+__global__ __launch_bounds__(AB) void adj_prims_kernel(StageArgs<double> a, PadGeom g,
+                                                       double *Vp, unsigned long long bits,
+                                                       int *idx, DtArgs dtw, int with_dt) {
+  const int nb = gridDim.x - with_dt;
+  if ((int)blockIdx.x >= nb) {
+    __shared__ double sh[AB];
+    dt_work(dtw, sh);
+    return;
+  }
+  for (long long t = blockIdx.x * (long long)AB + threadIdx.x; t < g.np;
+       t += (long long)nb * AB) {
+    const int x = (int)(t % g.m[0]) - 2;
+    const int y = (int)((t / g.m[0]) % g.m[1]) - 2;
+    const int z = (int)(t / ((long long)g.m[0] * g.m[1])) - 2;
+    const int out = (x < 0 || x >= g.n[0]) + (y < 0 || y >= g.n[1]) + (z < 0 || z >= g.n[2]);
+    double V[5] = {0, 0, 0, 0, 0};
+    if (out <= 1) load_prim(a, x, y, z, V);
+    store5(Vp, t, V);
+    if (idx && out == 0) {
+      const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, x + 2),
+                                 geo(a, 1, G_WIDTH, y + 2), geo(a, 2, G_WIDTH, z + 2));
+      if ((unsigned long long)__double_as_longlong(tc) == bits)
+        atomicMin(idx, (int)lin(a.n, x, y, z));
+    }
+  }
 }
 
 // ComputeFromInitial's reverse: D = N, S_k = (M N) V_k, G = E + S.V/2 with
@@ -904,21 +925,25 @@ static int line_blocks(const PadGeom &g, int d) {
 }
 
 // Transposed residual of one stage: the primitives pass of its input src
-// (with the CFL argmax search when idx is given), the x pass folding in the
+// (with the CFL argmax search when idx is given, and the dt work of the step
+// reversed before when dtw is given), the x pass folding in the
 // finished stage (fi, MODE) and forming this stage's bar Kout, then the y
 // and z passes with that bar; pass d's bars land in Vb.d[d], the dt and
 // gamma bar partials at part (3 passes).
 template <int MODE>
 static int stage_vjp(gx_ctx *c, const double *src, bool prim, const FoldIn &fi, double dt,
                      const uint8_t *flags, const StageBars &Vb, double *part,
-                     unsigned long long cfl_bits = 0, int *idx = nullptr) {
+                     unsigned long long cfl_bits = 0, int *idx = nullptr,
+                     const DtArgs *dtw = nullptr) {
   AdjointState *A = c->adj;
   const PadGeom &g = A->pg;
   StageArgs<double> a = base_args<double>(c);
   a.src = src;
   a.src_is_prim = prim ? 1 : 0;
   a.flags = flags;
-  adj_prims_kernel<<<blocks_for(g.np), AB, 0, c->stream>>>(a, g, A->Vp, cfl_bits, idx);
+  const int wdt = dtw ? 1 : 0;
+  adj_prims_kernel<<<blocks_for(g.np) + wdt, AB, 0, c->stream>>>(a, g, A->Vp, cfl_bits, idx,
+                                                                 dtw ? *dtw : DtArgs{}, wdt);
   HIP_TRY(hipGetLastError());
   // cells with two coordinates outside are never written nor read
   adj_line_x_kernel<MODE><<<line_blocks(g, 0), AB, 0, c->stream>>>(a, g, A->Vp, fi, dt,
@@ -1127,6 +1152,7 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
   const double *P0 = (const double *)c->P0;
   const int every = A->every, N = A->nsteps;
   const double *Unext = nullptr;  // the stage-1 input of the step reversed last
+  DtArgs dt_next = {};             // and its dt work, done by the next primitives pass
   for (int si = (N - 1) / every; si >= 0; si--) {
     const int seg0 = si * every, len = std::min(every, N - seg0);
     // Segment si's states (recomputed from its checkpoint with the recorded
@@ -1163,7 +1189,8 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
       if (Unext) {
         f2.Vb = bars1;
         f2.src = Unext;
-        rc = stage_vjp<FOLD_STAGE1>(c, Yn, false, f2, dt, Fn, bars2, part2);
+        rc = stage_vjp<FOLD_STAGE1>(c, Yn, false, f2, dt, Fn, bars2, part2, 0, nullptr,
+                                    &dt_next);
       } else {
         rc = stage_vjp<FOLD_SEED>(c, Yn, false, f2, dt, Fn, bars2, part2);
       }
@@ -1189,13 +1216,17 @@ int gx_adjoint_backward(gx_ctx *c, const double *Ubar, double *P0bar, double *ga
         adj_fold_kernel<<<blocks_for(c->ncell), AB, 0, c->stream>>>(a, A->pg, bars1, out);
         HIP_TRY(hipGetLastError());
       }
-      // dt = ts(invdt(U^n))
+      // dt = ts(invdt(U^n)): the partials of both stages summed now, the
+      // rest by the next primitives pass (or here, on the first step)
       adj_partsum_kernel<<<PART_BLOCKS, AB, 0, c->stream>>>(A->partials, 6 * A->part_stride,
                                                             A->partials2);
       HIP_TRY(hipGetLastError());
-      adj_dt_kernel<<<1, AB, 0, c->stream>>>(a, A->partials2, PART_BLOCKS, A->ts[n],
-                                             A->invdt[n], A->argmax, out, A->gb);
-      HIP_TRY(hipGetLastError());
+      dt_next = DtArgs{a, A->partials2, PART_BLOCKS, A->ts[n], A->invdt[n], A->argmax, out,
+                       A->gb};
+      if (first) {
+        adj_dt_kernel<<<1, AB, 0, c->stream>>>(dt_next);
+        HIP_TRY(hipGetLastError());
+      }
       Unext = Un;
     }
   }
