@@ -17,7 +17,10 @@
 // ghost rule), one line pass per dimension (slopes, the face flux's reverse
 // sweep and the limiter's reverse along lines, neighbours by DPP lane
 // shifts), a fold of the ghost bars onto the cells they copy, and the
-// conserved -> primitive reverse.  The recorded shock flags are replayed as
+// conserved -> primitive reverse.  The fold of a stage, with the RK and
+// conserved-fix reverses that follow it, is done by the x pass of the stage
+// transposed next, before its sweep, so that its memory traffic runs beside
+// the sweep's arithmetic.  The recorded shock flags are replayed as
 // constants; dt and gamma bars are reduced per block, then in two levels, in
 // a fixed order.
 #include <hip/hip_runtime.h>
