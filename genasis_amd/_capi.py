@@ -46,7 +46,8 @@ EXPORTS = (
     "gx_compute_time_step_noalloc",
     "gx_step_rk2", "gx_sync", "gx_totals", "gx_set_state", "gx_set_primitive",
     "gx_get_conserved", "gx_get_primitive", "gx_get_flags", "gx_set_flags",
-    "gx_set_tuning", "gx_get_tuning", "gx_set_tile", "gx_get_tile", "gx_set_record_sources", "gx_get_sources",
+    "gx_set_tuning", "gx_get_tuning", "gx_set_tile", "gx_get_tile", "gx_get_geometry_pow2",
+    "gx_set_record_sources", "gx_get_sources",
     "gx_stream", "gx_set_profiling", "gx_kernel_times",
     "gx_self_test_division", "gx_last_error",
     "gx_get_boundary_fluence", "gx_compute_tally",
@@ -134,6 +135,7 @@ def load(path: str = LIB_PATH):
         "gx_get_tuning": (i, [vp, P(ctypes.c_int)]),
         "gx_set_tile": (i, [vp, i, i]),
         "gx_get_tile": (i, [vp, P(ctypes.c_int), P(ctypes.c_int)]),
+        "gx_get_geometry_pow2": (i, [vp, P(ctypes.c_int)]),
         "gx_set_record_sources": (i, [vp, i]),
         "gx_get_sources": (i, [vp, pd]),
         "gx_stream": (vp, [vp]),

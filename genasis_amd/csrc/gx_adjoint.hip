@@ -897,7 +897,10 @@ static int launch_stage1(gx_ctx *c, hipStream_t s, const double *src, bool prim,
   a.out2 = out2;
   a.flags = flags;
   stage_geometry(c, a);
-  stage_kernel<double, 1><<<a.nblk, NT, 0, s>>>(a);
+  if (c->geom.pow2)
+    stage_kernel<double, 1, false, false, Tile16x16, true><<<a.nblk, NT, 0, s>>>(a);
+  else
+    stage_kernel<double, 1><<<a.nblk, NT, 0, s>>>(a);
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }
@@ -913,7 +916,10 @@ static int launch_stage2(gx_ctx *c, hipStream_t s, const double *Y, const double
   a.flags = flags;
   stage_geometry(c, a);
   HIP_TRY(hipMemsetAsync(c->invdt, 0, sizeof(unsigned long long), s));
-  stage_kernel<double, 2><<<a.nblk, NT, 0, s>>>(a);
+  if (c->geom.pow2)
+    stage_kernel<double, 2, false, false, Tile16x16, true><<<a.nblk, NT, 0, s>>>(a);
+  else
+    stage_kernel<double, 2><<<a.nblk, NT, 0, s>>>(a);
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }

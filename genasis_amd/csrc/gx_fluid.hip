@@ -16,6 +16,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/gx_fluid.h"
@@ -147,10 +148,22 @@ static void launch_stage(const gx_ctx *c, int nblk, hipStream_t st, const StageA
       stage_kernel<R, STAGE, true><<<nblk, NT, 0, st>>>(a);
   } else if (para) {
     stage_kernel_parabolic<R, STAGE><<<nblk, NT, 0, st>>>(a);
-  } else if (stage_shape(c) == 1) {
-    stage_kernel<R, STAGE, false, false, Tile32x8><<<nblk, NT, 0, st>>>(a);
   } else {
-    stage_kernel<R, STAGE><<<nblk, NT, 0, st>>>(a);
+    if constexpr (std::is_same<R, double>::value) {
+      if (c->geom.pow2) {
+        // every geometry divisor a power of two: the same bits, one
+        // multiplication per division by the geometry
+        if (stage_shape(c) == 1)
+          stage_kernel<R, STAGE, false, false, Tile32x8, true><<<nblk, NT, 0, st>>>(a);
+        else
+          stage_kernel<R, STAGE, false, false, Tile16x16, true><<<nblk, NT, 0, st>>>(a);
+        return;
+      }
+    }
+    if (stage_shape(c) == 1)
+      stage_kernel<R, STAGE, false, false, Tile32x8><<<nblk, NT, 0, st>>>(a);
+    else
+      stage_kernel<R, STAGE><<<nblk, NT, 0, st>>>(a);
   }
 }
 
@@ -371,6 +384,14 @@ int gx_get_sources(gx_ctx *c, double *sources) {
     return fail(GX_E_STATE, "gx_get_sources: recording was never enabled (gx_set_record_sources)");
   return get_values(c->stream, c->fp32, sources, c->sources, 5 * (size_t)c->ncell);
   GX_API_END("gx_get_sources")
+}
+
+int gx_get_geometry_pow2(gx_ctx *c, int *pow2) {
+  GX_API_BEGIN
+  if (!c || !pow2) return fail(GX_E_ARG, "gx_get_geometry_pow2: null argument");
+  *pow2 = c->geom.pow2 ? 1 : 0;
+  return GX_OK;
+  GX_API_END("gx_get_geometry_pow2")
 }
 
 int gx_get_tuning(gx_ctx *c, int *zChunk) {

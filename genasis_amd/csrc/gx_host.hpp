@@ -136,6 +136,10 @@ struct ChartGeometry {
   int gs = 0;
   std::vector<double> a[3][N_GEO];
   std::vector<double> center[3];  // Center (Chart_Template.f90:827-841), padded
+  // every centre spacing, slope denominator and cell width is a power of two
+  // (then so is every cell volume, and every division by them is exact as a
+  // multiplication by the reciprocal: the stage kernels' P2 variant)
+  bool pow2 = false;
 };
 
 // Builds the geometry and checks the divisors of the exactly rounded
@@ -152,14 +156,23 @@ inline int make_geometry(const gx_grid &grid, ChartGeometry &cg, const char *who
   const double lo = 0x1p-500, hi = 0x1p122;
   double vmin = 1.0, vmax = 1.0;
   bool ok = true;
+  auto is_pow2 = [](double x) {
+    int e;
+    return x > 0.0 && frexp(x, &e) == 0.5;
+  };
+  cg.pow2 = true;
   for (int d = 0; d < 3; d++) {
     const std::vector<double> *g = cg.a[d];
     double wmin = HUGE_VAL, wmax = 0.0;
     for (int q = 0; q < cg.n[d] + 4; q++) {
-      for (int k : {G_DXL, G_DXR, G_DEN}) ok = ok && g[k][q] >= lo && g[k][q] <= hi;
+      for (int k : {G_DXL, G_DXR, G_DEN}) {
+        ok = ok && g[k][q] >= lo && g[k][q] <= hi;
+        cg.pow2 = cg.pow2 && is_pow2(g[k][q]);
+      }
       if (q >= 2 && q < cg.n[d] + 2) {
         wmin = fmin(wmin, g[G_WIDTH][q]);
         wmax = fmax(wmax, g[G_WIDTH][q]);
+        cg.pow2 = cg.pow2 && is_pow2(g[G_WIDTH][q]);
       }
     }
     vmin *= wmin;

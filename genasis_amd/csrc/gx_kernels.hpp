@@ -92,6 +92,20 @@ template <> struct Arith<float> {
   __device__ static __forceinline__ float root(float a) { return __builtin_amdgcn_sqrtf(a); }
 };
 
+// a / b for a divisor b of the chart geometry (a spacing, a limiter
+// denominator or a cell volume) with y = RN(1/b).  P2: every such divisor of
+// the chart is a normal power of two, so y = 1/b exactly and a * y is the
+// correctly rounded quotient for every a (a / 2^k and a * 2^-k are the same
+// real number, rounded once): one multiplication instead of the corrected
+// quotient, bit for bit the same value.
+template <bool P2, typename R>
+__device__ __forceinline__ R gdiv(R a, R b, R y) {
+  if constexpr (P2)
+    return a * y;
+  else
+    return Arith<R>::rdiv(a, b, y);
+}
+
 // Fluid_P_I % ComputeFromConservedCommon, primitive part (F/Fluid_P__Kernel.
 // f90:94-192): N, V, E from D, S, G with the density floor and E >= 0.  The
 // floor N_Min is compared in FP64 whatever the state's precision.
@@ -172,17 +186,17 @@ __device__ __forceinline__ void prim_from_cons<float>(const float U[5], float ma
 // Gradient_Kernel limited slope (OP/Calculus/Derivatives/Gradient_Kernel.f90:
 // 166-190), or the unlimited centred slope when UseLimiter is false.
 // The three divisions use the precomputed reciprocals of the geometry.
-template <typename R>
+template <bool P2 = false, typename R>
 __device__ __forceinline__ R slope(R vm, R v0, R vp, R dxl, R dxr, R dxl2, R dxr2,
                                    R den, R rdxl, R rdxr, R rden, R theta,
                                    int use_limiter) {
   const R dVL = v0 - vm;
   const R dVR = vp - v0;
-  const R central = Arith<R>::rdiv(dxr2 * dVL + dxl2 * dVR, den, rden);
+  const R central = gdiv<P2>(dxr2 * dVL + dxl2 * dVR, den, rden);
   if (!use_limiter) return central;
   return (copysign(R(0.5), dVL) + copysign(R(0.5), dVR)) *
-         fmin(fmin(fabs(Arith<R>::rdiv(theta * dVL, dxl, rdxl)),
-                   fabs(Arith<R>::rdiv(theta * dVR, dxr, rdxr))),
+         fmin(fmin(fabs(gdiv<P2>(theta * dVL, dxl, rdxl)),
+                   fabs(gdiv<P2>(theta * dVR, dxr, rdxr))),
               fabs(central));
 }
 
@@ -191,15 +205,15 @@ __device__ __forceinline__ R slope(R vm, R v0, R vp, R dxl, R dxr, R dxl2, R dxr
 // its upper neighbour bit for bit (dV and the centre spacing are the same
 // differences of the same values), so a march along one dimension computes
 // each once.
-template <typename R>
+template <bool P2 = false, typename R>
 __device__ __forceinline__ R slope_carry(R vm, R v0, R vp, R dxr, R dxl2, R dxr2, R den,
                                          R rdxr, R rden, R theta, int use_limiter, R *q) {
   const R dVL = v0 - vm;
   const R dVR = vp - v0;
-  const R central = Arith<R>::rdiv(dxr2 * dVL + dxl2 * dVR, den, rden);
+  const R central = gdiv<P2>(dxr2 * dVL + dxl2 * dVR, den, rden);
   if (!use_limiter) return central;
   const R qL = *q;
-  const R qR = fabs(Arith<R>::rdiv(theta * dVR, dxr, rdxr));
+  const R qR = fabs(gdiv<P2>(theta * dVR, dxr, rdxr));
   *q = qR;
   return (copysign(R(0.5), dVL) + copysign(R(0.5), dVR)) * fmin(fmin(qL, qR), fabs(central));
 }

@@ -223,33 +223,33 @@ __device__ __forceinline__ void load_prim(const StageArgs<R> &a, int x, int y,
   raw_to_prim(a, raw, flip, V);
 }
 
-template <typename R>
+template <bool P2 = false, typename R>
 __device__ __forceinline__ R slope_at(const StageArgs<R> &a, int d, int c, R vm,
                                       R v0, R vp) {
   const R *g = a.geo + d * N_GEO * a.gs + (c + 2);
   const int s = a.gs;
-  return slope(vm, v0, vp, g[G_DXL * s], g[G_DXR * s], g[G_DXL2 * s], g[G_DXR2 * s],
+  return slope<P2>(vm, v0, vp, g[G_DXL * s], g[G_DXR * s], g[G_DXL2 * s], g[G_DXR2 * s],
                g[G_DEN * s], g[G_RDXL * s], g[G_RDXR * s], g[G_RDEN * s],
                R(a.ph.theta), a.ph.use_limiter);
 }
 
 // slope_at with the left quotient carried in *q (slope_carry).
-template <typename R>
+template <bool P2 = false, typename R>
 __device__ __forceinline__ R slope_carry_at(const StageArgs<R> &a, int d, int c, R vm,
                                             R v0, R vp, R *q) {
   const R *g = a.geo + d * N_GEO * a.gs + (c + 2);
   const int s = a.gs;
-  return slope_carry(vm, v0, vp, g[G_DXR * s], g[G_DXL2 * s], g[G_DXR2 * s], g[G_DEN * s],
+  return slope_carry<P2>(vm, v0, vp, g[G_DXR * s], g[G_DXL2 * s], g[G_DXR2 * s], g[G_DEN * s],
                      g[G_RDXR * s], g[G_RDEN * s], R(a.ph.theta), a.ph.use_limiter, q);
 }
 
 // Left quotient |theta dVL / dxl| of cell c (slope_carry's *q).
-template <typename R>
+template <bool P2 = false, typename R>
 __device__ __forceinline__ R left_quotient_at(const StageArgs<R> &a, int d, int c, R vm,
                                               R v0) {
   const R *g = a.geo + d * N_GEO * a.gs + (c + 2);
   const int s = a.gs;
-  return fabs(Arith<R>::rdiv(R(a.ph.theta) * (v0 - vm), g[G_DXL * s], g[G_RDXL * s]));
+  return fabs(gdiv<P2>(R(a.ph.theta) * (v0 - vm), g[G_DXL * s], g[G_RDXL * s]));
 }
 
 // Parabolic face values of cell c of dimension d from the global geometry.
@@ -294,10 +294,10 @@ __device__ __forceinline__ void halo_cell(int t, int x0, int y0, int tw, int th,
 // bounds both dimensions: X >= Y).
 template <typename R, typename T = Tile16x16> using TileGeo = R[2][N_GEO][T::HX];
 
-template <typename R, typename G>
+template <bool P2 = false, typename R, typename G>
 __device__ __forceinline__ R slope_lds(const G &sG, int d, int l, R vm,
                                        R v0, R vp, R theta, int use_limiter) {
-  return slope(vm, v0, vp, sG[d][G_DXL][l], sG[d][G_DXR][l], sG[d][G_DXL2][l],
+  return slope<P2>(vm, v0, vp, sG[d][G_DXL][l], sG[d][G_DXR][l], sG[d][G_DXL2][l],
                sG[d][G_DXR2][l], sG[d][G_DEN][l], sG[d][G_RDXL][l],
                sG[d][G_RDXR][l], sG[d][G_RDEN][l], theta, use_limiter);
 }
@@ -386,9 +386,10 @@ template <typename R, typename T = Tile16x16> struct EdgeShared {
 // REC: record the divergence sources (a.sources).  BRICK: a brick of a
 // chart split into bricks (ghosts of BC_NEIGHBOR faces from the halos, and
 // the flags of the faces beyond the brick from the flag halo).  T: the
-// tile's shape.
+// tile's shape.  P2: every divisor of the chart's geometry is a power of two
+// (gdiv).
 template <typename R, int STAGE, int RECON = RECON_LINEAR, bool REC = false,
-          bool BRICK = false, typename T = Tile16x16>
+          bool BRICK = false, typename T = Tile16x16, bool P2 = false>
 __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0, int z0,
                                            StageShared<R, T> &L,
                                            EdgeShared<R, T> *E = nullptr) {
@@ -469,12 +470,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
 #pragma unroll
       for (int f = 0; f < 5; f++) {
         if constexpr (ZCARRY) {
-          qz[f] = left_quotient_at(a, 2, z0 - 1, Vm2[f], Vm1[f]);
-          sm1[f] = slope_carry_at(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f], &qz[f]);
-          sk[f] = slope_carry_at(a, 2, z0, Vm1[f], Vk[f], Vk1[f], &qz[f]);
+          qz[f] = left_quotient_at<P2>(a, 2, z0 - 1, Vm2[f], Vm1[f]);
+          sm1[f] = slope_carry_at<P2>(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f], &qz[f]);
+          sk[f] = slope_carry_at<P2>(a, 2, z0, Vm1[f], Vk[f], Vk1[f], &qz[f]);
         } else {
-          sm1[f] = slope_at(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f]);
-          sk[f] = slope_at(a, 2, z0, Vm1[f], Vk[f], Vk1[f]);
+          sm1[f] = slope_at<P2>(a, 2, z0 - 1, Vm2[f], Vm1[f], Vk[f]);
+          sk[f] = slope_at<P2>(a, 2, z0, Vm1[f], Vk[f], Vk1[f]);
         }
         WL[f] = Vm1[f] + geo(a, 2, G_WR, z0 - 1 + 2) * sm1[f];
         WR[f] = Vk[f] - geo(a, 2, G_WL, z0 + 2) * sk[f];
@@ -567,12 +568,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
       } else {
 #pragma unroll
         for (int f = 0; f < 5; f++)
-          sSx[f][ty * SXW + tx + 1] = slope_lds(sG, 0, tx + 2, sP[f][bx - 1], sP[f][bx],
+          sSx[f][ty * SXW + tx + 1] = slope_lds<P2>(sG, 0, tx + 2, sP[f][bx - 1], sP[f][bx],
                                                 sP[f][bx + 1], theta, ph.use_limiter);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int f = 0; f < 5; f++)
-          sSy[f][(ty + 1) * TX + tx] = slope_lds(sG, 1, ty + 2, sP[f][bx - HX], sP[f][bx],
+          sSy[f][(ty + 1) * TX + tx] = slope_lds<P2>(sG, 1, ty + 2, sP[f][bx - HX], sP[f][bx],
                                                  sP[f][bx + HX], theta, ph.use_limiter);
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -629,7 +630,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         }
       } else if constexpr (SPREAD) {
         for (int f = wave; f < 5; f += wave == 3 ? 1 : 5) {
-          const R sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
+          const R sv = slope_lds<P2>(sG, gd, l, sP[f][base - stride], sP[f][base],
                                  sP[f][base + stride], theta, ph.use_limiter);
           if (isx)
             sSx[f][out] = sv;
@@ -639,7 +640,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
       } else {
 #pragma unroll
         for (int f = 0; f < 5; f++) {
-          const R sv = slope_lds(sG, gd, l, sP[f][base - stride], sP[f][base],
+          const R sv = slope_lds<P2>(sG, gd, l, sP[f][base - stride], sP[f][base],
                                  sP[f][base + stride], theta, ph.use_limiter);
           if (isx)
             sSx[f][out] = sv;
@@ -727,9 +728,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
           for (int f = 0; f < 5; f++) {
             const R vk = sP[f][ctr];
             if constexpr (ZCARRY)
-              sk1[f] = slope_carry_at(a, 2, k + 1, vk, Vk1[f], Vk2[f], &qz[f]);
+              sk1[f] = slope_carry_at<P2>(a, 2, k + 1, vk, Vk1[f], Vk2[f], &qz[f]);
             else
-              sk1[f] = slope_at(a, 2, k + 1, vk, Vk1[f], Vk2[f]);
+              sk1[f] = slope_at<P2>(a, 2, k + 1, vk, Vk1[f], Vk2[f]);
             WL[f] = vk + geo(a, 2, G_WR, k + 2) * sk[f];
             WR[f] = Vk1[f] - geo(a, 2, G_WL, k + 1 + 2) * sk1[f];
           }
@@ -742,7 +743,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
       const R dX = sG[0][G_WIDTH][tx + 2], dY = sG[1][G_WIDTH][ty + 2],
               dZ = geo(a, 2, G_WIDTH, k + 2);
       const R Vol = dX * dY * dZ;
-      const R rVol = Arith<R>::recip(Vol);
+      R rVol;
+      if constexpr (P2) {
+        // Vol = 2^(e-1) exactly (frexp gives the mantissa 1/2): 1/Vol = 2^(1-e)
+        int e;
+        (void)frexp(Vol, &e);
+        rVol = ldexp(R(1), 1 - e);
+      } else {
+        rVol = Arith<R>::recip(Vol);
+      }
       const R A1 = dY * dZ, A2 = dZ * dX, A3 = dX * dY;
       const R dt = R(a.dt);
       const unsigned ic = lin(a.n, x, y, k);
@@ -752,9 +761,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
         const R fxl = sFx[f][ty * FXW + tx], fxh = sFx[f][ty * FXW + tx + 1];
         const R fyl = sFy[f][ty * TX + tx], fyh = sFy[f][(ty + 1) * TX + tx];
         R K = R(0);
-        K = K + Arith<R>::rdiv(dt * (A1 * fxl - A1 * fxh), Vol, rVol);
-        K = K + Arith<R>::rdiv(dt * (A2 * fyl - A2 * fyh), Vol, rVol);
-        K = K + Arith<R>::rdiv(dt * (A3 * Flo[f] - A3 * Fhi[f]), Vol, rVol);
+        K = K + gdiv<P2>(dt * (A1 * fxl - A1 * fxh), Vol, rVol);
+        K = K + gdiv<P2>(dt * (A2 * fyl - A2 * fyh), Vol, rVol);
+        K = K + gdiv<P2>(dt * (A3 * Flo[f] - A3 * Fhi[f]), Vol, rVol);
         Kc[f] = K;
       }
       if constexpr (REC) {
@@ -835,12 +844,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<R> &a, int x0, int y0
 }
 
 template <typename R, int STAGE, bool REC = false, bool BRICK = false,
-          typename T = Tile16x16>
+          typename T = Tile16x16, bool P2 = false>
 __global__ __launch_bounds__(NT, (StageOccupancy<R, !BRICK>::waves)) void stage_kernel(
     StageArgs<R> a) {
   __shared__ StageShared<R, T> L;
   const int tile = tile_of_block(blockIdx.x, a.nblk);
-  stage_tile<R, STAGE, RECON_LINEAR, REC, BRICK, T>(a, (tile % a.nbx) * T::X,
+  stage_tile<R, STAGE, RECON_LINEAR, REC, BRICK, T, P2>(a, (tile % a.nbx) * T::X,
                                                     ((tile / a.nbx) % a.nby) * T::Y,
                                                     (tile / (a.nbx * a.nby)) * a.tz, L);
 }

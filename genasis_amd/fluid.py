@@ -252,6 +252,14 @@ class Step_RK2_C_ASC:
                     "gx_get_tile")
         return x.value, y.value
 
+    def geometry_pow2(self) -> bool:
+        """True when every divisor of the chart's geometry is a power of two
+        (the FP64 stage kernels then multiply by exact reciprocals)."""
+        v = ctypes.c_int()
+        _capi.check(self.lib.gx_get_geometry_pow2(self._h, ctypes.byref(v)),
+                    "gx_get_geometry_pow2")
+        return bool(v.value)
+
     def record_sources(self, on: bool = True):
         """Record Current % Sources of the divergence in every step
         (RecordDivergence); turning it on clears them."""

@@ -309,6 +309,14 @@ int gx_get_tuning(gx_ctx *ctx, int *zChunk);
 int gx_set_tile(gx_ctx *ctx, int tileX, int tileY);
 int gx_get_tile(gx_ctx *ctx, int *tileX, int *tileY);
 
+/* No counterpart in the reference: *pow2 = 1 when every centre spacing, slope
+ * denominator and cell width of the chart is a power of two (a uniform grid
+ * of 2^k cells per unit length, as the unit cube with a power-of-two cell
+ * count).  Every division by the geometry is then exact as a multiplication
+ * by the reciprocal, and the FP64 LINEAR stage kernels take that form: the
+ * same bits, fewer instructions.  Otherwise 0 (the corrected quotient). */
+int gx_get_geometry_pow2(gx_ctx *ctx, int *pow2);
+
 /* Current % Sources of the divergence (ApplyDivergence_C's recording,
  * Step_RK_C_ASC__Template.f90:1548-1595 with RecordDivergenceKernel,
  * Step_RK_C_ASC__Kernel.f90:12-46): with recording on, every gx_step_rk2
