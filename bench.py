@@ -572,6 +572,7 @@ def main():
     if args.z_chunk > 0:
         stepper.set_tuning(args.z_chunk)
     z_chunk = stepper.get_tuning()
+    geometry_pow2 = stepper.geometry_pow2()
     stepper.set_primitive(*problems.kelvin_helmholtz(chart, seed=1234, padded=False))
     total0 = stepper.totals()
     tally0 = stepper.compute_tally(False)
@@ -673,6 +674,9 @@ def main():
         "config": {"workload": f"Kelvin-Helmholtz {n}^3 periodic, PLM(MC theta=1.4)"
                                f"+HLLC, RK2, CFL 0.8, {args.precision}",
                    "cells": chart.n_proper, "z_chunk": z_chunk,
+                   # every geometry divisor a power of two (unit cube, 2^k
+                   # cells): divisions by it are exact reciprocal products
+                   "geometry_pow2": geometry_pow2,
                    "tile": [args.tile, 256 // args.tile]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
