@@ -179,10 +179,12 @@ __device__ inline double cell_width(const StageArgs<double> &a, int e, int c) {
 // bars in the segment that owns its cell c.
 constexpr int LINE_OUT = 58;  // complete cells per wave
 
-// a/b exactly as the forward forms it (div_rcp, y = RN(1/b)), so that the
-// limiter picks the forward's branch.
+// a/b exactly as the forward forms it (div_rcp, y = RN(1/b), or a * y for a
+// power-of-two geometry: gdiv), so that the limiter picks the forward's
+// branch.
+template <bool P2>
 __device__ __forceinline__ double quot_rev(double a, double b, double y) {
-  return div_rcp(a, b, y);
+  return gdiv<P2>(a, b, y);
 }
 
 // The limited slope of one cell and its reverse multipliers: s = slope(...)
@@ -200,6 +202,7 @@ struct SlopeFactors {
   double cl, cr, tl, tr;
 };
 
+template <bool P2>
 __device__ __forceinline__ double slope_fwd_rev(double vm, double v0, double vp, double dxl,
                                                 double dxr, double dxl2, double dxr2,
                                                 double den, double rdxl, double rdxr,
@@ -207,13 +210,14 @@ __device__ __forceinline__ double slope_fwd_rev(double vm, double v0, double vp,
                                                 const SlopeFactors &m, double &kl,
                                                 double &kr) {
   const double dVL = v0 - vm, dVR = vp - v0;
-  const double c = quot_rev(dxr2 * dVL + dxl2 * dVR, den, rden);
+  const double c = quot_rev<P2>(dxr2 * dVL + dxl2 * dVR, den, rden);
   if (!use_limiter) {  // uniform over the launch
     kl = m.cl;
     kr = m.cr;
     return c;
   }
-  const double ql = quot_rev(theta * dVL, dxl, rdxl), qr = quot_rev(theta * dVR, dxr, rdxr);
+  const double ql = quot_rev<P2>(theta * dVL, dxl, rdxl);
+  const double qr = quot_rev<P2>(theta * dVR, dxr, rdxr);
   const double tl = fabs(ql), tr = fabs(qr), ac = fabs(c);
   // sf = sign(1/2, dVL) + sign(1/2, dVR), in {-1, 0, 1}: products with it
   // are exact
@@ -255,7 +259,7 @@ __device__ __forceinline__ double lane_down(double v) {
 // get_K(K) gives the lane's stage bar K of its cell (0 off the interior); it
 // is called after the slopes, so the loads behind it overlap their
 // arithmetic.
-template <int D, typename GetK>
+template <int D, bool P2, typename GetK>
 __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int c, int lane,
                                          bool own, double A, double dt, const double V[5],
                                          GetK get_K, bool df, double volc,
@@ -281,7 +285,7 @@ __device__ __forceinline__ void line_vjp(const StageArgs<double> &a, int n, int 
     const SlopeFactors m = {dxr2 * rden, dxl2 * rden, a.ph.theta * rdxl, a.ph.theta * rdxr};
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      s[f] = slope_fwd_rev(Vm[f], V[f], Vq[f], dxl, dxr, dxl2, dxr2, den, rdxl, rdxr, rden,
+      s[f] = slope_fwd_rev<P2>(Vm[f], V[f], Vq[f], dxl, dxr, dxl2, dxr2, den, rdxl, rdxr, rden,
                            a.ph.theta, a.ph.use_limiter, m, kl[f], kr[f]);
       if (!has_slope) s[f] = kl[f] = kr[f] = 0.0;
     }
@@ -504,7 +508,7 @@ __device__ __forceinline__ void fold_finish(const StageArgs<double> &a, const Pa
 // row's cell range, and the lanes of another row contribute zeros there.
 // Before the sweep the lanes fold in the finished stage (fold_cell), so that
 // the fold's memory traffic overlaps the sweep's arithmetic.
-template <int MODE>
+template <int MODE, bool P2>
 __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, PadGeom g,
                                                         const double *Vp, FoldIn fi,
                                                         double dt, double *Vb,
@@ -551,7 +555,7 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
       for (int f = 0; f < 5; f++) K[f] = 0.0;
       if (interior) fold_finish<MODE>(a, g, fi, ic, own, fl, fv, K);
     };
-    line_vjp<0>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
+    line_vjp<0, P2>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
     if (own) store5(Vb, pl + c, out);  // this pass's own array
   }
   const double s1 = block_sum(dtb, sh);
@@ -570,7 +574,7 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
 // the ghost positions along D) to an array of its own, which the fold adds
 // to the other passes', so it reads no earlier bars.
 constexpr int DXW = AB / WAVE;  // x columns per block
-template <int D>
+template <int D, bool P2>
 __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, PadGeom g,
                                                         const double *Vp, const double *Kb,
                                                         double dt, double *Vb,
@@ -626,7 +630,7 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
 #pragma unroll
       for (int f = 0; f < 5; f++) Kc[f] = K[f];
     };
-    line_vjp<D>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
+    line_vjp<D, P2>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
     if (own) store5(Vb, pv, out);  // this pass's own array
   }
   const double s1 = block_sum(dtb, sh);
@@ -955,14 +959,19 @@ static int stage_vjp(gx_ctx *c, const double *src, bool prim, const FoldIn &fi, 
                                                                  dtw ? *dtw : DtArgs{}, wdt);
   HIP_TRY(hipGetLastError());
   // cells with two coordinates outside are never written nor read
-  adj_line_x_kernel<MODE><<<line_blocks(g, 0), AB, 0, c->stream>>>(a, g, A->Vp, fi, dt,
-                                                                   Vb.d[0], part);
-  HIP_TRY(hipGetLastError());
-  adj_line_t_kernel<1><<<line_blocks(g, 1), AB, 0, c->stream>>>(
-      a, g, A->Vp, fi.Kout, dt, Vb.d[1], part + 2 * (size_t)A->part_stride);
-  HIP_TRY(hipGetLastError());
-  adj_line_t_kernel<2><<<line_blocks(g, 2), AB, 0, c->stream>>>(
-      a, g, A->Vp, fi.Kout, dt, Vb.d[2], part + 4 * (size_t)A->part_stride);
+  auto passes = [&](auto p2) {
+    constexpr bool P2 = decltype(p2)::value;
+    adj_line_x_kernel<MODE, P2><<<line_blocks(g, 0), AB, 0, c->stream>>>(a, g, A->Vp, fi, dt,
+                                                                         Vb.d[0], part);
+    adj_line_t_kernel<1, P2><<<line_blocks(g, 1), AB, 0, c->stream>>>(
+        a, g, A->Vp, fi.Kout, dt, Vb.d[1], part + 2 * (size_t)A->part_stride);
+    adj_line_t_kernel<2, P2><<<line_blocks(g, 2), AB, 0, c->stream>>>(
+        a, g, A->Vp, fi.Kout, dt, Vb.d[2], part + 4 * (size_t)A->part_stride);
+  };
+  if (c->geom.pow2)
+    passes(std::true_type{});
+  else
+    passes(std::false_type{});
   HIP_TRY(hipGetLastError());
   return GX_OK;
 }

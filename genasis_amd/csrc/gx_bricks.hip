@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/gx_fluid.h"
@@ -294,7 +295,9 @@ __device__ __forceinline__ int launch_brick(const BrickLaunch<R> &L, int t) {
   return b;
 }
 
-template <typename R, int STAGE, typename T>
+// P2: every divisor of the chart's geometry is a power of two (gdiv); the
+// bricks' geometries are slices of the chart's.
+template <typename R, int STAGE, typename T, bool P2 = false>
 __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void bricks_stage_kernel(
     BrickLaunch<R> L) {
   __shared__ StageShared<R, T> S;
@@ -302,7 +305,7 @@ __global__ __launch_bounds__(NT, StageOccupancy<R>::waves) void bricks_stage_ker
   const int b = launch_brick(L, t);
   const StageArgs<R> &a = L.a[b];
   const int lt = t - L.tstart[b];
-  stage_tile<R, STAGE, RECON_LINEAR, false, true, T>(a, (lt % a.nbx) * T::X,
+  stage_tile<R, STAGE, RECON_LINEAR, false, true, T, P2>(a, (lt % a.nbx) * T::X,
                                                      ((lt / a.nbx) % a.nby) * T::Y,
                                                      (lt / (a.nbx * a.nby)) * a.tz, S);
 }
@@ -378,6 +381,15 @@ static int launch_bricks_stage(gx_ctx *c, Set set) {
   const int shape = stage_shape(c);
   auto tiles = [](const StageArgs<R> &a) { return (long long)a.nblk; };
   return launch_groups<R>(c, set, tiles, [&](const BrickLaunch<R> &L) {
+    if constexpr (std::is_same<R, double>::value) {
+      if (c->geom.pow2) {
+        if (shape == 1)
+          bricks_stage_kernel<R, STAGE, Tile32x8, true><<<L.ntile, NT, 0, c->stream>>>(L);
+        else
+          bricks_stage_kernel<R, STAGE, Tile16x16, true><<<L.ntile, NT, 0, c->stream>>>(L);
+        return;
+      }
+    }
     if (shape == 1)
       bricks_stage_kernel<R, STAGE, Tile32x8><<<L.ntile, NT, 0, c->stream>>>(L);
     else
