@@ -44,12 +44,36 @@ namespace gx {
 
 constexpr int AB = 256;  // threads per block of the adjoint kernels
 
-// The reverse sweeps' work arrays (primitives and their bars on the padded
-// layout, the stage bars K1b, K2b) hold the five values of a cell together,
-// padded to six (48 bytes, three 16-byte loads): a y or z line pass, whose
-// lanes walk a line across rows or planes, then touches one cache line per
-// cell and array instead of five.
-constexpr int AOS = 6;
+// The reverse sweeps' work arrays hold the five values of a cell together,
+// so that a y or z line pass, whose lanes walk a line across rows or planes,
+// touches one or two cache lines per cell and array instead of five: the
+// primitives and their bars on the padded layout in 40-byte records (two
+// 16-byte and one 8-byte access, 8-byte aligned), the stage bars K1b, K2b in
+// 48-byte records whose sixth slot carries the cell's shock-flag byte.
+constexpr int AOS = 6;   // stage bars
+constexpr int PAD5 = 5;  // padded primitives and bars
+
+struct __attribute__((aligned(8))) Pair8 {  // two doubles, 8-byte aligned
+  double x, y;
+};
+
+__device__ __forceinline__ void loadp(const double *base, long long pos, double v[5]) {
+  const double *q = base + PAD5 * pos;
+  const Pair8 a = reinterpret_cast<const Pair8 *>(q)[0];
+  const Pair8 b = reinterpret_cast<const Pair8 *>(q + 2)[0];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+  v[4] = q[4];
+}
+
+__device__ __forceinline__ void storep(double *base, long long pos, const double v[5]) {
+  double *q = base + PAD5 * pos;
+  reinterpret_cast<Pair8 *>(q)[0] = Pair8{v[0], v[1]};
+  reinterpret_cast<Pair8 *>(q + 2)[0] = Pair8{v[2], v[3]};
+  q[4] = v[4];
+}
 
 __device__ __forceinline__ void load5(const double *base, long long cell, double v[5]) {
   const double2 *p = reinterpret_cast<const double2 *>(base + AOS * cell);
@@ -59,13 +83,6 @@ __device__ __forceinline__ void load5(const double *base, long long cell, double
   v[2] = b.x;
   v[3] = b.y;
   v[4] = c.x;
-}
-
-__device__ __forceinline__ void store5(double *base, long long cell, const double v[5]) {
-  double2 *p = reinterpret_cast<double2 *>(base + AOS * cell);
-  p[0] = make_double2(v[0], v[1]);
-  p[1] = make_double2(v[2], v[3]);
-  p[2] = make_double2(v[4], 0.0);
 }
 
 // A stage bar K of a cell with the cell's shock-flag byte in the sixth slot,
@@ -390,9 +407,9 @@ __device__ __forceinline__ void gather_bars(const StageArgs<double> &a, const Pa
                                             double V[5]) {
   {
     double vy[5], vz[5];
-    load5(Vb.d[0], p, V);
-    load5(Vb.d[1], p, vy);
-    load5(Vb.d[2], p, vz);
+    loadp(Vb.d[0], p, V);
+    loadp(Vb.d[1], p, vy);
+    loadp(Vb.d[2], p, vz);
 #pragma unroll
     for (int f = 0; f < 5; f++) V[f] = (V[f] + vy[f]) + vz[f];
   }
@@ -416,7 +433,7 @@ __device__ __forceinline__ void gather_bars(const StageArgs<double> &a, const Pa
     for (int k = 0; k < ng; k++) {
       const long long pg = p + (long long)(gh[k] - c) * g.s[d];
       double gv[5];
-      load5(Vb.d[d], pg, gv);
+      loadp(Vb.d[d], pg, gv);
 #pragma unroll
       for (int f = 0; f < 5; f++) V[f] += (fl[k] && f == 1 + d) ? -gv[f] : gv[f];
     }
@@ -533,7 +550,7 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
     double volc = 0.0, volm = 0.0;
     const double wy = cell_width(a, 1, y), wz = cell_width(a, 2, z);
     const double A = wy * wz;
-    load5(Vp, pl + (valid ? c : 0), V);
+    loadp(Vp, pl + (valid ? c : 0), V);
 #pragma unroll
     for (int f = 0; f < 5; f++) V[f] = valid ? V[f] : 0.0;
     const bool own = valid && lane >= 3 && lane <= 60;
@@ -556,7 +573,7 @@ __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, Pad
       if (interior) fold_finish<MODE>(a, g, fi, ic, own, fl, fv, K);
     };
     line_vjp<0, P2>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
-    if (own) store5(Vb, pl + c, out);  // this pass's own array
+    if (own) storep(Vb, pl + c, out);  // this pass's own array
   }
   const double s1 = block_sum(dtb, sh);
   const double s2 = block_sum(gbs, sh);
@@ -606,7 +623,7 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
     const bool interior = c >= 0 && c < n;
     const bool own = valid && lane >= 3 && lane <= 60;
     // pv is in the padded array for every lane (q[D] = 0 off the sequence)
-    load5(Vp, pv, V);
+    loadp(Vp, pv, V);
 #pragma unroll
     for (int f = 0; f < 5; f++) {
       V[f] = valid ? V[f] : 0.0;
@@ -631,7 +648,7 @@ __global__ __launch_bounds__(AB) void adj_line_t_kernel(StageArgs<double> a, Pad
       for (int f = 0; f < 5; f++) Kc[f] = K[f];
     };
     line_vjp<D, P2>(a, n, c, lane, own, A, dt, V, get_K, df, volc, volm, out, dtb, gbs, kst);
-    if (own) store5(Vb, pv, out);  // this pass's own array
+    if (own) storep(Vb, pv, out);  // this pass's own array
   }
   const double s1 = block_sum(dtb, sh);
   const double s2 = block_sum(gbs, sh);
@@ -770,7 +787,7 @@ __global__ __launch_bounds__(AB) void adj_prims_kernel(StageArgs<double> a, PadG
     const int out = (x < 0 || x >= g.n[0]) + (y < 0 || y >= g.n[1]) + (z < 0 || z >= g.n[2]);
     double V[5] = {0, 0, 0, 0, 0};
     if (out <= 1) load_prim(a, x, y, z, V);
-    store5(Vp, t, V);
+    storep(Vp, t, V);
     if (idx && out == 0) {
       const double tc = cfl_term(V, a.ph.gamma, a.ph.mass, geo(a, 0, G_WIDTH, x + 2),
                                  geo(a, 1, G_WIDTH, y + 2), geo(a, 2, G_WIDTH, z + 2));
@@ -1048,12 +1065,13 @@ static int ensure_state(gx_ctx *c, int nsteps, int every) {
   HIP_TRY(hipStreamCreate(&A->side));
   HIP_TRY(hipMalloc(&A->stepF, (size_t)c->ncell * nsteps));
   for (double **p : {&A->trash, &A->Ub, &A->P0b}) HIP_TRY(hipMalloc(p, s5));
-  // cell-interleaved work arrays (AOS values per cell)
+  // cell-interleaved work arrays: stage bars in AOS records per cell,
+  // padded primitives and bars in PAD5 records per position
   for (double **p : {&A->K1b, &A->K2b})
     HIP_TRY(hipMalloc(p, AOS * sizeof(double) * (size_t)c->ncell));
   for (double **p : {&A->Vp, &A->Vb[0][0], &A->Vb[0][1], &A->Vb[0][2], &A->Vb[1][0],
                      &A->Vb[1][1], &A->Vb[1][2]})
-    HIP_TRY(hipMalloc(p, AOS * sizeof(double) * (size_t)g.np));
+    HIP_TRY(hipMalloc(p, PAD5 * sizeof(double) * (size_t)g.np));
   A->part_stride = 0;
   for (int d = 0; d < 3; d++) A->part_stride = std::max(A->part_stride, line_blocks(g, d));
   // every line block writes its pair; the tails of the shorter passes stay 0
