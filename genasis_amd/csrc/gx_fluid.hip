@@ -515,12 +515,6 @@ int gx_set_primitive(gx_ctx *c, const double *primitive) {
 static const int PRIM_FIELDS[5] = {GX_COMOVING_BARYON_DENSITY, GX_VELOCITY_U_1,
                                    GX_VELOCITY_U_2, GX_VELOCITY_U_3,
                                    GX_INTERNAL_ENERGY};
-static const int CONS_FIELDS[5] = {GX_CONSERVED_BARYON_DENSITY,
-                                   GX_MOMENTUM_DENSITY_D_1,
-                                   GX_MOMENTUM_DENSITY_D_2,
-                                   GX_MOMENTUM_DENSITY_D_3,
-                                   GX_CONSERVED_ENERGY};
-
 int gx_upload(gx_ctx *c, const double *value, long long ld, const int *map) {
   GX_API_BEGIN
   if (!c || !value || !map) return fail(GX_E_ARG, "gx_upload: null argument");

@@ -122,7 +122,7 @@ __device__ inline void stencil_flux(const StageArgs<R> &a, int cl, const R Vm2[5
   if constexpr (RECON == RECON_PARABOLIC) {
 #pragma unroll
     for (int f = 0; f < 5; f++) {
-      R iL, iR, oR;
+      R iL, oR;
       parabolic_at(a, D, cl, Vm2[f], Vm1[f], V0[f], &iL, &WL[f]);
       parabolic_at(a, D, cr, Vm1[f], V0[f], Vp1[f], &WR[f], &oR);
     }

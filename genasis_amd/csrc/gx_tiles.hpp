@@ -1093,12 +1093,12 @@ __device__ __forceinline__ void shock_flags_tile(const StageArgs<R> &a, int x0, 
           const uint8_t *Bm = sB[(kf + 2) % 3], *B0 = sB[(kf + 3) % 3],
                         *Bp = sB[(kf + 4) % 3];
           const int i = (ty + 1) * BW + (tx + 1);
-          const unsigned fx = (B0[i] | B0[i + BW] | B0[i - 1] | B0[i + BW - 1]) & 2u |
-                              (B0[i] | Bp[i] | B0[i - 1] | Bp[i - 1]) & 4u;
-          const unsigned fy = (B0[i] | B0[i + 1] | B0[i - BW] | B0[i + 1 - BW]) & 1u |
-                              (B0[i] | Bp[i] | B0[i - BW] | Bp[i - BW]) & 4u;
-          const unsigned fz = (B0[i] | B0[i + 1] | Bm[i] | Bm[i + 1]) & 1u |
-                              (B0[i] | B0[i + BW] | Bm[i] | Bm[i + BW]) & 2u;
+          const unsigned fx = ((B0[i] | B0[i + BW] | B0[i - 1] | B0[i + BW - 1]) & 2u) |
+                              ((B0[i] | Bp[i] | B0[i - 1] | Bp[i - 1]) & 4u);
+          const unsigned fy = ((B0[i] | B0[i + 1] | B0[i - BW] | B0[i + 1 - BW]) & 1u) |
+                              ((B0[i] | Bp[i] | B0[i - BW] | Bp[i - BW]) & 4u);
+          const unsigned fz = ((B0[i] | B0[i + 1] | Bm[i] | Bm[i + 1]) & 1u) |
+                              ((B0[i] | B0[i + BW] | Bm[i] | Bm[i + BW]) & 2u);
           out = (uint8_t)((fx ? 1u : 0u) | (fy ? 2u : 0u) | (fz ? 4u : 0u));
         }
         flags[lin(a.n, x, y, kf)] = out;
