@@ -140,12 +140,12 @@ struct AdjointState {
   hipEvent_t mark = nullptr;
   uint8_t *stepF = nullptr;    // the shock flags each step used, nsteps ncell
   double *trash = nullptr;
-  double *Vp = nullptr;               // primitives of a stage input, AOS np
-  // bars of stage 2 and stage 1 (AOS np), one array per line pass: the x
+  double *Vp = nullptr;               // primitives of a stage input, PAD5 np
+  // bars of stage 2 and stage 1 (PAD5 np), one array per line pass: the x
   // pass of one stage folds in the other's three
   double *Vb[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double *Ub = nullptr, *P0b = nullptr;    // 5 ncell, field slabs
-  double *K1b = nullptr, *K2b = nullptr;   // AOS ncell, cell-interleaved
+  double *K1b = nullptr, *K2b = nullptr;   // AOS ncell, with the flag byte
   bool seed_ready = false;     // Ub holds dJ/dU_final (gx_adjoint_kinetic_energy)
   bool bar_ready = false;      // P0b holds the last reverse pass's result
   double *partials = nullptr;  // 2 per line block, both stages, three dimensions
@@ -452,7 +452,7 @@ __device__ __forceinline__ void gather_bars(const StageArgs<double> &a, const Pa
 enum { FOLD_SEED = 0, FOLD_STAGE1 = 1, FOLD_STAGE2 = 2 };
 
 struct FoldIn {
-  StageBars Vb;       // the finished stage's bars on the padded layout (AOS)
+  StageBars Vb;       // the finished stage's bars on the padded layout (PAD5)
   const double *src;  // that stage's input (conserved field slabs)
   double *out;        // Ub (field slabs)
   const double *K2b;  // FOLD_STAGE2: the stage-2 bar (AOS)
@@ -523,8 +523,9 @@ __device__ __forceinline__ void fold_finish(const StageArgs<double> &a, const Pa
 // (lanes along x, coalesced), so a wave may hold the end of one row and the
 // start of the next.  Every use of a neighbour lane is guarded by the own
 // row's cell range, and the lanes of another row contribute zeros there.
-// Before the sweep the lanes fold in the finished stage (fold_cell), so that
-// the fold's memory traffic overlaps the sweep's arithmetic.
+// The lanes fold in the finished stage: fold_load issues its loads before
+// the slopes and fold_finish (through get_K) uses them after, so that the
+// fold's memory traffic overlaps the sweep's arithmetic.
 template <int MODE, bool P2>
 __global__ __launch_bounds__(AB) void adj_line_x_kernel(StageArgs<double> a, PadGeom g,
                                                         const double *Vp, FoldIn fi,
