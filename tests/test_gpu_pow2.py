@@ -46,3 +46,14 @@ def test_general_geometry_bitwise():
     o, g = _pair(chart, prim)
     assert not g.geometry_pow2()
     _run_lockstep(o, g, chart, 6, "general")
+
+
+@pytest.mark.parametrize("riemann,use_limiter", [("HLL", True), ("HLLC", False)])
+def test_power_of_two_geometry_hll_and_unlimited_bitwise(riemann, use_limiter):
+    """The centred slope without the limiter, and HLL everywhere, in the
+    power-of-two form."""
+    chart = Chart((16, 32, 8), lo=(0.0, -0.25, 0.0), hi=(0.5, 0.25, 0.25), bc=MIXED)
+    prim = _random_state(chart, 12)
+    o, g = _pair(chart, prim, riemann=riemann, use_limiter=use_limiter)
+    assert g.geometry_pow2()
+    _run_lockstep(o, g, chart, 5, f"pow2 {riemann} limiter={use_limiter}")
