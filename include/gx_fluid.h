@@ -313,8 +313,10 @@ int gx_get_tile(gx_ctx *ctx, int *tileX, int *tileY);
  * denominator and cell width of the chart is a power of two (a uniform grid
  * of 2^k cells per unit length, as the unit cube with a power-of-two cell
  * count).  Every division by the geometry is then exact as a multiplication
- * by the reciprocal, and the FP64 LINEAR stage kernels take that form: the
- * same bits, fewer instructions.  Otherwise 0 (the corrected quotient). */
+ * by the reciprocal, and the FP64 LINEAR stage kernels (single charts,
+ * bricks, the adjoint's line passes) take that form: the same bits, fewer
+ * instructions.  Steps that record the divergence sources, PARABOLIC steps
+ * and batches keep the corrected quotient.  Otherwise 0. */
 int gx_get_geometry_pow2(gx_ctx *ctx, int *pow2);
 
 /* Current % Sources of the divergence (ApplyDivergence_C's recording,
