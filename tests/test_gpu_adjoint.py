@@ -47,10 +47,14 @@ def test_kh_gradient_matches_autograd():
     assert loss2 == loss and np.array_equal(dN, pb[0]) and dg == gb
 
 
+@pytest.mark.parametrize("n", [(11, 9, 7), (16, 8, 8)])
 @pytest.mark.parametrize("riemann", ["HLLC", "HLL"])
-def test_general_functional_mixed_boundaries(riemann):
+def test_general_functional_mixed_boundaries(riemann, n):
+    """Shock, mixed boundaries, all five primitive bars; (16, 8, 8) on the
+    unit cube has a power-of-two geometry, whose line passes replay the
+    slope quotients as products."""
     bc = (("REFLECTING", "OUTFLOW"), ("OUTFLOW", "REFLECTING"), ("PERIODIC", "PERIODIC"))
-    chart = Chart((11, 9, 7), bc=bc)
+    chart = Chart(n, bc=bc)
     rng = np.random.default_rng(12)
     shape = chart.n[::-1]
     x = chart.center[0][2:-2].reshape(1, 1, -1)
@@ -61,6 +65,7 @@ def test_general_functional_mixed_boundaries(riemann):
     ubar = rng.standard_normal((5,) + shape)
     n_steps = 5
     st = Step_RK2_C_ASC(chart, FluidParameters(CourantFactor=0.8, RiemannSolverType=riemann))
+    assert st.geometry_pow2() == (n == (16, 8, 8))
     st.set_primitive(*prim)
     adj = DiscreteAdjoint(st)
     adj.forward(n_steps, 2)
