@@ -75,17 +75,21 @@ def _read_traffic():
     return None
 
 
-def _read_stage2_flop():
-    """FP64 FLOP per stage-2 launch at 256^3 from the committed rocprofv3 SQ
-    counter summary (64 x SQ_INSTS_VALU_FLOPS_FP64)."""
+def _read_sq(key):
+    """A per-launch figure of the stage-2 kernel at 256^3 from the committed
+    rocprofv3 SQ counter summary: FP64 FLOP (64 x SQ_INSTS_VALU_FLOPS_FP64)
+    or wave VALU instructions (SQ_INSTS_VALU)."""
     try:
         with open(SQ_FILE) as f:
             for line in f:
-                if line.startswith("stage2_fp64_flop_per_launch"):
+                if line.startswith(key):
                     return float(line.split("=")[1])
     except (OSError, ValueError):
         pass
     return None
+
+
+SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs, peak engine clock
 
 
 def host_cpu():
@@ -604,7 +608,9 @@ def main():
     achieved = alg_stage2 * chart.n_proper / s2_avg / 1e9
     step_kernel_s = sum(v[0] for v in kt.values()) / max(s2_n, 1) * 1e-3
     traffic = _read_traffic() if args.precision == "FP64" and n == 256 else None
-    flop = _read_stage2_flop() if args.precision == "FP64" and n == 256 else None
+    at256 = args.precision == "FP64" and n == 256
+    flop = _read_sq("stage2_fp64_flop_per_launch") if at256 else None
+    insts = _read_sq("stage2_valu_wave_insts_per_launch") if at256 else None
     valu = None
     if flop is not None:
         tf = flop / s2_avg / 1e12
@@ -613,6 +619,16 @@ def main():
                 "flop_per_launch": flop,
                 "source": "FP64 FLOP per stage-2 launch from the committed SQ counter "
                           "summary, over this run's average stage-2 launch time"}
+    issue = None
+    if insts is not None:
+        # a wave64 VALU instruction holds a 16-lane SIMD four cycles
+        frac = insts * 4 / (SIMDS * CLOCK_HZ * s2_avg)
+        issue = {"bound": "VALU issue", "achieved": frac, "peak": 1.0,
+                 "unit": "fraction of the SIMDs' issue cycles", "frac": frac,
+                 "wave_insts_per_launch": insts,
+                 "source": "stage-2 wave VALU instructions per launch from the committed "
+                           "SQ counter summary, x 4 cycles, over 1024 SIMDs x 2.4 GHz x "
+                           "this run's average stage-2 launch time"}
     stepper.close()
     print(f"# main run: {elapsed / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
 
@@ -682,6 +698,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},
         "valu_roofline": valu,
+        "valu_issue": issue,
         "hbm_measured": None if traffic is None else {
             "bytes_per_launch": traffic,
             "achieved_gbs": traffic / s2_avg / 1e9,
